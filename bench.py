@@ -1,0 +1,160 @@
+"""Throughput of the polyenergetic cone-beam forward projection on one MI355X.
+
+One step = one full scan of the 3-material 512^3 configuration: 360 views of
+a 1024x768 detector, K = 1082 samples per ray (306.33 G samples), every
+material marched once per ray and the 101-bin spectrum (grid + CsI response)
+applied per ray; outputs I and IAbsorb stay in HBM.  Inputs are synthetic
+(seeded phantom, Kramers spectrum, model attenuation tables) and resident in
+HBM before timing starts.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gsamples/s & views/s, 3-mat 512³ poly+grid; mixed vs FP32 at ≤1e-3 rel err"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(setup, vols, kw, target_s: float):
+    """The oracle's FP32 mirror (march + spectrum), OpenMP on this host, on a
+    bounded number of views of the same workload."""
+    import numpy as np
+    from oracle import oracle as O
+    g = O.geom_from_setup(setup)
+    p = O.paras(g)
+    threads = O.num_threads()
+    det = np.stack([O.det_response(1.0, float(m), kw["scint_thickness"]) for m in kw["scint_mu"]])
+
+    def run(v0, nv):
+        t = time.perf_counter()
+        L = O.march_f32(g, p, vols, v0, nv, threads=threads)
+        O.spectrum_f32(L, kw["atten"], kw["spectrum"], kw["i0_val"], kw["grid"], det, threads=threads)
+        return time.perf_counter() - t
+
+    t1 = run(0, 1)
+    n = int(max(1, min(16, round(target_s / max(t1, 1e-3)) - 1)))
+    t2 = run(1, n) if n > 0 else 0.0
+    views = 1 + n
+    secs = t1 + t2
+    spv = setup.det[0] * setup.det[1] * p.n_steps
+    return {"value": round(views * spv / secs / 1e9, 4), "unit": "Gsamples/s", "cores": threads,
+            "kind": "port",
+            "sample": f"views 0-{views - 1} of {setup.views} ({views * spv / 1e9:.2f} G samples), "
+                      f"march + spectrum, {secs:.1f} s on {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--views", type=int, default=0, help="views per step (0 = the full 360)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also launch the copy kernel over this many MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "views shard with no exchange, see the sharding design notes", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import ct_forward_projection_amd as P
+    from ct_forward_projection_amd import synthetic as S
+
+    setup = S.config2()
+    t = time.perf_counter()
+    vols = list(S.three_material_phantom(512))
+    kw = P.spectral_inputs(setup, *setup.det)
+    log(f"inputs ready in {time.perf_counter() - t:.1f} s")
+    proj = P.Projector(P.scan_params(**setup.scan_kwargs()))
+    proj.set_volumes(vols)
+    proj.set_spectral(**kw)
+    views = setup.views if args.views <= 0 else min(args.views, setup.views)
+    nu, nv = setup.det
+    dev = torch.device("cuda", 0)
+    I = torch.empty((views, nv, nu), dtype=torch.float32, device=dev)
+    IA = torch.empty_like(I)
+    stream = torch.cuda.current_stream()
+    samples = views * proj.samples_per_view
+    n_mat = len(vols)
+    bytes_per_sample = 4 * n_mat  # one FP32 voxel-equivalent per material per sample
+
+    if args.calibrate_copy_mib > 0:
+        nb = args.calibrate_copy_mib << 20
+        a = torch.empty(nb // 4, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        from ct_forward_projection_amd import _lib as L
+        L.check(L.lib().ctfp_bench_copy(a.data_ptr(), b.data_ptr(), nb, stream.cuda_stream))
+        torch.cuda.synchronize()
+        log(f"copy kernel: {nb} bytes read, {nb} bytes written")
+
+    for i in range(args.warmup):
+        proj.forward(0, views, I, IA, stream=stream)
+        torch.cuda.synchronize()
+        log(f"warmup {i + 1}/{args.warmup} done")
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        proj.forward(0, views, I, IA, stream=stream)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    log(f"timed {args.steps} steps: wall {wall:.3f} s, per-launch ms {kern_ms}")
+    ms_per_step = wall * 1e3 / args.steps
+    gsps = samples / (ms_per_step / 1e3) / 1e9
+    avg_launch_s = sum(kern_ms) / len(kern_ms) / 1e3
+    achieved = samples * bytes_per_sample / avg_launch_s / 1e9
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "march_config2_traffic.json")
+    if os.path.exists(tfile) and views == setup.views:
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    result = {
+        "metric": METRIC, "value": round(gsps, 3), "unit": "Gsamples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "3-material 512^3 FP32 (water/bone/iodine, 0.5 mm), 1024x768 "
+                               "@ 0.4 mm, 360 views, K=1082, 120 kVp 101 bins, Pb grid 10:1 "
+                               "40 l/cm, CsI response",
+                   "views_per_step": views, "samples_per_step": samples,
+                   "materials": n_mat, "volume_dtype": "f32"},
+        "views_per_s": round(views / (ms_per_step / 1e3), 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic},
+    }
+    del I, IA
+    proj.close()
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        result["cpu_baseline"] = cpu_baseline(setup, vols, kw, args.cpu_seconds)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,932 @@
+// ctfp_device.hip -- gfx950 kernels and the device half of the C ABI (include/ctfp.h).
+//
+// Data layout in HBM
+//   volume : the n_mat density channels interleaved per voxel ([z][y][x][m]),
+//            padded with one zero voxel below and two above on every axis so the
+//            8-corner gather of any clamped index in [-1, N] stays in bounds;
+//            read with raw buffer loads (one VGPR byte offset per sample, the
+//            three other corner rows as uniform SGPR offsets).
+//   weights: W[e][v][u] = I0Val * S_e * T_e(u) * eta_e(v,u) for the active
+//            (non-zero) bins; the v stride is 0 when eta does not vary per pixel.
+//   outputs: I, IAbsorb [view][v][u]; optional L [view][m][v][u].
+//
+// The march evaluates the reference's sample lattice t_k = SOD - R + dx/2 + k dx
+// (CBCTPolyProjGPU.cu:1846-1848) with zero-padded trilinear interpolation
+// (CBCTPolyProjGPU.cu:1920, 2200-2205).  Per-ray geometry is set up in FP64:
+// the sample index position is affine in k, so each ray carries its FP64-derived
+// start and step (FP32) and its exact support interval [k_lo, k_hi) where any
+// channel can be non-zero; samples outside it are exact zeros in the reference
+// and are skipped.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctfp.h"
+#include "ctfp_internal.h"
+
+// ---------------------------------------------------------------- errors ---
+
+namespace ctfp {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+const char *last_error() { return g_last_error.c_str(); }
+void clear_error() { g_last_error.clear(); }
+
+} // namespace ctfp
+
+using ctfp::set_error;
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return set_error(e_ == hipErrorOutOfMemory ? CTFP_ERR_NOMEM : CTFP_ERR_DEVICE,  \
+                             "%s failed: %s", #expr, hipGetErrorString(e_));                \
+    } while (0)
+
+// --------------------------------------------------------------- context ---
+
+struct ctfp_context {
+    int device = 0;
+    ctfp_scan_params scan{};
+    ctfp_system_info sys{};
+    // volume
+    int n_mat = 0;
+    int dtype = CTFP_F32;
+    void *vol = nullptr;
+    size_t vol_bytes = 0;
+    int64_t px = 0, py = 0, pz = 0; // padded extents
+    // spectral
+    int n_act = 0;
+    int spec_n_mat = 0;  // materials the spectral tables were built for
+    float *A = nullptr;  // [n_act][n_mat]
+    float *W = nullptr;  // [n_act][nv or 1][nu]
+    int64_t w_stride_e = 0;
+    int32_t w_stride_v = 0;
+    float *I0 = nullptr; // [nv][nu]
+    // per-view rotation (cos, sin) of the reference's FP32 angle
+    double2 *view_cs = nullptr;
+    // host-mode output staging
+    float *dI = nullptr, *dIA = nullptr;
+    int64_t staged_views = 0;
+};
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+template <typename T> void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// ------------------------------------------------------------- kernels ---
+
+constexpr int kVoxPadLo = 1; // zero voxels below index 0 on each axis
+constexpr int kVoxPadHi = 2; // zero voxels above index N-1 on each axis
+
+// Scatter n_mat FP32 volumes [m][z][y][x] into the padded interleaved layout.
+__global__ void pad_interleave_f32(const float *__restrict__ src, float *__restrict__ dst,
+                                   int n_mat, int64_t nx, int64_t ny, int64_t nz, int64_t px,
+                                   int64_t py, int64_t pz)
+{
+    const int64_t total = px * py * pz;
+    const int64_t nvox = nx * ny * nz;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t x = i % px - kVoxPadLo;
+        int64_t y = (i / px) % py - kVoxPadLo;
+        int64_t z = i / (px * py) - kVoxPadLo;
+        bool in = x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz;
+        int64_t s = (z * ny + y) * nx + x;
+        for (int m = 0; m < n_mat; m++) dst[i * n_mat + m] = in ? src[m * nvox + s] : 0.0f;
+    }
+}
+
+// Same from an array of per-material device pointers.
+struct VolPtrs {
+    const float *p[4];
+};
+__global__ void pad_interleave_ptrs_f32(VolPtrs src, float *__restrict__ dst, int n_mat,
+                                        int64_t nx, int64_t ny, int64_t nz, int64_t px,
+                                        int64_t py, int64_t pz)
+{
+    const int64_t total = px * py * pz;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t x = i % px - kVoxPadLo;
+        int64_t y = (i / px) % py - kVoxPadLo;
+        int64_t z = i / (px * py) - kVoxPadLo;
+        bool in = x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz;
+        int64_t s = (z * ny + y) * nx + x;
+        for (int m = 0; m < n_mat; m++) dst[i * n_mat + m] = in ? src.p[m][s] : 0.0f;
+    }
+}
+
+// Density channels straight from the label volume into the padded layout.
+struct Densities {
+    float d[4];
+};
+__global__ void labels_to_channels(const uint8_t *__restrict__ labels, float *__restrict__ dst,
+                                   int n_mat, int single_material, Densities rho, int64_t nx,
+                                   int64_t ny, int64_t nz, int64_t px, int64_t py, int64_t pz)
+{
+    const int64_t total = px * py * pz;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t x = i % px - kVoxPadLo;
+        int64_t y = (i / px) % py - kVoxPadLo;
+        int64_t z = i / (px * py) - kVoxPadLo;
+        bool in = x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz;
+        unsigned lab = in ? labels[(z * ny + y) * nx + x] : 0u;
+        if (single_material) {
+            dst[i] = (float)lab * rho.d[0];
+        } else {
+            for (int m = 0; m < n_mat; m++) dst[i * n_mat + m] = (lab == (unsigned)(m + 1)) ? rho.d[m] : 0.0f;
+        }
+    }
+}
+
+// W[e][v][u] = ((I0Val * S_e) * T_e(u)) * eta_e(v,u) for active bins; the same
+// product order as the reference's addI0 (CBCTPolyForwardProjGrid.cpp:336).
+struct WeightArgs {
+    const float *S;     // [n_act] spectrum of active bins (already energy-weighted)
+    const float *T;     // [n_act][nu] or null
+    const float *eta;   // [n_act][npix] or null
+    const float *mu_c;  // [n_act] or null (scintillator model)
+    const float *thick; // [npix]
+    float factor;
+    float i0;
+    int n_act;
+    int64_t nu, rows; // rows = nv when per pixel, else 1
+};
+__global__ void build_weights(WeightArgs a, float *__restrict__ W)
+{
+    const int64_t per = a.rows * a.nu;
+    const int64_t total = per * a.n_act;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int e = (int)(i / per);
+        int64_t p = i % per;
+        int64_t u = p % a.nu;
+        float w = a.i0 * a.S[e];
+        if (a.T) w = w * a.T[(int64_t)e * a.nu + u];
+        if (a.eta) w = w * a.eta[(int64_t)e * per + p];
+        else if (a.mu_c) w = w * (a.factor * (1.0f - expf(-a.mu_c[e] * a.thick[p])));
+        W[i] = w;
+    }
+}
+
+// I0[v][u] = sum_e W[e][v][u] in bin order (FP32).
+__global__ void flat_field_kernel(const float *__restrict__ W, int n_act, int64_t stride_e,
+                                  int stride_v, int64_t nu, int64_t nv, float *__restrict__ I0)
+{
+    const int64_t npix = nu * nv;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        int64_t u = p % nu, v = p / nu;
+        float acc = 0.0f;
+        for (int e = 0; e < n_act; e++) acc += W[e * stride_e + v * stride_v + u];
+        I0[p] = acc;
+    }
+}
+
+__global__ void postprocess_kernel(int64_t n_views, int64_t npix, const float *__restrict__ I,
+                                   const float *__restrict__ IA, const float *__restrict__ I0,
+                                   uint32_t *__restrict__ Iu, uint32_t *__restrict__ IAu,
+                                   float *__restrict__ proj)
+{
+    const int64_t total = n_views * npix;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        float i = I[q];
+        uint32_t iu = (uint32_t)floorf(i);
+        if (Iu) Iu[q] = iu;
+        if (IAu) IAu[q] = (uint32_t)floorf(IA[q]);
+        if (proj) proj[q] = logf(I0[q % npix] / (float)iu);
+    }
+}
+
+__global__ void copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// ------------------------------------------------------------ the march ---
+
+struct MarchArgs {
+    // volume (padded, interleaved, bytes)
+    const void *vol;
+    uint32_t vol_bytes;
+    uint32_t row_y, row_z; // byte strides of one y row / one z slice
+    float nx, ny, nz;      // clamp limits (= N per axis)
+    // geometry (FP64, reference scalars)
+    double sod, sdd, dsize, half_u, half_v, t0, dx;
+    double half_x, half_y, half_z, inv_px, inv_py, inv_pz;
+    int32_t nu, nv, K;
+    const double2 *view_cs;
+    int64_t view_begin;
+    int32_t n_views;
+    int32_t tiles_u, tiles_v;
+    uint32_t n_blocks; // logical blocks (n_views * tiles_u * tiles_v)
+    uint32_t per_xcd;  // launched blocks / 8
+    float dx_f;
+    // spectrum
+    int32_t n_act;
+    const float *A; // [n_act][M]
+    const float *W;
+    int64_t w_stride_e;
+    int32_t w_stride_v;
+    // outputs
+    float *I, *IA, *L;
+};
+
+constexpr int kWaveU = 8, kWaveV = 8;  // pixels per wavefront
+constexpr int kTileU = 16, kTileV = 16; // pixels per workgroup (4 wavefronts)
+constexpr int kBlock = 256;
+
+template <int M> struct VoxTraits;
+template <> struct VoxTraits<1> { static constexpr int bytes = 4; };
+template <> struct VoxTraits<2> { static constexpr int bytes = 8; };
+template <> struct VoxTraits<3> { static constexpr int bytes = 12; };
+template <> struct VoxTraits<4> { static constexpr int bytes = 16; };
+
+template <int M>
+__device__ __forceinline__ void load_vox(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
+                                         float (&c)[M])
+{
+    if constexpr (M == 1) {
+        c[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    } else if constexpr (M == 2) {
+        auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        c[0] = __uint_as_float(v[0]);
+        c[1] = __uint_as_float(v[1]);
+    } else if constexpr (M == 3) {
+        auto v = __builtin_amdgcn_raw_buffer_load_b96(r, voff, soff, 0);
+        c[0] = __uint_as_float(v[0]);
+        c[1] = __uint_as_float(v[1]);
+        c[2] = __uint_as_float(v[2]);
+    } else {
+        auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+        c[0] = __uint_as_float(v[0]);
+        c[1] = __uint_as_float(v[1]);
+        c[2] = __uint_as_float(v[2]);
+        c[3] = __uint_as_float(v[3]);
+    }
+}
+
+// Support interval of index s + k*d inside (-1, n): intersect into [lo, hi].
+__device__ __forceinline__ void clip_axis(double s, double d, double n, double &lo, double &hi)
+{
+    if (fabs(d) < 1e-300) {
+        if (!(s > -1.0 && s < n)) { lo = 1.0; hi = 0.0; }
+        return;
+    }
+    double k1 = (-1.0 - s) / d, k2 = (n - s) / d;
+    lo = fmax(lo, fmin(k1, k2));
+    hi = fmin(hi, fmax(k1, k2));
+}
+
+// Ray (u, v) of view `vw`: FP64 set-up of the index-space start (at k_lo) and
+// step, and the sample range [k_lo, k_hi).
+struct RaySetup {
+    float a0, b0, c0, da, db, dc;
+    int k_lo, n;
+};
+
+__device__ __forceinline__ RaySetup setup_ray(const MarchArgs &P, int u, int v, int vw)
+{
+    // computeIntPointCoordinatesKernel (CBCTPolyProjGPU.cu:1813-1829), FP64.
+    double detU = P.half_u - P.dsize * 0.5 - (double)u * P.dsize;
+    double detV = P.half_v - P.dsize * 0.5 - (double)v * P.dsize;
+    double sdd2 = P.sdd * P.sdd;
+    double r = sqrt(sdd2 + detU * detU + detV * detV);
+    double inv_r = 1.0 / r;
+    double sg = detV * inv_r;
+    double cg = sqrt(sdd2 + detU * detU) * inv_r;
+    double sb = detU * inv_r;
+    double cb = sqrt(sdd2 + detV * detV) * inv_r;
+    double dX = cg * cb, dY = cg * sb, dZ = sg;
+    // transformKernel (CBCTPolyProjGPU.cu:1915-1917) is affine in t.
+    double2 cs = P.view_cs[vw];
+    double ca = (dX * cs.x - dY * cs.y) * P.inv_px;         // d(a)/dt
+    double cb_ = -(dX * cs.y + dY * cs.x) * P.inv_py;       // d(b)/dt
+    double cc = -dZ * P.inv_pz;                             // d(c)/dt
+    double a_off = (P.half_x - P.sod * cs.x) * P.inv_px;    // a at t = 0
+    double b_off = (P.half_y + P.sod * cs.y) * P.inv_py;
+    double c_off = P.half_z * P.inv_pz;
+    double sa = a_off + ca * P.t0, sb2 = b_off + cb_ * P.t0, sc = c_off + cc * P.t0; // k = 0
+    double da = ca * P.dx, db = cb_ * P.dx, dc = cc * P.dx;
+    double lo = 0.0, hi = (double)P.K - 1.0;
+    clip_axis(sa, da, (double)P.nx, lo, hi);
+    clip_axis(sb2, db, (double)P.ny, lo, hi);
+    clip_axis(sc, dc, (double)P.nz, lo, hi);
+    RaySetup R;
+    int k_lo = 0, k_hi = 0;
+    if (hi >= lo) {
+        // widen by one sample each side; extra samples evaluate to exact zeros
+        k_lo = max(0, (int)floor(lo) - 1);
+        k_hi = min(P.K, (int)ceil(hi) + 2);
+    }
+    R.k_lo = k_lo;
+    R.n = max(0, k_hi - k_lo);
+    R.a0 = (float)(sa + da * k_lo);
+    R.b0 = (float)(sb2 + db * k_lo);
+    R.c0 = (float)(sc + dc * k_lo);
+    R.da = (float)da;
+    R.db = (float)db;
+    R.dc = (float)dc;
+    return R;
+}
+
+template <int M, bool kSpectrum>
+__global__ __launch_bounds__(kBlock) void march_kernel(MarchArgs P)
+{
+    // XCD-aware block order: consecutive logical blocks (adjacent tiles of one
+    // view) are kept on one XCD so they share its L2.
+    const uint32_t bid = blockIdx.x;
+    const uint32_t logical = (bid & 7u) * P.per_xcd + (bid >> 3);
+    if (logical >= P.n_blocks) return;
+    const uint32_t tiles = (uint32_t)(P.tiles_u * P.tiles_v);
+    const int vw_local = (int)(logical / tiles);
+    const uint32_t t = logical % tiles;
+    const int tile_v = (int)(t / P.tiles_u), tile_u = (int)(t % P.tiles_u);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int u = tile_u * kTileU + (wave & 1) * kWaveU + (lane % kWaveU);
+    const int v = tile_v * kTileV + (wave >> 1) * kWaveV + (lane / kWaveU);
+    const bool active = (u < P.nu) && (v < P.nv);
+    const int vw = (int)(P.view_begin + vw_local);
+
+    RaySetup R;
+    if (active) {
+        R = setup_ray(P, u, v, vw);
+    } else {
+        R.a0 = R.b0 = R.c0 = R.da = R.db = R.dc = 0.0f;
+        R.k_lo = 0;
+        R.n = 0;
+    }
+
+    __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(P.vol), 0, (int)P.vol_bytes, 0x00020000);
+    constexpr uint32_t VB = VoxTraits<M>::bytes;
+    const uint32_t oy = P.row_y, oz = P.row_z, oyz = P.row_y + P.row_z;
+    const uint32_t ry = P.row_y / VB; // voxels per row
+    const uint32_t rz = P.row_z / P.row_y; // rows per slice
+
+    float acc[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) acc[m] = 0.0f;
+
+    float jf = 0.0f;
+    for (int j = 0; j < R.n; j++, jf += 1.0f) {
+        float fa = fminf(fmaxf(fmaf(jf, R.da, R.a0), -1.0f), P.nx);
+        float fb = fminf(fmaxf(fmaf(jf, R.db, R.b0), -1.0f), P.ny);
+        float fc = fminf(fmaxf(fmaf(jf, R.dc, R.c0), -1.0f), P.nz);
+        float ia = floorf(fa), ib = floorf(fb), ic = floorf(fc);
+        float ax = fa - ia, ay = fb - ib, az = fc - ic;
+        uint32_t xi = (uint32_t)((int)ia + kVoxPadLo);
+        uint32_t yi = (uint32_t)((int)ib + kVoxPadLo);
+        uint32_t zi = (uint32_t)((int)ic + kVoxPadLo);
+        // 24-bit multiplies (full rate); every factor is < 2^24 and the byte
+        // offset < 2^32 (checked when the volume is allocated).
+        uint32_t voff = __umul24(__umul24(__umul24(zi, rz) + yi, ry) + xi, VB);
+        float c000[M], c100[M], c010[M], c110[M], c001[M], c101[M], c011[M], c111[M];
+        load_vox<M>(rsrc, voff, 0, c000);
+        load_vox<M>(rsrc, voff + VB, 0, c100);
+        load_vox<M>(rsrc, voff, oy, c010);
+        load_vox<M>(rsrc, voff + VB, oy, c110);
+        load_vox<M>(rsrc, voff, oz, c001);
+        load_vox<M>(rsrc, voff + VB, oz, c101);
+        load_vox<M>(rsrc, voff, oyz, c011);
+        load_vox<M>(rsrc, voff + VB, oyz, c111);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            float e00 = fmaf(ax, c100[m] - c000[m], c000[m]);
+            float e10 = fmaf(ax, c110[m] - c010[m], c010[m]);
+            float e01 = fmaf(ax, c101[m] - c001[m], c001[m]);
+            float e11 = fmaf(ax, c111[m] - c011[m], c011[m]);
+            float e0 = fmaf(ay, e10 - e00, e00);
+            float e1 = fmaf(ay, e11 - e01, e01);
+            acc[m] += fmaf(az, e1 - e0, e0);
+        }
+    }
+    if (!active) return;
+
+    const int64_t npix = (int64_t)P.nu * P.nv;
+    const int64_t pix = (int64_t)v * P.nu + u;
+    float Lm[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) Lm[m] = acc[m] * P.dx_f;
+    if (P.L) {
+#pragma unroll
+        for (int m = 0; m < M; m++) P.L[((int64_t)vw_local * M + m) * npix + pix] = Lm[m];
+    }
+    if constexpr (kSpectrum) {
+        // Per-ray spectrum: tau_e = sum_m A[e][m] L_m, then the energy-ordered
+        // FP32 sums of W e^-tau and W (1 - e^-tau).
+        const float *wp = P.W + (int64_t)v * P.w_stride_v + u;
+        float sI = 0.0f, sA = 0.0f;
+        for (int e = 0; e < P.n_act; e++) {
+            float tau = 0.0f;
+#pragma unroll
+            for (int m = 0; m < M; m++) tau = fmaf(P.A[e * M + m], Lm[m], tau);
+            float ex = expf(-tau);
+            float w = wp[(int64_t)e * P.w_stride_e];
+            sI = fmaf(w, ex, sI);
+            sA = fmaf(w, 1.0f - ex, sA);
+        }
+        P.I[(int64_t)vw_local * npix + pix] = sI;
+        P.IA[(int64_t)vw_local * npix + pix] = sA;
+    }
+}
+
+using MarchFn = void (*)(MarchArgs);
+
+MarchFn pick_march(int n_mat, bool spectrum)
+{
+    switch (n_mat) {
+    case 1: return spectrum ? march_kernel<1, true> : march_kernel<1, false>;
+    case 2: return spectrum ? march_kernel<2, true> : march_kernel<2, false>;
+    case 3: return spectrum ? march_kernel<3, true> : march_kernel<3, false>;
+    case 4: return spectrum ? march_kernel<4, true> : march_kernel<4, false>;
+    default: return nullptr;
+    }
+}
+
+int grid_for(int64_t n, int block = 256)
+{
+    int64_t g = (n + block - 1) / block;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 65536));
+}
+
+int check_ctx(const ctfp_context *ctx)
+{
+    if (!ctx) return set_error(CTFP_ERR_INVALID, "null context");
+    return CTFP_OK;
+}
+
+int alloc_volume(ctfp_context *ctx, int n_mat)
+{
+    if (n_mat < 1 || n_mat > 4)
+        return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d: 1..4 materials are supported", n_mat);
+    const ctfp_scan_params &s = ctx->scan;
+    int64_t px = s.p_num_x + kVoxPadLo + kVoxPadHi;
+    int64_t py = s.p_num_y + kVoxPadLo + kVoxPadHi;
+    int64_t pz = s.p_num_z + kVoxPadLo + kVoxPadHi;
+    size_t bytes = (size_t)(px * py * pz) * (size_t)n_mat * sizeof(float);
+    if (bytes >= 0xFFFFFFFFull)
+        return set_error(CTFP_ERR_UNSUPPORTED,
+                         "padded volume of %zu bytes exceeds the 4 GiB buffer-load range", bytes);
+    if (ctx->vol && (ctx->vol_bytes != bytes)) dfree(ctx->vol);
+    if (!ctx->vol) HIP_TRY(hipMalloc(&ctx->vol, bytes));
+    ctx->vol_bytes = bytes;
+    ctx->n_mat = n_mat;
+    ctx->px = px;
+    ctx->py = py;
+    ctx->pz = pz;
+    return CTFP_OK;
+}
+
+int fill_march_args(ctfp_context *ctx, int64_t view_begin, int64_t view_count, MarchArgs &P)
+{
+    const ctfp_scan_params &s = ctx->scan;
+    const ctfp_system_info &y = ctx->sys;
+    if (!ctx->vol) return set_error(CTFP_ERR_STATE, "no volumes uploaded");
+    if (view_begin < 0 || view_count < 1 || view_begin + view_count > s.proj_num)
+        return set_error(CTFP_ERR_INVALID, "view range [%lld, %lld) outside [0, %lld)",
+                         (long long)view_begin, (long long)(view_begin + view_count),
+                         (long long)s.proj_num);
+    if (y.int_num < 1 || y.int_num > (1 << 30))
+        return set_error(CTFP_ERR_INVALID, "samples per ray %lld out of range", (long long)y.int_num);
+    std::memset(&P, 0, sizeof P);
+    const uint32_t VB = (uint32_t)(ctx->n_mat * sizeof(float));
+    P.vol = ctx->vol;
+    P.vol_bytes = (uint32_t)ctx->vol_bytes;
+    P.row_y = (uint32_t)(ctx->px * VB);
+    P.row_z = (uint32_t)(ctx->px * ctx->py * VB);
+    P.nx = (float)s.p_num_x;
+    P.ny = (float)s.p_num_y;
+    P.nz = (float)s.p_num_z;
+    P.sod = s.sod;
+    P.sdd = s.sdd;
+    P.dsize = s.d_size;
+    P.half_u = y.d_half_lu;
+    P.half_v = y.d_half_lv;
+    P.t0 = (double)s.sod - (double)y.fov_r + (double)y.dx / 2.0;
+    P.dx = y.dx;
+    P.half_x = y.p_half_x;
+    P.half_y = y.p_half_y;
+    P.half_z = y.p_half_z;
+    P.inv_px = 1.0 / (double)y.p_size_x;
+    P.inv_py = 1.0 / (double)y.p_size_y;
+    P.inv_pz = 1.0 / (double)y.p_size_z;
+    P.nu = (int32_t)s.d_num_u;
+    P.nv = (int32_t)s.d_num_v;
+    P.K = (int32_t)y.int_num;
+    P.view_cs = ctx->view_cs;
+    P.view_begin = view_begin;
+    P.n_views = (int32_t)view_count;
+    P.tiles_u = (int32_t)((s.d_num_u + kTileU - 1) / kTileU);
+    P.tiles_v = (int32_t)((s.d_num_v + kTileV - 1) / kTileV);
+    uint64_t nb = (uint64_t)view_count * P.tiles_u * P.tiles_v;
+    if (nb * 1ull >= (1ull << 31))
+        return set_error(CTFP_ERR_INVALID, "too many tiles in one launch (%llu)", (unsigned long long)nb);
+    P.n_blocks = (uint32_t)nb;
+    P.per_xcd = (uint32_t)((nb + 7) / 8);
+    P.dx_f = y.dx;
+    return CTFP_OK;
+}
+
+} // namespace
+
+// ------------------------------------------------------------- C ABI ---
+
+extern "C" {
+
+const char *ctfp_last_error(void) { return ctfp::last_error(); }
+const char *ctfp_version(void) { return "ctfp 0.1.0 (gfx950)"; }
+
+int ctfp_create(const ctfp_scan_params *scan, int device, ctfp_context **out)
+{
+    if (!scan || !out) return set_error(CTFP_ERR_INVALID, "null argument");
+    *out = nullptr;
+    ctfp_system_info sys;
+    int rc = ctfp_compute_paras(scan, &sys);
+    if (rc != CTFP_OK) return rc;
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return set_error(CTFP_ERR_INVALID, "device %d not present (%d devices)", device, ndev);
+    DeviceGuard g(device);
+    ctfp_context *ctx = new (std::nothrow) ctfp_context();
+    if (!ctx) return set_error(CTFP_ERR_NOMEM, "context allocation failed");
+    ctx->device = device;
+    ctx->scan = *scan;
+    ctx->sys = sys;
+    // Rotation of view i: the reference's FP32 angle (CBCTPolyProjGPU.cu:51),
+    // cos/sin evaluated in FP64.
+    std::vector<double2> cs((size_t)scan->proj_num);
+    for (int64_t i = 0; i < scan->proj_num; i++) {
+        double th = (double)ctfp::view_angle(scan, &sys, i);
+        cs[(size_t)i] = make_double2(cos(th), sin(th));
+    }
+    hipError_t e = hipMalloc(&ctx->view_cs, cs.size() * sizeof(double2));
+    if (e == hipSuccess)
+        e = hipMemcpy(ctx->view_cs, cs.data(), cs.size() * sizeof(double2), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        ctfp_destroy(ctx);
+        return set_error(CTFP_ERR_DEVICE, "view table: %s", hipGetErrorString(e));
+    }
+    *out = ctx;
+    return CTFP_OK;
+}
+
+int ctfp_destroy(ctfp_context *ctx)
+{
+    if (!ctx) return CTFP_OK;
+    DeviceGuard g(ctx->device);
+    (void)hipDeviceSynchronize();
+    dfree(ctx->vol);
+    dfree(ctx->A);
+    dfree(ctx->W);
+    dfree(ctx->I0);
+    dfree(ctx->view_cs);
+    dfree(ctx->dI);
+    dfree(ctx->dIA);
+    delete ctx;
+    return CTFP_OK;
+}
+
+int ctfp_get_system_info(const ctfp_context *ctx, ctfp_system_info *out)
+{
+    if (check_ctx(ctx) || !out) return set_error(CTFP_ERR_INVALID, "null argument");
+    *out = ctx->sys;
+    return CTFP_OK;
+}
+
+int ctfp_set_volumes(ctfp_context *ctx, int n_mat, const float *const *volumes,
+                     int volumes_on_device, int storage_dtype)
+{
+    if (check_ctx(ctx)) return CTFP_ERR_INVALID;
+    if (!volumes) return set_error(CTFP_ERR_INVALID, "null volume list");
+    if (storage_dtype != CTFP_F32)
+        return set_error(CTFP_ERR_UNSUPPORTED, "storage dtype %d not supported yet", storage_dtype);
+    for (int m = 0; m < n_mat; m++)
+        if (!volumes[m]) return set_error(CTFP_ERR_INVALID, "volume %d is null", m);
+    DeviceGuard g(ctx->device);
+    int rc = alloc_volume(ctx, n_mat);
+    if (rc) return rc;
+    const ctfp_scan_params &s = ctx->scan;
+    const int64_t nvox = s.p_num_x * s.p_num_y * s.p_num_z;
+    const int64_t total = ctx->px * ctx->py * ctx->pz;
+    if (volumes_on_device) {
+        VolPtrs vp{};
+        for (int m = 0; m < n_mat; m++) vp.p[m] = volumes[m];
+        pad_interleave_ptrs_f32<<<grid_for(total), 256>>>(vp, (float *)ctx->vol, n_mat, s.p_num_x,
+                                                         s.p_num_y, s.p_num_z, ctx->px, ctx->py,
+                                                         ctx->pz);
+        HIP_TRY(hipGetLastError());
+    } else {
+        float *stage = nullptr;
+        HIP_TRY(hipMalloc(&stage, (size_t)nvox * n_mat * sizeof(float)));
+        hipError_t e = hipSuccess;
+        for (int m = 0; m < n_mat && e == hipSuccess; m++)
+            e = hipMemcpy(stage + m * nvox, volumes[m], (size_t)nvox * sizeof(float),
+                          hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            pad_interleave_f32<<<grid_for(total), 256>>>(stage, (float *)ctx->vol, n_mat, s.p_num_x,
+                                                        s.p_num_y, s.p_num_z, ctx->px, ctx->py,
+                                                        ctx->pz);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        (void)hipFree(stage);
+        if (e != hipSuccess) return set_error(CTFP_ERR_DEVICE, "volume upload: %s", hipGetErrorString(e));
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    ctx->dtype = storage_dtype;
+    return CTFP_OK;
+}
+
+int ctfp_set_labels(ctfp_context *ctx, const uint8_t *labels, int labels_on_device, int n_mat,
+                    const float *density, int single_material, int storage_dtype)
+{
+    if (check_ctx(ctx)) return CTFP_ERR_INVALID;
+    if (!labels || !density) return set_error(CTFP_ERR_INVALID, "null labels or densities");
+    if (storage_dtype != CTFP_F32)
+        return set_error(CTFP_ERR_UNSUPPORTED, "storage dtype %d not supported yet", storage_dtype);
+    if (n_mat < 1 || n_mat > 4)
+        return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d: 1..4 materials are supported", n_mat);
+    DeviceGuard g(ctx->device);
+    int channels = single_material ? 1 : n_mat;
+    int rc = alloc_volume(ctx, channels);
+    if (rc) return rc;
+    const ctfp_scan_params &s = ctx->scan;
+    const int64_t nvox = s.p_num_x * s.p_num_y * s.p_num_z;
+    const uint8_t *dl = labels;
+    uint8_t *stage = nullptr;
+    if (!labels_on_device) {
+        HIP_TRY(hipMalloc(&stage, (size_t)nvox));
+        hipError_t e = hipMemcpy(stage, labels, (size_t)nvox, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(stage);
+            return set_error(CTFP_ERR_DEVICE, "label upload: %s", hipGetErrorString(e));
+        }
+        dl = stage;
+    }
+    Densities rho{};
+    for (int m = 0; m < (single_material ? 1 : n_mat); m++) rho.d[m] = density[m];
+    const int64_t total = ctx->px * ctx->py * ctx->pz;
+    labels_to_channels<<<grid_for(total), 256>>>(dl, (float *)ctx->vol, channels, single_material,
+                                                 rho, s.p_num_x, s.p_num_y, s.p_num_z, ctx->px,
+                                                 ctx->py, ctx->pz);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (stage) (void)hipFree(stage);
+    if (e != hipSuccess) return set_error(CTFP_ERR_DEVICE, "label channels: %s", hipGetErrorString(e));
+    ctx->dtype = storage_dtype;
+    return CTFP_OK;
+}
+
+int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
+{
+    if (check_ctx(ctx)) return CTFP_ERR_INVALID;
+    if (!sp || !sp->spectrum || !sp->atten)
+        return set_error(CTFP_ERR_INVALID, "spectral: spectrum and atten are required");
+    if (sp->n_energy < 1 || sp->n_energy > 4096)
+        return set_error(CTFP_ERR_INVALID, "spectral: n_energy=%d", sp->n_energy);
+    if (sp->n_mat < 1 || sp->n_mat > 4)
+        return set_error(CTFP_ERR_UNSUPPORTED, "spectral: n_mat=%d", sp->n_mat);
+    if (ctx->vol && sp->n_mat != ctx->n_mat)
+        return set_error(CTFP_ERR_INVALID, "spectral: n_mat=%d but volumes have %d channels",
+                         sp->n_mat, ctx->n_mat);
+    if (sp->scint_mu && !sp->det && !sp->scint_thickness)
+        return set_error(CTFP_ERR_INVALID, "spectral: scint_mu needs scint_thickness");
+    DeviceGuard g(ctx->device);
+    const ctfp_scan_params &s = ctx->scan;
+    const int64_t nu = s.d_num_u, nv = s.d_num_v, npix = nu * nv;
+    // Active bins: a zero spectrum entry contributes exactly zero.
+    std::vector<int> act;
+    for (int e = 0; e < sp->n_energy; e++)
+        if (sp->spectrum[e] != 0.0f) act.push_back(e);
+    const int na = (int)act.size();
+    std::vector<float> S(std::max(na, 1)), A((size_t)std::max(na, 1) * sp->n_mat);
+    for (int i = 0; i < na; i++) {
+        int e = act[i];
+        S[i] = sp->spectrum[e] * (sp->energy_weighting ? (float)(e + 1) : 1.0f);
+        for (int m = 0; m < sp->n_mat; m++) A[(size_t)i * sp->n_mat + m] = sp->atten[(size_t)e * sp->n_mat + m];
+    }
+    const bool per_pixel = sp->det != nullptr || sp->scint_mu != nullptr;
+    const int64_t rows = per_pixel ? nv : 1;
+    dfree(ctx->A);
+    dfree(ctx->W);
+    if (!ctx->I0) HIP_TRY(hipMalloc(&ctx->I0, (size_t)npix * sizeof(float)));
+    HIP_TRY(hipMalloc(&ctx->A, A.size() * sizeof(float)));
+    HIP_TRY(hipMemcpy(ctx->A, A.data(), A.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&ctx->W, (size_t)std::max(na, 1) * rows * nu * sizeof(float)));
+    // stage the per-bin tables of the active bins
+    std::vector<float> T, eta, mu;
+    float *dS = nullptr, *dT = nullptr, *dEta = nullptr, *dMu = nullptr, *dThick = nullptr;
+    auto cleanup = [&]() {
+        dfree(dS); dfree(dT); dfree(dEta); dfree(dMu); dfree(dThick);
+    };
+    hipError_t e = hipMalloc(&dS, S.size() * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(dS, S.data(), S.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess && sp->grid && na > 0) {
+        T.resize((size_t)na * nu);
+        for (int i = 0; i < na; i++)
+            std::memcpy(&T[(size_t)i * nu], sp->grid + (size_t)act[i] * nu, nu * sizeof(float));
+        e = hipMalloc(&dT, T.size() * sizeof(float));
+        if (e == hipSuccess) e = hipMemcpy(dT, T.data(), T.size() * sizeof(float), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && sp->det && na > 0) {
+        e = hipMalloc(&dEta, (size_t)na * npix * sizeof(float));
+        for (int i = 0; i < na && e == hipSuccess; i++)
+            e = hipMemcpy(dEta + (size_t)i * npix, sp->det + (size_t)act[i] * npix,
+                          (size_t)npix * sizeof(float), hipMemcpyHostToDevice);
+    } else if (e == hipSuccess && sp->scint_mu && na > 0) {
+        mu.resize(na);
+        for (int i = 0; i < na; i++) mu[i] = sp->scint_mu[act[i]];
+        e = hipMalloc(&dMu, mu.size() * sizeof(float));
+        if (e == hipSuccess) e = hipMemcpy(dMu, mu.data(), mu.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc(&dThick, (size_t)npix * sizeof(float));
+        if (e == hipSuccess)
+            e = hipMemcpy(dThick, sp->scint_thickness, (size_t)npix * sizeof(float), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && na > 0) {
+        WeightArgs wa{};
+        wa.S = dS; wa.T = dT; wa.eta = dEta; wa.mu_c = dMu; wa.thick = dThick;
+        wa.factor = sp->det_factor;
+        wa.i0 = sp->i0_val;
+        wa.n_act = na;
+        wa.nu = nu;
+        wa.rows = rows;
+        build_weights<<<grid_for((int64_t)na * rows * nu), 256>>>(wa, ctx->W);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        ctx->n_act = na;
+        ctx->spec_n_mat = sp->n_mat;
+        ctx->w_stride_e = rows * nu;
+        ctx->w_stride_v = per_pixel ? (int32_t)nu : 0;
+        flat_field_kernel<<<grid_for(npix), 256>>>(ctx->W, na, ctx->w_stride_e, ctx->w_stride_v, nu,
+                                                   nv, ctx->I0);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    cleanup();
+    if (e != hipSuccess) return set_error(CTFP_ERR_DEVICE, "spectral set-up: %s", hipGetErrorString(e));
+    return CTFP_OK;
+}
+
+static int launch_march(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *I,
+                        float *IA, float *L, void *stream, bool spectrum)
+{
+    if (check_ctx(ctx)) return CTFP_ERR_INVALID;
+    MarchArgs P;
+    int rc = fill_march_args(ctx, view_begin, view_count, P);
+    if (rc) return rc;
+    if (spectrum) {
+        if (!ctx->W || !ctx->A) return set_error(CTFP_ERR_STATE, "spectral weights not set");
+        if (ctx->spec_n_mat != ctx->n_mat)
+            return set_error(CTFP_ERR_STATE, "spectral tables are for %d materials, volumes have %d",
+                             ctx->spec_n_mat, ctx->n_mat);
+        if (!I || !IA) return set_error(CTFP_ERR_INVALID, "null output");
+        P.n_act = ctx->n_act;
+        P.A = ctx->A;
+        P.W = ctx->W;
+        P.w_stride_e = ctx->w_stride_e;
+        P.w_stride_v = ctx->w_stride_v;
+    } else if (!L) {
+        return set_error(CTFP_ERR_INVALID, "null output");
+    }
+    P.I = I;
+    P.IA = IA;
+    P.L = L;
+    MarchFn fn = pick_march(ctx->n_mat, spectrum);
+    if (!fn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->n_mat);
+    DeviceGuard g(ctx->device);
+    dim3 grid(P.per_xcd * 8), block(kBlock);
+    hipLaunchKernelGGL(fn, grid, block, 0, (hipStream_t)stream, P);
+    HIP_TRY(hipGetLastError());
+    return CTFP_OK;
+}
+
+int ctfp_forward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *I,
+                 float *IAbsorb, float *L, void *stream)
+{
+    return launch_march(ctx, view_begin, view_count, I, IAbsorb, L, stream, true);
+}
+
+int ctfp_line_integrals(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *L,
+                        void *stream)
+{
+    return launch_march(ctx, view_begin, view_count, nullptr, nullptr, L, stream, false);
+}
+
+int ctfp_flat_field(ctfp_context *ctx, float *I0, void *stream)
+{
+    if (check_ctx(ctx)) return CTFP_ERR_INVALID;
+    if (!ctx->I0) return set_error(CTFP_ERR_STATE, "spectral weights not set");
+    if (!I0) return set_error(CTFP_ERR_INVALID, "null output");
+    DeviceGuard g(ctx->device);
+    const int64_t npix = ctx->scan.d_num_u * ctx->scan.d_num_v;
+    HIP_TRY(hipMemcpyAsync(I0, ctx->I0, (size_t)npix * sizeof(float), hipMemcpyDeviceToDevice,
+                           (hipStream_t)stream));
+    return CTFP_OK;
+}
+
+int ctfp_forward_host(ctfp_context *ctx, float *I, float *IAbsorb, float *I0)
+{
+    if (check_ctx(ctx)) return CTFP_ERR_INVALID;
+    if (!I || !IAbsorb) return set_error(CTFP_ERR_INVALID, "null output");
+    DeviceGuard g(ctx->device);
+    const int64_t npix = ctx->scan.d_num_u * ctx->scan.d_num_v;
+    // Bounded device staging: views in chunks of at most ~1 GiB of output.
+    int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(ctx->scan.proj_num,
+                                                           (int64_t)(1ll << 27) / std::max<int64_t>(npix, 1)));
+    if (ctx->staged_views < chunk) {
+        dfree(ctx->dI);
+        dfree(ctx->dIA);
+        HIP_TRY(hipMalloc(&ctx->dI, (size_t)chunk * npix * sizeof(float)));
+        HIP_TRY(hipMalloc(&ctx->dIA, (size_t)chunk * npix * sizeof(float)));
+        ctx->staged_views = chunk;
+    }
+    for (int64_t v0 = 0; v0 < ctx->scan.proj_num; v0 += chunk) {
+        int64_t n = std::min(chunk, ctx->scan.proj_num - v0);
+        int rc = ctfp_forward(ctx, v0, n, ctx->dI, ctx->dIA, nullptr, nullptr);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpy(I + v0 * npix, ctx->dI, (size_t)n * npix * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(IAbsorb + v0 * npix, ctx->dIA, (size_t)n * npix * sizeof(float),
+                          hipMemcpyDeviceToHost));
+    }
+    if (I0) {
+        if (!ctx->I0) return set_error(CTFP_ERR_STATE, "spectral weights not set");
+        HIP_TRY(hipMemcpy(I0, ctx->I0, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return CTFP_OK;
+}
+
+int ctfp_postprocess(int64_t n_views, int64_t npix, const float *I, const float *IAbsorb,
+                     const float *I0, uint32_t *I_u32, uint32_t *IAbsorb_u32, float *proj,
+                     void *stream)
+{
+    if (n_views < 0 || npix < 0) return set_error(CTFP_ERR_INVALID, "negative size");
+    if (!I || (IAbsorb_u32 && !IAbsorb) || (proj && !I0))
+        return set_error(CTFP_ERR_INVALID, "null input for a requested output");
+    if (n_views * npix == 0) return CTFP_OK;
+    postprocess_kernel<<<grid_for(n_views * npix), 256, 0, (hipStream_t)stream>>>(
+        n_views, npix, I, IAbsorb, I0, I_u32, IAbsorb_u32, proj);
+    HIP_TRY(hipGetLastError());
+    return CTFP_OK;
+}
+
+int ctfp_synchronize(void *stream)
+{
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return CTFP_OK;
+}
+
+int ctfp_bench_copy(const void *src, void *dst, int64_t bytes, void *stream)
+{
+    if (!src || !dst || bytes < 0 || bytes % 16)
+        return set_error(CTFP_ERR_INVALID, "copy: pointers and a multiple of 16 bytes required");
+    int64_t n = bytes / 16;
+    copy_kernel<<<grid_for(n), 256, 0, (hipStream_t)stream>>>((const float4 *)src, (float4 *)dst, n);
+    HIP_TRY(hipGetLastError());
+    return CTFP_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,289 @@
+"""Python face of the C ABI: scan set-up, the resident projector context, and
+the host-side detector / grid models.
+
+Device buffers are torch tensors (torch is the allocator and stream provider
+here); every computation runs in libctfp.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def scan_params(**kw) -> L.ScanParams:
+    """ctfp_scan_params from keyword arguments (the member names)."""
+    p = L.ScanParams()
+    defaults = dict(rotated_direction=1)
+    defaults.update(kw)
+    for k, v in defaults.items():
+        setattr(p, k, v)
+    return p
+
+
+def compute_paras(scan: L.ScanParams) -> dict:
+    """CBCTForwardProj::computeParas (Source/CBCTForwardProj.cpp:23-62)."""
+    out = L.SystemInfo()
+    L.check(L.lib().ctfp_compute_paras(C.byref(scan), C.byref(out)))
+    return {name: getattr(out, name) for name, _ in L.SystemInfo._fields_}
+
+
+# ------------------------------------------------------------ host models
+
+def scint_thickness(npix: int, thickness: float, thickness_err: float) -> np.ndarray:
+    """Scintillator thickness map (CBCTForwardProj.cpp:99-111)."""
+    out = np.empty(npix, dtype=np.float32)
+    L.check(L.lib().ctfp_scint_thickness(npix, thickness, thickness_err, _fptr(out)))
+    return out
+
+
+def det_response(factor: float, mu_c: float, thickness: np.ndarray) -> np.ndarray:
+    """eta = f (1 - exp(-mu_c t)) (CBCTForwardProj.cpp:114-120)."""
+    t = np.ascontiguousarray(thickness, dtype=np.float32)
+    out = np.empty_like(t)
+    L.check(L.lib().ctfp_det_response(t.size, factor, mu_c, _fptr(t), _fptr(out)))
+    return out
+
+
+def grid_transmission(d_num: int, d_size_mm: float, interspace_width: int, strip_width: int,
+                      height_mm: float, focal_mm: float, mu_strip, mu_inter) -> np.ndarray:
+    """T_E(u) [n_energy][d_num] of the focused grid (Grid.cpp:373-443)."""
+    g = L.GridParams(d_num, d_size_mm, interspace_width, strip_width, height_mm, focal_mm)
+    ms = np.ascontiguousarray(np.atleast_1d(mu_strip), dtype=np.float32)
+    mi = np.ascontiguousarray(np.atleast_1d(mu_inter), dtype=np.float32)
+    if ms.shape != mi.shape:
+        raise ValueError("mu_strip and mu_inter must have the same length")
+    T = np.empty((ms.size, d_num), dtype=np.float32)
+    L.check(L.lib().ctfp_grid_transmission(C.byref(g), ms.size, _fptr(ms), _fptr(mi), _fptr(T)))
+    return T
+
+
+def grid_period_index(T: np.ndarray) -> np.ndarray:
+    """Grid.cpp:605-620: 0, strict local minima, d_num-1."""
+    t = np.ascontiguousarray(T, dtype=np.float32)
+    cap = t.size + 2
+    out = np.empty(cap, dtype=np.int64)
+    n = C.c_int64(0)
+    L.check(L.lib().ctfp_grid_period_index(t.size, _fptr(t),
+                                           out.ctypes.data_as(C.POINTER(C.c_int64)), cap,
+                                           C.byref(n)))
+    return out[: n.value].copy()
+
+
+# ------------------------------------------------------------- projector
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_handle(stream):
+    if stream is None:
+        return _torch().cuda.current_stream().cuda_stream
+    return getattr(stream, "cuda_stream", stream)
+
+
+class Projector:
+    """One scan resident on one GPU: volumes, spectral weights, geometry.
+
+    Mirrors the life cycle of the reference's device layer (initDevice* ->
+    per-energy forward*GPU -> freeDeviceMemory, CBCTPolyProjGPU.cuh:33-64) with
+    the energy loop folded into one call.
+    """
+
+    def __init__(self, scan: L.ScanParams, device: int = 0):
+        self.scan = scan
+        self.device = device
+        self.nu, self.nv, self.views = scan.d_num_u, scan.d_num_v, scan.proj_num
+        self.n_mat = 0
+        self._ctx = C.c_void_p()
+        L.check(L.lib().ctfp_create(C.byref(scan), device, C.byref(self._ctx)))
+        info = L.SystemInfo()
+        L.check(L.lib().ctfp_get_system_info(self._ctx, C.byref(info)))
+        self.info = {name: getattr(info, name) for name, _ in L.SystemInfo._fields_}
+
+    # -- lifetime
+    def close(self):
+        if self._ctx:
+            L.lib().ctfp_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def samples_per_view(self) -> int:
+        return int(self.nu * self.nv * self.info["int_num"])
+
+    # -- inputs
+    def set_volumes(self, volumes, storage_dtype: int = L.CTFP_F32):
+        """Density volumes [z][y][x] FP32: numpy arrays (host) or CUDA tensors."""
+        vols = list(volumes)
+        on_device = not isinstance(vols[0], np.ndarray)
+        keep = []
+        ptrs = (C.c_void_p * len(vols))()
+        for i, v in enumerate(vols):
+            if on_device:
+                v = v.contiguous()
+                if v.dtype != _torch().float32:
+                    raise TypeError("volumes must be float32")
+                ptrs[i] = v.data_ptr()
+            else:
+                v = np.ascontiguousarray(v, dtype=np.float32)
+                ptrs[i] = v.ctypes.data
+            expect = (self.scan.p_num_z, self.scan.p_num_y, self.scan.p_num_x)
+            if tuple(v.shape) != expect:
+                raise ValueError(f"volume {i} has shape {tuple(v.shape)}, expected {expect}")
+            keep.append(v)
+        L.check(L.lib().ctfp_set_volumes(self._ctx, len(vols), ptrs, int(on_device), storage_dtype))
+        self.n_mat = len(vols)
+
+    def set_labels(self, labels: np.ndarray, density, single_material: bool,
+                   storage_dtype: int = L.CTFP_F32):
+        lab = np.ascontiguousarray(labels, dtype=np.uint8)
+        rho = np.ascontiguousarray(np.atleast_1d(density), dtype=np.float32)
+        n_mat = rho.size
+        L.check(L.lib().ctfp_set_labels(self._ctx, lab.ctypes.data, 0, n_mat, _fptr(rho),
+                                        int(single_material), storage_dtype))
+        self.n_mat = 1 if single_material else n_mat
+
+    def set_spectral(self, spectrum, atten, i0_val: float, grid=None, det=None,
+                     scint_mu=None, scint_thickness=None, det_factor: float = 1.0,
+                     energy_weighting: bool = False):
+        """spectrum [E]; atten [E][n_mat] (1/mm per g/cm^3); grid [E][nu];
+        det [E][nv*nu]; or scint_mu [E] with scint_thickness [nv*nu]."""
+        s = np.ascontiguousarray(spectrum, dtype=np.float32)
+        a = np.ascontiguousarray(atten, dtype=np.float32).reshape(s.size, -1)
+        keep = [s, a]
+        sp = L.Spectral()
+        sp.n_energy = s.size
+        sp.n_mat = a.shape[1]
+        sp.spectrum = _fptr(s)
+        sp.atten = _fptr(a)
+        sp.i0_val = i0_val
+        sp.det_factor = det_factor
+        sp.energy_weighting = int(energy_weighting)
+        if grid is not None:
+            g = np.ascontiguousarray(grid, dtype=np.float32).reshape(s.size, self.nu)
+            keep.append(g)
+            sp.grid = _fptr(g)
+        if det is not None:
+            d = np.ascontiguousarray(det, dtype=np.float32).reshape(s.size, self.nu * self.nv)
+            keep.append(d)
+            sp.det = _fptr(d)
+        elif scint_mu is not None:
+            m = np.ascontiguousarray(scint_mu, dtype=np.float32).reshape(s.size)
+            t = np.ascontiguousarray(scint_thickness, dtype=np.float32).reshape(self.nu * self.nv)
+            keep += [m, t]
+            sp.scint_mu = _fptr(m)
+            sp.scint_thickness = _fptr(t)
+        L.check(L.lib().ctfp_set_spectral(self._ctx, C.byref(sp)))
+
+    # -- compute (device)
+    def _views(self, view_begin, view_count):
+        if view_count is None:
+            view_count = self.views - view_begin
+        return int(view_begin), int(view_count)
+
+    def forward(self, view_begin: int = 0, view_count: int | None = None, I=None, IA=None,
+                L_out=None, want_L: bool = False, stream=None):
+        """Polyenergetic projection; returns (I, IAbsorb[, L]) CUDA tensors."""
+        torch = _torch()
+        vb, vc = self._views(view_begin, view_count)
+        dev = torch.device("cuda", self.device)
+        if I is None:
+            I = torch.empty((vc, self.nv, self.nu), dtype=torch.float32, device=dev)
+        if IA is None:
+            IA = torch.empty((vc, self.nv, self.nu), dtype=torch.float32, device=dev)
+        if want_L and L_out is None:
+            L_out = torch.empty((vc, self.n_mat, self.nv, self.nu), dtype=torch.float32, device=dev)
+        for t in (I, IA) + ((L_out,) if L_out is not None else ()):
+            if not t.is_contiguous() or t.dtype != torch.float32:
+                raise ValueError("outputs must be contiguous float32 CUDA tensors")
+        if I.numel() != vc * self.nv * self.nu or IA.numel() != I.numel():
+            raise ValueError("output size does not match the view range")
+        if L_out is not None and L_out.numel() != vc * self.n_mat * self.nv * self.nu:
+            raise ValueError("L size does not match the view range")
+        L.check(L.lib().ctfp_forward(self._ctx, vb, vc, I.data_ptr(), IA.data_ptr(),
+                                     L_out.data_ptr() if L_out is not None else None,
+                                     _stream_handle(stream)))
+        return (I, IA, L_out) if want_L or L_out is not None else (I, IA)
+
+    def line_integrals(self, view_begin: int = 0, view_count: int | None = None, out=None,
+                       stream=None):
+        torch = _torch()
+        vb, vc = self._views(view_begin, view_count)
+        if out is None:
+            out = torch.empty((vc, self.n_mat, self.nv, self.nu), dtype=torch.float32,
+                              device=torch.device("cuda", self.device))
+        if out.numel() != vc * self.n_mat * self.nv * self.nu or not out.is_contiguous():
+            raise ValueError("L output has the wrong size or layout")
+        L.check(L.lib().ctfp_line_integrals(self._ctx, vb, vc, out.data_ptr(),
+                                            _stream_handle(stream)))
+        return out
+
+    def flat_field(self, stream=None):
+        torch = _torch()
+        out = torch.empty((self.nv, self.nu), dtype=torch.float32,
+                          device=torch.device("cuda", self.device))
+        L.check(L.lib().ctfp_flat_field(self._ctx, out.data_ptr(), _stream_handle(stream)))
+        return out
+
+    # -- compute (host buffers)
+    def forward_host(self):
+        """Whole scan into numpy arrays: (I, IAbsorb, I0)."""
+        I = np.empty((self.views, self.nv, self.nu), dtype=np.float32)
+        IA = np.empty_like(I)
+        I0 = np.empty((self.nv, self.nu), dtype=np.float32)
+        L.check(L.lib().ctfp_forward_host(self._ctx, I.ctypes.data, IA.ctypes.data, I0.ctypes.data))
+        return I, IA, I0
+
+
+def postprocess(I, IA, I0, stream=None):
+    """Floor to uint32 and p = ln(I0 / floor(I)) on the device
+    (CBCTForwardProj.cpp:79-90, 376-387).  Returns (I_u32, IA_u32, proj);
+    the uint32 images come back as int32 tensors holding the same bits."""
+    torch = _torch()
+    n_views = I.shape[0]
+    npix = I[0].numel()
+    Iu = torch.empty(I.shape, dtype=torch.int32, device=I.device)
+    IAu = torch.empty(I.shape, dtype=torch.int32, device=I.device)
+    proj = torch.empty(I.shape, dtype=torch.float32, device=I.device)
+    L.check(L.lib().ctfp_postprocess(n_views, npix, I.data_ptr(), IA.data_ptr(), I0.data_ptr(),
+                                     Iu.data_ptr(), IAu.data_ptr(), proj.data_ptr(),
+                                     _stream_handle(stream)))
+    return Iu, IAu, proj
+
+
+def spectral_inputs(setup, nu: int, nv: int, with_response: bool = True,
+                    with_grid: bool | None = None) -> dict:
+    """Host tables of a synthetic ScanSetup, ready for Projector.set_spectral."""
+    spec = setup.spectrum()
+    kw = dict(spectrum=spec, atten=setup.atten(), i0_val=setup.i0_val,
+              energy_weighting=setup.energy_weighting, det_factor=setup.det_factor)
+    use_grid = (setup.grid is not None) if with_grid is None else with_grid
+    if use_grid:
+        g = setup.grid
+        mu_s, mu_i = setup.grid_mu()
+        kw["grid"] = grid_transmission(nu, setup.d_size, g["interspace_width"], g["strip_width"],
+                                       g["height_mm"], g["focal_mm"], mu_s, mu_i)
+    if with_response:
+        kw["scint_mu"] = setup.scint_mu()
+        kw["scint_thickness"] = scint_thickness(nu * nv, setup.scint_thickness,
+                                                setup.scint_thickness_err)
+    return kw

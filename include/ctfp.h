@@ -1,0 +1,210 @@
+/*
+ * ctfp.h -- C ABI of the MI355X polyenergetic cone-beam forward projector.
+ *
+ * This is the drop-in boundary for the reference's GPU layer
+ * (Source/CUDAFun/CBCTPolyProjGPU.cuh:33-64).  The reference exposes C++
+ * functions taking non-POD structs by reference/value and keeps its texture in
+ * hidden globals; this ABI replaces them with plain structs, plain pointers,
+ * an opaque per-scan context and integer status codes.  Every entry point
+ * names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Status: every function returns CTFP_OK (0) or a negative CTFP_ERR_*;
+ *     ctfp_last_error() returns the calling thread's last message.  Nothing
+ *     here calls exit() (the reference does, e.g. CBCTForwardProj.cpp:155-160).
+ *   - Layouts follow the reference: volumes [z][y][x] (x fastest,
+ *     CBCTPolyProjGPU.cu:2184), projections [view][v][u]
+ *     (CBCTPolyProjGPU.cu:1994), line integrals [view][material][v][u].
+ *   - "Device pointer" arguments are HIP device allocations on the context's
+ *     device; "stream" arguments are hipStream_t passed as void* (NULL = the
+ *     null stream).
+ *   - Units: mm, g/cm^3, 1/mm.  Energy bin k (0-based) is (k+1) keV
+ *     (CBCTPolyForwardProjGrid.cpp:40, 387).
+ *   - A context is not thread-safe; use one context per host thread.  Several
+ *     contexts may coexist on one device.
+ */
+#ifndef CTFP_H
+#define CTFP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CTFP_OK 0
+#define CTFP_ERR_INVALID -1     /* bad argument or geometry */
+#define CTFP_ERR_DEVICE -2      /* HIP runtime error */
+#define CTFP_ERR_NOMEM -3       /* host or device allocation failed */
+#define CTFP_ERR_STATE -4       /* call out of order (e.g. forward before volumes) */
+#define CTFP_ERR_IO -5          /* file could not be opened / read / written */
+#define CTFP_ERR_CONFIG -6      /* configuration entry missing or invalid */
+#define CTFP_ERR_UNSUPPORTED -7 /* valid request this build does not handle */
+
+/* Storage precision of the resident material volumes. */
+#define CTFP_F32 0
+#define CTFP_F16 1
+#define CTFP_BF16 2
+
+/* Scan description.  Mirrors the geometry members of CTScanParas
+ * (Source/DataType.h:33-55) as a POD struct (no std::string, Q24). */
+typedef struct ctfp_scan_params {
+    int64_t p_num_x, p_num_y, p_num_z;     /* phantom voxels (pNumX/Y/Z) */
+    float p_length_x, p_length_y, p_length_z; /* phantom size, mm (pLengthX/Y/Z) */
+    int64_t proj_num;                       /* views over 360 degrees (projNum) */
+    int64_t d_num_u, d_num_v;               /* detector pixels (dNumU, dNumV) */
+    float sod, sdd;                         /* mm */
+    int32_t rotated_direction;              /* +1 or -1 */
+    float d_size;                           /* detector pitch, mm (dSize) */
+} ctfp_scan_params;
+
+/* Derived scalars.  Mirrors CTScanSystemInfo (Source/DataType.h:78-111). */
+typedef struct ctfp_system_info {
+    int32_t rotated_direction;
+    float theta_step;                  /* degrees */
+    float d_half_lu, d_half_lv;        /* half detector extents, mm */
+    float fov_r, fov_h;                /* mm */
+    float p_half_x, p_half_y, p_half_z;
+    float p_size_x, p_size_y, p_size_z;
+    float dx, dy, dz;                  /* integration step; dx is used */
+    int64_t int_num;                   /* samples per ray, K */
+} ctfp_system_info;
+
+/* Spectral weights of one scan.  Replaces the per-energy uploads of the
+ * reference's forward*GPU wrappers (CBCTPolyProjGPU.cu:25-30, 204-207, 393)
+ * and the scalars spectrumVal / phantomMAtten (Source/DataType.h:109-110).
+ * All pointers are host pointers and are copied; they may be freed after the
+ * call.  Detector response, by priority: `det` table, else the scintillator
+ * model (scint_mu with scint_thickness), else none (NoResponse variants). */
+typedef struct ctfp_spectral {
+    int32_t n_energy;              /* bins (specEnergyNum) */
+    int32_t n_mat;                 /* materials; must match the volumes */
+    const float *spectrum;         /* [n_energy] normalised spectrum S_E */
+    const float *atten;            /* [n_energy][n_mat] attenuation per unit density,
+                                      1/mm per g/cm^3 = (mu/rho)/10 */
+    float i0_val;                  /* photons per pixel before response (I0Val) */
+    const float *grid;             /* [n_energy][d_num_u] grid transmission T_E(u), or NULL */
+    const float *det;              /* [n_energy][d_num_v*d_num_u] response eta, or NULL */
+    const float *scint_mu;         /* [n_energy] scintillator mu, 1/mm, or NULL */
+    const float *scint_thickness;  /* [d_num_v*d_num_u] scintillator thickness, mm */
+    float det_factor;              /* photoelectric conversion factor f */
+    int32_t energy_weighting;      /* 0: photon counting (the reference); 1: energy
+                                      integrating, weights scaled by (k+1) keV */
+} ctfp_spectral;
+
+/* Focused anti-scatter grid.  Mirrors GridAndDetectorSystem (Source/Grid.h:42-66). */
+typedef struct ctfp_grid_params {
+    int64_t d_num;            /* pixels along u (dNums) */
+    float d_size_mm;          /* pixel pitch, mm; truncated to whole um as the reference does */
+    int32_t interspace_width; /* um */
+    int32_t strip_width;      /* um */
+    float height_mm;          /* lamella height h */
+    float focal_mm;           /* focal distance FD */
+} ctfp_grid_params;
+
+typedef struct ctfp_context ctfp_context;
+
+/* ---- library ------------------------------------------------------------ */
+const char *ctfp_last_error(void);
+const char *ctfp_version(void);
+
+/* Replaces CBCTForwardProj::computeParas (Source/CBCTForwardProj.cpp:23-62). */
+int ctfp_compute_paras(const ctfp_scan_params *scan, ctfp_system_info *out);
+
+/* ---- scan context --------------------------------------------------------- */
+/* Replaces the allocation half of initDeviceGrid / initDeviceNoGrid /
+ * initDeviceSinMat{,NoResponse}{Grid,NoGrid} (CBCTPolyProjGPU.cu:478-571,
+ * 1112-1208, 573-745, 1210-1380 ...).  No cudaDeviceReset, no globals. */
+int ctfp_create(const ctfp_scan_params *scan, int device, ctfp_context **out);
+/* Replaces freeDeviceMemory (CBCTPolyProjGPU.cu:1774-1797). */
+int ctfp_destroy(ctfp_context *ctx);
+int ctfp_get_system_info(const ctfp_context *ctx, ctfp_system_info *out);
+
+/* Upload n_mat density volumes ([z][y][x] FP32 each), kept resident in
+ * `storage_dtype`.  Replaces initTexture3D / updateTex
+ * (CBCTPolyProjGPU.cu:2175-2221), which re-uploaded mu(E) every energy.
+ * volumes_on_device: 0 = host pointers, 1 = device pointers. */
+int ctfp_set_volumes(ctfp_context *ctx, int n_mat, const float *const *volumes,
+                     int volumes_on_device, int storage_dtype);
+
+/* Build the density channels from a uint8 label volume on the device.
+ * single_material = 0: channel m = density[m] where label == m+1, else 0
+ *   (PhantomObject::updataLineAtten, PhantomObject.cpp:38-58, with (mu/rho)/10
+ *   moved to ctfp_spectral.atten);
+ * single_material = 1: one channel = label * density[0]
+ *   (PhantomObject::computeDesity, PhantomObject.cpp:65-71). */
+int ctfp_set_labels(ctfp_context *ctx, const uint8_t *labels, int labels_on_device,
+                    int n_mat, const float *density, int single_material,
+                    int storage_dtype);
+
+/* Replaces the per-energy host uploads and scalars; see ctfp_spectral. */
+int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *spec);
+
+/* Polyenergetic forward projection of views [view_begin, view_begin+view_count):
+ * ray-march of every material once, then the whole spectrum per ray.
+ * Replaces forward{Proj,SinMatProj,SinMatNoResponseProj}{Grid,NoGrid}GPU
+ * (CBCTPolyProjGPU.cu:12-474) together with the host energy loops that
+ * called them per energy and accumulated (CBCTForwardProj.cpp:65-76).
+ * I, IAbsorb: device [view_count][d_num_v][d_num_u]; L: device
+ * [view_count][n_mat][d_num_v][d_num_u] or NULL.  Asynchronous on `stream`. */
+int ctfp_forward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *I,
+                 float *IAbsorb, float *L, void *stream);
+
+/* Line integrals only (the density march of initDeviceSinMat*, replacing
+ * computeSinMatIndensityProjKernel, CBCTPolyProjGPU.cu:2005-2020).
+ * L: device [view_count][n_mat][d_num_v][d_num_u]. */
+int ctfp_line_integrals(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *L,
+                        void *stream);
+
+/* Flat field I0 = sum_E I0Val*S_E*T_E*eta_E in bin order.  Replaces addI0 /
+ * addI0NoResponse (CBCTPolyForwardProjGrid.cpp:330-350,
+ * CBCTPolyForwardProjNoGrid.cpp:290-310).  I0: device [d_num_v][d_num_u]. */
+int ctfp_flat_field(ctfp_context *ctx, float *I0, void *stream);
+
+/* Whole scan into host buffers (convenience for the reference-shaped classes):
+ * I, IAbsorb [proj_num][d_num_v][d_num_u], I0 [d_num_v][d_num_u] (NULL skips). */
+int ctfp_forward_host(ctfp_context *ctx, float *I, float *IAbsorb, float *I0);
+
+/* Floor to uint32 and p = log(I0 / floor(I)).  Replaces convertIandIAbsorbDt
+ * and computeProj (CBCTForwardProj.cpp:79-90, 376-387).  Device pointers;
+ * any output may be NULL. */
+int ctfp_postprocess(int64_t n_views, int64_t npix, const float *I, const float *IAbsorb,
+                     const float *I0, uint32_t *I_u32, uint32_t *IAbsorb_u32, float *proj,
+                     void *stream);
+
+int ctfp_synchronize(void *stream);
+
+/* ---- host-side detector and grid models ------------------------------------ */
+/* Scintillator thickness map with the reference's jitter: seed 100, MSVC CRT
+ * rand() (x = 214013x + 2531011 mod 2^32, bits 30..16).  Replaces
+ * CBCTForwardProj::computePerScinThinckness (CBCTForwardProj.cpp:99-111). */
+int ctfp_scint_thickness(int64_t npix, float thickness, float thickness_err, float *out);
+
+/* eta = f (1 - exp(-mu_c t_j)).  Replaces CBCTForwardProj::updateDetResponse
+ * (CBCTForwardProj.cpp:114-120). */
+int ctfp_det_response(int64_t npix, float factor, float mu_c, const float *thickness,
+                      float *out);
+
+/* Grid transmission T_E(u) for n_energy energies, 1-um lamella model.
+ * Replaces Grid::defGrid / updataGrid / binning1D (Grid.cpp:373-443, 297-309).
+ * mu_strip, mu_inter: [n_energy] in 1/mm; T: [n_energy][d_num]. */
+int ctfp_grid_transmission(const ctfp_grid_params *grid, int n_energy, const float *mu_strip,
+                           const float *mu_inter, float *T);
+
+/* Grid period indices: 0, every strict local minimum of T, d_num-1.
+ * Replaces Grid::RetrievePeriodIndex (Grid.cpp:605-620).  `out` holds up to
+ * `capacity` entries; *count receives the number found. */
+int ctfp_grid_period_index(int64_t d_num, const float *T, int64_t *out, int64_t capacity,
+                           int64_t *count);
+
+/* ---- measurement helper ----------------------------------------------------- */
+/* Streaming device copy of `bytes` (a multiple of 16) on `stream`.  No
+ * counterpart in the reference: a kernel of known traffic used to calibrate
+ * hardware byte counters (FETCH_SIZE / WRITE_SIZE) against the projector. */
+int ctfp_bench_copy(const void *src, void *dst, int64_t bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CTFP_H */

@@ -1,0 +1,289 @@
+"""Parity of the HIP path (through the C ABI) with the oracle's FP32 mirror of
+the reference method, on seeded inputs.
+
+Tolerance (FP32 path vs FP32 mirror): 1e-4 relative, measured as
+max |d| / max(|ref|, 1e-3 max|ref|) on I, IAbsorb, the line integrals and the
+log projection p = ln(I0 / I) before the floor.  Full-size scans are checked
+through size-independent properties (I + IAbsorb = I0, exact linearity in
+density, view-range independence).
+"""
+import numpy as np
+import pytest
+
+import ct_forward_projection_amd as P
+from ct_forward_projection_amd import synthetic as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-4
+
+
+def rel_err(got, ref):
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    floor = 1e-3 * np.abs(ref).max() if ref.size else 0.0
+    if floor == 0.0:
+        return float(np.abs(got - ref).max()) if ref.size else 0.0
+    return float((np.abs(got - ref) / np.maximum(np.abs(ref), floor)).max())
+
+
+def to_np(t):
+    return t.detach().cpu().numpy()
+
+
+def run_projector(setup, vols, views=None, view_begin=0, spec_kw=None, want_L=True):
+    proj = P.Projector(P.scan_params(**setup.scan_kwargs()))
+    proj.set_volumes(vols)
+    proj.set_spectral(**spec_kw)
+    n = setup.views - view_begin if views is None else views
+    I, IA, L = proj.forward(view_begin, n, want_L=want_L)
+    I0 = proj.flat_field()
+    import torch
+    torch.cuda.synchronize()
+    out = to_np(I), to_np(IA), to_np(L), to_np(I0)
+    proj.close()
+    return out
+
+
+def oracle_expect(setup, vols, spec_kw, view_begin, views):
+    g = O.geom_from_setup(setup)
+    p = O.paras(g)
+    L = O.march_f32(g, p, vols, view_begin, views)
+    nu, nv = setup.det
+    grid = spec_kw.get("grid")
+    det = spec_kw.get("det")
+    spectrum = np.asarray(spec_kw["spectrum"], np.float32)
+    if spec_kw.get("energy_weighting"):
+        spectrum = spectrum * np.arange(1, spectrum.size + 1, dtype=np.float32)
+    if det is None and spec_kw.get("scint_mu") is not None:
+        mu = spec_kw["scint_mu"]
+        det = np.stack([O.det_response(spec_kw.get("det_factor", 1.0), float(mu[e]),
+                                       spec_kw["scint_thickness"]) for e in range(mu.size)])
+    I, IA, I0 = O.spectrum_f32(L, spec_kw["atten"], spectrum, spec_kw["i0_val"], grid, det)
+    return I, IA, L, I0
+
+
+def check_all(got, ref, tol=TOL):
+    I, IA, L, I0 = got
+    rI, rIA, rL, rI0 = ref
+    assert rel_err(L, rL) < tol, "line integrals"
+    assert rel_err(I, rI) < tol, "I"
+    assert rel_err(IA, rIA) < tol, "IAbsorb"
+    assert rel_err(I0, rI0) < 1e-6, "I0"
+    p = np.log(I0[None].astype(np.float64) / I)
+    pr = np.log(rI0[None].astype(np.float64) / rI)
+    assert rel_err(p, pr) < tol, "log projection"
+
+
+@pytest.fixture(scope="module")
+def cfg1_phantom():
+    return S.cylinder_phantom()
+
+
+def test_config1_single_material(require_gpu, cfg1_phantom):
+    """128^3 water/ellipsoid phantom, 256^2 detector, K=272, energy-integrating
+    CsI response, views 0..3 and 44..47 of 90."""
+    s = S.config1()
+    kw = P.spectral_inputs(s, *s.det)
+    for vb in (0, 44):
+        got = run_projector(s, [cfg1_phantom], views=4, view_begin=vb, spec_kw=kw)
+        ref = oracle_expect(s, [cfg1_phantom], kw, vb, 4)
+        check_all(got, ref)
+
+
+def small_three_material(n=96, views=12, det=(160, 120), d_size=0.8, terr=0.05):
+    w, b, i = S.three_material_phantom(n, seed=99)
+    s = S.config2()
+    s.n_vox = (n, n, n)
+    s.length_mm = (n * 0.5,) * 3
+    s.det = det
+    s.d_size = d_size
+    s.views = views
+    s.scint_thickness_err = terr
+    return s, [w, b, i]
+
+
+def test_three_material_grid_response(require_gpu):
+    s, vols = small_three_material()
+    kw = P.spectral_inputs(s, *s.det)
+    got = run_projector(s, vols, spec_kw=kw)
+    ref = oracle_expect(s, vols, kw, 0, s.views)
+    check_all(got, ref)
+
+
+def test_no_grid_no_response_variants(require_gpu):
+    s, vols = small_three_material(views=5)
+    for grid, resp in [(False, True), (True, False), (False, False)]:
+        kw = P.spectral_inputs(s, *s.det, with_response=resp, with_grid=grid)
+        got = run_projector(s, vols, spec_kw=kw)
+        ref = oracle_expect(s, vols, kw, 0, s.views)
+        check_all(got, ref)
+
+
+def test_ragged_detector_and_view_range(require_gpu):
+    s, vols = small_three_material(n=64, views=17, det=(37, 29), d_size=2.3)
+    kw = P.spectral_inputs(s, *s.det)
+    got = run_projector(s, vols, views=3, view_begin=13, spec_kw=kw)
+    ref = oracle_expect(s, vols, kw, 13, 3)
+    check_all(got, ref)
+
+
+def test_single_view_and_negative_rotation(require_gpu):
+    s, vols = small_three_material(n=48, views=7, det=(50, 40), d_size=1.6)
+    s.rot = -1
+    kw = P.spectral_inputs(s, *s.det)
+    got = run_projector(s, vols, spec_kw=kw)
+    ref = oracle_expect(s, vols, kw, 0, s.views)
+    check_all(got, ref)
+
+
+def test_labels_multi_material_literal_method(require_gpu):
+    """Label volume through ctfp_set_labels vs the reference's literal method
+    (per-energy mu(E) volume re-march), including labels above MaterialNum."""
+    n = 40
+    labels = S.label_phantom(n, 3, seed=3)
+    density = np.array([1.0, 1.92, 2.699], np.float32)
+    s = S.config2()
+    s.n_vox, s.length_mm, s.det, s.d_size, s.views = (n, n, n), (40.0,) * 3, (48, 36), 1.5, 4
+    s.materials = (S.WATER, S.BONE, S.ALUMINIUM)
+    s.kvp = 40.0
+    kw = P.spectral_inputs(s, *s.det)
+    proj = P.Projector(P.scan_params(**s.scan_kwargs()))
+    proj.set_labels(labels, density, single_material=False)
+    proj.set_spectral(**kw)
+    I, IA = proj.forward()
+    I0 = proj.flat_field()
+    I, IA, I0 = to_np(I), to_np(IA), to_np(I0)
+    proj.close()
+    g = O.geom_from_setup(s)
+    p = O.paras(g)
+    mass = (s.atten() * np.float32(10)).astype(np.float32)
+    det = np.stack([O.det_response(1.0, float(m), kw["scint_thickness"]) for m in kw["scint_mu"]])
+    rI, rIA, rI0 = O.forward_literal_f32(g, p, labels, density, mass, kw["spectrum"],
+                                         kw["i0_val"], kw["grid"], det)
+    assert rel_err(I, rI) < TOL
+    assert rel_err(IA, rIA) < 5 * TOL
+    assert rel_err(I0, rI0) < 1e-6
+
+
+def test_labels_single_material_density(require_gpu):
+    """computeDesity path: density = label * rho0 (labels above 1 scale)."""
+    n = 40
+    labels = S.label_phantom(n, 2, seed=4)
+    s = S.config1()
+    s.n_vox, s.length_mm, s.det, s.d_size, s.views = (n, n, n), (40.0,) * 3, (48, 36), 1.5, 3
+    kw = P.spectral_inputs(s, *s.det)
+    proj = P.Projector(P.scan_params(**s.scan_kwargs()))
+    proj.set_labels(labels, [1.05], single_material=True)
+    proj.set_spectral(**kw)
+    I, IA, L = proj.forward(want_L=True)
+    got = to_np(I), to_np(IA), to_np(L), to_np(proj.flat_field())
+    proj.close()
+    dens = O.labels_to_density_single(labels, 1.05)
+    ref = oracle_expect(s, [dens], kw, 0, s.views)
+    check_all(got, ref)
+
+
+def test_empty_volume_is_flat_field(require_gpu):
+    import torch
+    s, vols = small_three_material(n=32, views=3, det=(40, 24), d_size=2.0)
+    kw = P.spectral_inputs(s, *s.det)
+    zeros = [np.zeros_like(v) for v in vols]
+    I, IA, L, I0 = run_projector(s, zeros, spec_kw=kw)
+    assert np.all(L == 0) and np.all(IA == 0)
+    np.testing.assert_array_equal(I, np.broadcast_to(I0, I.shape))
+
+
+def test_postprocess_matches_oracle(require_gpu):
+    import torch
+    rng = np.random.default_rng(8)
+    I = rng.uniform(0, 5000, size=(3, 20, 30)).astype(np.float32)
+    I[0, 0, :5] = [0.0, 0.5, 1.0, 1.999, 2.0]
+    IA = rng.uniform(0, 5000, size=I.shape).astype(np.float32)
+    I0 = rng.uniform(4000, 6000, size=(20, 30)).astype(np.float32)
+    dev = torch.device("cuda")
+    Iu, IAu, proj = P.postprocess(torch.from_numpy(I).to(dev), torch.from_numpy(IA).to(dev),
+                                  torch.from_numpy(I0).to(dev))
+    rIu, rIAu, rproj = O.postprocess(I, IA, I0)
+    np.testing.assert_array_equal(to_np(Iu).view(np.uint32), rIu)
+    np.testing.assert_array_equal(to_np(IAu).view(np.uint32), rIAu)
+    got = to_np(proj)
+    finite = np.isfinite(rproj)
+    assert np.array_equal(np.isinf(got), np.isinf(rproj))
+    np.testing.assert_allclose(got[finite], rproj[finite], rtol=2e-6, atol=1e-6)
+
+
+def test_forward_host_matches_device(require_gpu):
+    s, vols = small_three_material(n=48, views=5, det=(64, 48), d_size=1.6)
+    kw = P.spectral_inputs(s, *s.det)
+    I, IA, _, I0 = run_projector(s, vols, spec_kw=kw)
+    proj = P.Projector(P.scan_params(**s.scan_kwargs()))
+    proj.set_volumes(vols)
+    proj.set_spectral(**kw)
+    hI, hIA, hI0 = proj.forward_host()
+    proj.close()
+    np.testing.assert_array_equal(hI, I)
+    np.testing.assert_array_equal(hIA, IA)
+    np.testing.assert_array_equal(hI0, I0)
+
+
+def test_errors_without_inputs(require_gpu):
+    s, vols = small_three_material(n=32, views=2, det=(16, 16), d_size=4.0)
+    proj = P.Projector(P.scan_params(**s.scan_kwargs()))
+    with pytest.raises(P.CTFPError):
+        proj.forward()                       # no volumes
+    proj.set_volumes(vols)
+    with pytest.raises(P.CTFPError):
+        proj.forward()                       # no spectral weights
+    with pytest.raises(P.CTFPError):
+        proj.forward(1, 5)                   # view range outside the scan
+    kw = P.spectral_inputs(s, *s.det)
+    kw["atten"] = kw["atten"][:, :2]
+    with pytest.raises(P.CTFPError):
+        proj.set_spectral(**kw)              # material count mismatch
+    proj.close()
+
+
+# ------------------------------------------------------- full-size config 2
+
+@pytest.fixture(scope="module")
+def cfg2_volumes():
+    return list(S.three_material_phantom(512))
+
+
+def test_config2_full_size_view_parity(require_gpu, cfg2_volumes):
+    """One full 1024x768 view (K=1082, 850.9 M samples) of the 3 x 512^3
+    phantom with grid and CsI response, against the oracle."""
+    s = S.config2()
+    kw = P.spectral_inputs(s, *s.det)
+    got = run_projector(s, cfg2_volumes, views=1, view_begin=137, spec_kw=kw)
+    ref = oracle_expect(s, cfg2_volumes, kw, 137, 1)
+    check_all(got, ref)
+
+
+def test_config2_full_scan_properties(require_gpu, cfg2_volumes):
+    """All 360 views: I + IAbsorb = I0, doubling the densities doubles the
+    line integrals exactly, and a view sub-range reproduces its slice."""
+    import torch
+    s = S.config2()
+    kw = P.spectral_inputs(s, *s.det)
+    proj = P.Projector(P.scan_params(**s.scan_kwargs()))
+    proj.set_volumes(cfg2_volumes)
+    proj.set_spectral(**kw)
+    I, IA = proj.forward()
+    I0 = proj.flat_field()
+    torch.cuda.synchronize()
+    total = (I + IA).double()
+    err = ((total - I0.double()[None]).abs() / I0.double()[None]).max().item()
+    assert err < 1e-5
+    sub_I, sub_IA = proj.forward(200, 7)
+    torch.cuda.synchronize()
+    assert torch.equal(sub_I, I[200:207]) and torch.equal(sub_IA, IA[200:207])
+    del I, IA, total
+    L1 = proj.line_integrals(90, 2)
+    proj.set_volumes([2 * v for v in cfg2_volumes])
+    L2 = proj.line_integrals(90, 2)
+    torch.cuda.synchronize()
+    assert torch.equal(L2, 2 * L1)
+    assert (L1[:, 0] > 0).float().mean().item() > 0.3
+    proj.close()

@@ -1,0 +1,363 @@
+"""Pins the CPU oracle (oracle/ctfp_oracle.c) against what the mathematics and
+the reference's own fixed data fix: closed forms, invariants, special cases and
+brute force on tiny inputs.  The reference ships no tests or golden outputs, so
+each check below is chosen so that a plausible mistake (a dropped term, a wrong
+sign or index, a transposed operand) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from ct_forward_projection_amd import synthetic as S
+from oracle import oracle as O
+
+
+def small_geom(n=32, nu=24, nv=16, views=4, lx=32.0, dsize=1.2, sod=400.0, sdd=600.0):
+    return O.geom(n, n, n, lx, lx, lx, views, nu, nv, sod, sdd, dsize)
+
+
+# ----------------------------------------------------------------- scalars
+
+@pytest.mark.parametrize("setup,K,R", [(S.config1(), 272, 68.108), (S.config2(), 1082, 135.278)])
+def test_compute_paras_closed_form(setup, K, R):
+    g = O.geom_from_setup(setup)
+    p = O.paras(g)
+    hu = setup.det[0] * setup.d_size / 2
+    r64 = setup.sod * hu / math.sqrt(hu * hu + setup.sdd ** 2)
+    assert p.fov_r == pytest.approx(r64, rel=1e-6)
+    assert p.fov_r == pytest.approx(R, abs=1e-3)
+    assert p.dx == pytest.approx(0.5 * setup.length_mm[0] / setup.n_vox[0], rel=1e-7)
+    assert p.n_steps == K == round(2 * r64 / p.dx)
+    hv = setup.det[1] * setup.d_size / 2
+    assert p.fov_h == pytest.approx((setup.sod - r64) * hv / setup.sdd, rel=1e-6)
+
+
+def test_view_angles():
+    g = small_geom(views=90)
+    p = O.paras(g)
+    for i in (0, 1, 45, 89):
+        assert O.view_angle(g, p, i) == pytest.approx(math.radians(4.0 * i), rel=1e-6, abs=1e-7)
+    g.rot = -1
+    assert O.view_angle(g, p, 10) == pytest.approx(-math.radians(40.0), rel=1e-6)
+
+
+# --------------------------------------------------------------- detector
+
+def test_msvc_rand_published_sequence():
+    # First ten rand() values after srand(1) on the MSVC CRT.
+    assert O.msvc_rand_sequence(1, 10).tolist() == [41, 18467, 6334, 26500, 19169, 15724,
+                                                    11478, 29358, 26962, 24464]
+
+
+def test_scint_thickness_jitter():
+    t = O.scint_thickness(1000, 0.6, 0.0)
+    assert np.all(t == np.float32(0.6))
+    t = O.scint_thickness(5000, 0.4, 0.05)
+    assert t.min() >= np.float32(0.35) - 1e-6 and t.max() <= np.float32(0.45) + 1e-6
+    r = O.msvc_rand_sequence(100, 3)
+    span = 1001
+    for i in range(3):
+        expect = np.float32(0.4) + (np.float32(-0.05) + np.float32(r[i] % span) / np.float32(1e4))
+        assert t[i] == pytest.approx(float(expect), abs=1e-7)
+    steps = (t.astype(np.float64) - 0.35) / 1e-4
+    assert np.abs(steps - np.round(steps)).max() < 1e-2  # 0.1-um quantisation
+
+
+def test_det_response_closed_form():
+    th = np.linspace(0.1, 1.0, 50).astype(np.float32)
+    eta = O.det_response(0.8, 2.3, th)
+    np.testing.assert_allclose(eta, 0.8 * (1 - np.exp(-2.3 * th.astype(np.float64))), rtol=2e-6)
+
+
+# ---------------------------------------------------------------- geometry
+
+def test_ray_direction_unit_and_axes():
+    g = O.geom(32, 32, 32, 32.0, 32.0, 32.0, 4, 5, 7, 400.0, 600.0, 2.0)
+    p = O.paras(g)
+    for u in range(5):
+        for v in range(7):
+            d = O.ray_direction_f64(g, p, u, v)
+            assert np.linalg.norm(d) == pytest.approx(1.0, abs=1e-14)
+            U = 5 * 2.0 / 2 - 1.0 - 2.0 * u
+            W = 7 * 2.0 / 2 - 1.0 - 2.0 * v
+            true = np.array([600.0, U, W]) / math.sqrt(600.0 ** 2 + U * U + W * W)
+            if U == 0 or W == 0:
+                # on the central row / column the mixed convention is exact
+                np.testing.assert_allclose(d, true, atol=1e-15)
+            else:
+                assert np.abs(d - true).max() > 1e-9
+    # u = 0 is at +U, v = 0 at +W
+    d = O.ray_direction_f64(g, p, 0, 0)
+    assert d[1] > 0 and d[2] > 0
+
+
+@pytest.mark.parametrize("setup,offset_mm", [(S.config1(), 0.156), (S.config2(), 0.69)])
+def test_ray_direction_corner_offset(setup, offset_mm):
+    # The mixed gamma/beta convention misses the corner pixel: at distance SOD
+    # along the ray the sample is ~0.156 mm (256^2 / 0.8 mm detector) and
+    # ~0.69 mm (1024x768 / 0.4 mm) away from the ray through the pixel centre.
+    g = O.geom_from_setup(setup, views=1)
+    p = O.paras(g)
+    d = O.ray_direction_f64(g, p, 0, 0)
+    half = setup.d_size / 2
+    true = np.array([setup.sdd, p.half_u - half, p.half_v - half])
+    true /= np.linalg.norm(true)
+    assert np.linalg.norm(d - true) * setup.sod == pytest.approx(offset_mm, abs=0.004)
+
+
+# --------------------------------------------------------------- trilinear
+
+def test_trilinear_known_answers():
+    vol = np.arange(1, 9, dtype=np.float32).reshape(2, 2, 2)  # [z][y][x]
+    for z in range(2):
+        for y in range(2):
+            for x in range(2):
+                # texture coordinate i + 0.5 is voxel i
+                assert O.tex3d_f32(vol, x + 0.5, y + 0.5, z + 0.5) == vol[z, y, x]
+                assert O.tri_f64(vol, x, y, z) == vol[z, y, x]
+    assert O.tex3d_f32(vol, 1.0, 1.0, 1.0) == pytest.approx(vol.mean())
+    # zero border: half way to the border keeps half of the edge voxel
+    assert O.tex3d_f32(vol, 0.0, 0.5, 0.5) == pytest.approx(0.5 * vol[0, 0, 0])
+    assert O.tri_f64(vol, -0.5, 0.0, 0.0) == pytest.approx(0.5 * vol[0, 0, 0])
+    assert O.tex3d_f32(vol, 2.0, 1.5, 1.5) == pytest.approx(0.5 * vol[1, 1, 1])
+    assert O.tex3d_f32(vol, 2.5, 1.5, 1.5) == 0.0
+    for c in (-0.5, -3.0, 3.0, 100.0):
+        assert O.tex3d_f32(vol, c, 0.5, 0.5) == 0.0
+        assert O.tri_f64(vol, 0.0, c - 0.5, 0.0) == 0.0
+
+
+def test_trilinear_constant_and_ramp():
+    n = 6
+    const = np.full((n, n, n), 3.25, dtype=np.float32)
+    ramp = np.broadcast_to(np.arange(n, dtype=np.float32) * 2.0, (n, n, n)).copy()
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        x, y, z = rng.uniform(0.0, n - 1.0, size=3)
+        assert O.tri_f64(const, x, y, z) == pytest.approx(3.25, abs=1e-12)
+        assert O.tri_f64(ramp, x, y, z) == pytest.approx(2.0 * x, abs=1e-9)
+        assert O.tex3d_f32(ramp, x + 0.5, y + 0.5, z + 0.5) == pytest.approx(2.0 * x, abs=1e-4)
+    # within one voxel of the edge the zero border ramps the constant down
+    assert O.tri_f64(const, -0.25, 2.0, 2.0) == pytest.approx(0.75 * 3.25)
+
+
+# ------------------------------------------------------------ line integral
+
+def _sphere(n, r, centre_idx, rho=1.0):
+    z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64),) * 3, indexing="ij")
+    d2 = (x - centre_idx[2]) ** 2 + (y - centre_idx[1]) ** 2 + (z - centre_idx[0]) ** 2
+    return (d2 <= r * r).astype(np.float32) * rho
+
+
+def test_fixed_window_truncates():
+    # Samples cover t in [SOD - R_FOV, SOD + R_FOV] only: a uniform slab wider
+    # than the window integrates to exactly 2 R_FOV * rho on the central ray.
+    n = 64
+    g = O.geom(n, n, n, 64.0, 64.0, 64.0, 1, 21, 15, 300.0, 450.0, 2.0)
+    p = O.paras(g)
+    assert p.fov_r < 20.0
+    vol = np.full((n, n, n), 1.5, dtype=np.float32)
+    L = O.march_f32(g, p, [vol])[0, 0]
+    assert L[7, 10] == pytest.approx(p.n_steps * p.dx * 1.5, rel=1e-6)
+    assert L[7, 10] == pytest.approx(2 * p.fov_r * 1.5, abs=p.dx * 1.5)
+
+
+def test_sphere_chords():
+    n, r = 64, 20.0
+    g = O.geom(n, n, n, 64.0, 64.0, 64.0, 3, 35, 25, 300.0, 450.0, 4.0)
+    p = O.paras(g)
+    assert p.fov_r > r + 2
+    vol = _sphere(n, r, (32.0, 32.0, 32.0), rho=1.5)
+    L = O.march_f32(g, p, [vol])[:, 0]
+    # voxel i sits at physical i*p - len/2, so index 32 is the origin and the
+    # sphere centre is on the rotation axis
+    for view in range(3):
+        th = O.view_angle(g, p, view)
+        for v in range(g.nv):
+            for u in range(g.nu):
+                d = O.ray_direction_f64(g, p, u, v)
+                # object frame: x = X cos - Y sin, y = -(X sin + Y cos), z = -Z
+                # with (X, Y, Z) = t d - (SOD, 0, 0)
+                c, s = math.cos(th), math.sin(th)
+                p0 = np.array([-g.sod * c, g.sod * s, 0.0])
+                dirn = np.array([d[0] * c - d[1] * s, -(d[0] * s + d[1] * c), -d[2]])
+                b = np.linalg.norm(np.cross(-p0, dirn))
+                chord = 2 * math.sqrt(max(r * r - b * b, 0.0)) * 1.5
+                if b < r - 2.0:
+                    assert abs(L[view, v, u] - chord) < 1.6, (view, u, v)
+                elif b > r + 1.8:
+                    assert L[view, v, u] == 0.0
+
+
+def test_linearity_and_superposition():
+    g = small_geom()
+    p = O.paras(g)
+    rng = np.random.default_rng(1)
+    a = rng.uniform(0, 2, size=(32, 32, 32)).astype(np.float32)
+    b = rng.uniform(0, 1, size=(32, 32, 32)).astype(np.float32)
+    La = O.march_f32(g, p, [a])
+    np.testing.assert_array_equal(O.march_f32(g, p, [2 * a]), 2 * La)   # exact scaling
+    Lab = O.march_f32(g, p, [a + b])
+    Lsum = La + O.march_f32(g, p, [b])
+    np.testing.assert_allclose(Lab, Lsum, rtol=2e-5, atol=1e-4)
+    L2 = O.march_f32(g, p, [a, b])   # two channels = two single marches
+    np.testing.assert_array_equal(L2[:, 0], La[:, 0])
+
+
+def test_quarter_turn_symmetry():
+    # A disk centred on the rotation axis (index n/2) is invariant under
+    # quarter turns of the voxel grid, so views 0, 90, 180, 270 deg agree.
+    n = 32
+    y, x = np.meshgrid(np.arange(n) - n / 2, np.arange(n) - n / 2, indexing="ij")
+    vol = np.zeros((n, n, n), dtype=np.float32)
+    vol[6:26][:, x * x + y * y <= 11.0 ** 2] = 1.0
+    g = small_geom(views=4)
+    p = O.paras(g)
+    L = O.march_f32(g, p, [vol])[:, 0]
+    for v in range(1, 4):
+        np.testing.assert_allclose(L[v], L[0], rtol=1e-5, atol=2e-4)
+
+
+def test_f32_mirror_close_to_f64():
+    s = S.config1()
+    g = O.geom_from_setup(s, views=2)
+    p = O.paras(g)
+    vol = S.cylinder_phantom()
+    L32 = O.march_f32(g, p, [vol])
+    L64 = O.march_f64(g, p, [vol])
+    assert np.abs(L32 - L64).max() <= 1e-5 * L64.max()
+
+
+# ---------------------------------------------------------------- spectrum
+
+def test_monoenergetic_beer_lambert():
+    rng = np.random.default_rng(2)
+    L = rng.uniform(0, 30, size=(2, 1, 4, 5)).astype(np.float32)
+    a = np.array([[0.0206]], dtype=np.float32)
+    T = rng.uniform(0.6, 0.9, size=(1, 5)).astype(np.float32)
+    eta = rng.uniform(0.5, 0.99, size=(1, 20)).astype(np.float32)
+    I, IA, I0 = O.spectrum_f32(L, a, np.array([1.0], np.float32), 6.0e4, T, eta)
+    w = 6.0e4 * T[0][None, :] * eta[0].reshape(4, 5)
+    ex = np.exp(-0.0206 * L[:, 0].astype(np.float64))
+    np.testing.assert_allclose(I, w * ex, rtol=2e-6)
+    np.testing.assert_allclose(IA, w * (1 - ex), rtol=2e-5)
+    np.testing.assert_allclose(I0, w, rtol=1e-6)
+
+
+def test_flat_field_and_conservation():
+    s = S.config2()
+    spec = s.spectrum()
+    assert spec.sum() == pytest.approx(1.0, abs=1e-6)
+    assert spec[:19].sum() == 0.0 and np.count_nonzero(spec) == 101
+    atten = s.atten()
+    L = np.zeros((1, 3, 2, 3), dtype=np.float32)
+    I, IA, I0 = O.spectrum_f32(L, atten, spec, 1.0e5)
+    np.testing.assert_allclose(I[0], I0, rtol=1e-6)
+    assert np.all(IA == 0)
+    assert I0[0, 0] == pytest.approx(1.0e5, rel=1e-5)   # sum S = 1, no grid / response
+    rng = np.random.default_rng(3)
+    L = rng.uniform(0, 50, size=(2, 3, 2, 3)).astype(np.float32) * np.float32([1, 0.1, 0.01])[None, :, None, None]
+    I, IA, I0 = O.spectrum_f32(L, atten, spec, 1.0e5)
+    np.testing.assert_allclose(I + IA, np.broadcast_to(I0, I.shape), rtol=1e-5)
+
+
+def test_spectrum_f32_matches_f64():
+    s = S.config2()
+    rng = np.random.default_rng(4)
+    L = (rng.uniform(0, 60, size=(2, 3, 3, 4)) * np.array([1, 0.2, 0.05])[None, :, None, None]).astype(np.float32)
+    I, IA, I0 = O.spectrum_f32(L, s.atten(), s.spectrum(), 1.0e5)
+    I64, IA64, I064 = O.spectrum_f64(L, s.atten(), s.spectrum(), 1.0e5)
+    np.testing.assert_allclose(I, I64, rtol=2e-5)
+    np.testing.assert_allclose(I0, I064, rtol=2e-6)
+
+
+def test_literal_method_equals_decomposition():
+    """The reference re-marches a mu(E) volume per energy; the decomposition
+    marches each material once.  Both must agree (linearity of the filter and
+    the sum), including labels above MaterialNum being empty."""
+    n = 24
+    labels = S.label_phantom(n, 3, seed=11)
+    assert (labels == 4).any()
+    density = np.array([1.0, 1.8, 2.699], np.float32)
+    g = O.geom(n, n, n, 24.0, 24.0, 24.0, 3, 16, 12, 300.0, 450.0, 1.8)
+    p = O.paras(g)
+    ne = 8
+    e = S.bin_energies(40)[-ne:]
+    mass = np.stack([S.WATER.mass_atten(e), S.BONE.mass_atten(e), S.ALUMINIUM.mass_atten(e)], 1)
+    mass = mass.astype(np.float32)
+    spec = np.random.default_rng(5).uniform(0, 1, ne).astype(np.float32)
+    spec /= spec.sum()
+    T = np.random.default_rng(6).uniform(0.7, 0.9, (ne, 16)).astype(np.float32)
+    eta = np.random.default_rng(7).uniform(0.6, 0.95, (ne, 16 * 12)).astype(np.float32)
+    I_lit, IA_lit, I0_lit = O.forward_literal_f32(g, p, labels, density, mass, spec, 6.0e4, T, eta)
+    ch = O.labels_to_density_channels(labels, density)
+    Lm = O.march_f32(g, p, list(ch))
+    I, IA, I0 = O.spectrum_f32(Lm, mass / np.float32(10), spec, 6.0e4, T, eta)
+    np.testing.assert_allclose(I, I_lit, rtol=2e-5)
+    np.testing.assert_allclose(IA, IA_lit, rtol=2e-4, atol=1e-2)
+    np.testing.assert_array_equal(I0, I0_lit)
+
+
+def test_label_channels():
+    lab = np.array([0, 1, 2, 3, 4, 200], dtype=np.uint8)
+    ch = O.labels_to_density_channels(lab, np.array([1.0, 2.0, 3.0], np.float32))
+    assert ch.tolist() == [[0, 1, 0, 0, 0, 0], [0, 0, 2, 0, 0, 0], [0, 0, 0, 3, 0, 0]]
+    assert O.labels_to_density_single(lab, 1.5).tolist() == [0, 1.5, 3.0, 4.5, 6.0, 300.0]
+
+
+# -------------------------------------------------------------------- grid
+
+def test_grid_transparent():
+    for nu, d, w, sw in [(8, 250, 214, 36), (7, 125, 89, 36), (8, 125, 89, 36), (7, 125, 90, 35)]:
+        T = O.grid_transmission(nu, d, w, sw, 2.0, 1500.0, 0.0, 0.0)
+        assert np.all(T == 1.0)
+
+
+@pytest.mark.parametrize("nu,d,w,sw", [(8, 250, 214, 36),   # G even, w even
+                                        (8, 125, 89, 36),    # G even, w odd
+                                        (7, 125, 90, 35),    # G odd, w even
+                                        (7, 125, 89, 36),    # G odd, w odd
+                                        (6, 500, 200, 50)])  # two periods per pixel
+def test_grid_periodic_closed_form(nu, d, w, sw):
+    # With a focal distance so large that every lamella has height h, each
+    # pixel spanning whole periods transmits (w e^-mu_i h + sw e^-mu_s h) / P.
+    h, mus, mui = 1.5, 25.0, 0.7
+    T = O.grid_transmission(nu, d, w, sw, h, 1.0e9, mus, mui)
+    P = w + sw
+    assert d % P == 0
+    expect = (w * math.exp(-mui / 1000 * h * 1000) + sw * math.exp(-mus / 1000 * h * 1000)) / P
+    np.testing.assert_allclose(T, expect, rtol=2e-5)
+
+
+def test_grid_unit_heights_and_symmetry():
+    H = O.grid_unit_heights(2000, 2.14, 1500.0)
+    i = np.arange(2000)
+    dc = i + 0.5 - 1000.0
+    np.testing.assert_allclose(H, 2140.0 * np.sqrt(1.5e6 ** 2 + dc ** 2) / 1.5e6, rtol=1e-6)
+    T = O.grid_transmission(64, 400, 214, 36, 2.14, 1500.0, 57.0, 0.0)
+    np.testing.assert_allclose(T, T[::-1], rtol=1e-6)       # uEdE is mirror symmetric
+    # opaque strips (e^-122): a 400-um pixel holds one or two 36-um strips
+    np.testing.assert_allclose(np.where(T > 0.86, T - (1 - 36 / 400), T - (1 - 72 / 400)), 0,
+                               atol=1e-6)
+    assert T.mean() == pytest.approx(1 - 36 / 250, abs=0.01)
+    # 1-um pixels expose the units: strips carry their own slant height
+    H4 = O.grid_unit_heights(4000, 2.14, 1500.0)
+    T1 = O.grid_transmission(4000, 1, 214, 36, 2.14, 1.0e9, 0.5, 0.0)
+    strip = T1 < 1.0
+    assert strip.sum() == 36 * 16 and (~strip).sum() == 4000 - 36 * 16
+    T1 = O.grid_transmission(4000, 1, 214, 36, 2.14, 1500.0, 0.5, 0.0)
+    np.testing.assert_allclose(T1[strip], np.exp(-0.5 / 1000 * H4[strip].astype(np.float64)),
+                               rtol=1e-6)
+
+
+# -------------------------------------------------------------- post step
+
+def test_postprocess_floor_and_log():
+    I = np.array([[0.0, 0.99, 1.0, 1.5, 7.999, 123456.7]], dtype=np.float32)
+    IA = np.array([[3.7, 0.0, 2.2, 9.9, 0.5, 1.0]], dtype=np.float32)
+    I0 = np.full(6, 1000.0, dtype=np.float32)
+    Iu, IAu, proj = O.postprocess(I, IA, I0)
+    assert Iu.tolist() == [[0, 0, 1, 1, 7, 123456]]
+    assert IAu.tolist() == [[3, 0, 2, 9, 0, 1]]
+    assert np.isinf(proj[0, 0]) and proj[0, 0] > 0
+    np.testing.assert_allclose(proj[0, 2:], np.log(1000.0 / Iu[0, 2:].astype(np.float64)), rtol=1e-6)
