@@ -54,12 +54,39 @@ def cpu_baseline(setup, vols, kw, target_s: float):
                       f"march + spectrum, {secs:.1f} s on {threads} threads"}
 
 
+def accuracy(proj, setup, vols, kw):
+    """Max relative error of view 0 against the FP64 evaluation of the method
+    (oracle, CPU): p = ln(I0/I) floored at 1e-2 max|p|, I floored at 1e-3 max."""
+    import numpy as np
+    import torch
+    from oracle import oracle as O
+    I, IA = proj.forward(0, 1)
+    I0 = proj.flat_field()
+    torch.cuda.synchronize()
+    I, I0 = I.cpu().numpy().astype(np.float64), I0.cpu().numpy().astype(np.float64)
+    g = O.geom_from_setup(setup)
+    p = O.paras(g)
+    L64 = O.march_f64(g, p, vols, 0, 1, threads=O.num_threads())
+    det = np.stack([O.det_response(1.0, float(m), kw["scint_thickness"]) for m in kw["scint_mu"]])
+    I64, _, I064 = O.spectrum_f64(L64, kw["atten"], kw["spectrum"], kw["i0_val"], kw["grid"], det)
+
+    def rel(a, b, f):
+        return float((np.abs(a - b) / np.maximum(np.abs(b), f * np.abs(b).max())).max())
+
+    pg, pr = np.log(I0[None] / I), np.log(I064[None] / I64)
+    return {"p": rel(pg, pr, 1e-2), "I": rel(I, I64, 1e-3), "sample": "view 0 of 360 vs FP64 oracle"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--views", type=int, default=0, help="views per step (0 = the full 360)")
+    ap.add_argument("--dtype", choices=["f32", "f16", "bf16"], default="f32",
+                    help="storage precision of the resident volumes")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the one-view error check against the FP64 oracle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -81,8 +108,9 @@ def main():
     vols = list(S.three_material_phantom(512))
     kw = P.spectral_inputs(setup, *setup.det)
     log(f"inputs ready in {time.perf_counter() - t:.1f} s")
+    dtypes = {"f32": P.CTFP_F32, "f16": P.CTFP_F16, "bf16": P.CTFP_BF16}
     proj = P.Projector(P.scan_params(**setup.scan_kwargs()))
-    proj.set_volumes(vols)
+    proj.set_volumes(vols, storage_dtype=dtypes[args.dtype])
     proj.set_spectral(**kw)
     views = setup.views if args.views <= 0 else min(args.views, setup.views)
     nu, nv = setup.det
@@ -92,7 +120,8 @@ def main():
     stream = torch.cuda.current_stream()
     samples = views * proj.samples_per_view
     n_mat = len(vols)
-    bytes_per_sample = 4 * n_mat  # one FP32 voxel-equivalent per material per sample
+    # one voxel-equivalent per material per sample, in the stored precision
+    bytes_per_sample = (4 if args.dtype == "f32" else 2) * n_mat
 
     if args.calibrate_copy_mib > 0:
         nb = args.calibrate_copy_mib << 20
@@ -103,23 +132,26 @@ def main():
         torch.cuda.synchronize()
         log(f"copy kernel: {nb} bytes read, {nb} bytes written")
 
-    for i in range(args.warmup):
-        proj.forward(0, views, I, IA, stream=stream)
+    def timed(pr, label):
+        for i in range(args.warmup):
+            pr.forward(0, views, I, IA, stream=stream)
+            torch.cuda.synchronize()
+            log(f"{label}: warmup {i + 1}/{args.warmup} done")
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
         torch.cuda.synchronize()
-        log(f"warmup {i + 1}/{args.warmup} done")
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            starts[i].record(stream)
+            pr.forward(0, views, I, IA, stream=stream)
+            ends[i].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kern = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+        log(f"{label}: timed {args.steps} steps, wall {wall:.3f} s, per-launch ms {kern}")
+        return wall, kern
 
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        starts[i].record(stream)
-        proj.forward(0, views, I, IA, stream=stream)
-        ends[i].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
-    log(f"timed {args.steps} steps: wall {wall:.3f} s, per-launch ms {kern_ms}")
+    wall, kern_ms = timed(proj, args.dtype)
     ms_per_step = wall * 1e3 / args.steps
     gsps = samples / (ms_per_step / 1e3) / 1e9
     avg_launch_s = sum(kern_ms) / len(kern_ms) / 1e3
@@ -134,18 +166,28 @@ def main():
     result = {
         "metric": METRIC, "value": round(gsps, 3), "unit": "Gsamples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.dtype,
         "data": "synthetic",
         "config": {"workload": "3-material 512^3 FP32 (water/bone/iodine, 0.5 mm), 1024x768 "
                                "@ 0.4 mm, 360 views, K=1082, 120 kVp 101 bins, Pb grid 10:1 "
                                "40 l/cm, CsI response",
                    "views_per_step": views, "samples_per_step": samples,
-                   "materials": n_mat, "volume_dtype": "f32"},
+                   "materials": n_mat, "volume_dtype": args.dtype},
         "views_per_s": round(views / (ms_per_step / 1e3), 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
     }
+    if args.dtype != "f32":
+        ref = P.Projector(P.scan_params(**setup.scan_kwargs()))
+        ref.set_volumes(vols, storage_dtype=P.CTFP_F32)
+        ref.set_spectral(**kw)
+        wall32, _ = timed(ref, "f32")
+        result["fp32_value"] = round(samples / (wall32 / args.steps) / 1e9, 3)
+        result["speedup_vs_fp32"] = round(result["value"] / result["fp32_value"], 3)
+        ref.close()
+    if not args.no_accuracy:
+        result["rel_err_vs_fp64"] = accuracy(proj, setup, vols, kw)
     del I, IA
     proj.close()
     if not args.no_cpu_baseline:

@@ -81,7 +81,7 @@ class Spectral(C.Structure):
 class GridParams(C.Structure):
     """ctfp_grid_params; mirrors GridAndDetectorSystem (Source/Grid.h:42-66)."""
     _fields_ = [
-        ("d_num", C.c_int64), ("d_size_mm", C.c_float),
+        ("d_num", C.c_int64), ("d_size_um", C.c_float),
         ("interspace_width", C.c_int32), ("strip_width", C.c_int32),
         ("height_mm", C.c_float), ("focal_mm", C.c_float),
     ]

@@ -2,8 +2,9 @@
 //
 // Data layout in HBM
 //   volume : the n_mat density channels interleaved per voxel ([z][y][x][m]),
-//            padded with one zero voxel below and two above on every axis so the
-//            8-corner gather of any clamped index in [-1, N] stays in bounds;
+//            padded with two zero voxels on both sides of every axis so the
+//            8-corner gather of any integer cell index clamped to [-2, N]
+//            stays in bounds and reads only zeros outside the support;
 //            read with raw buffer loads (one VGPR byte offset per sample, the
 //            three other corner rows as uniform SGPR offsets).
 //   weights: W[e][v][u] = I0Val * S_e * T_e(u) * eta_e(v,u) for the active
@@ -14,9 +15,13 @@
 // (CBCTPolyProjGPU.cu:1846-1848) with zero-padded trilinear interpolation
 // (CBCTPolyProjGPU.cu:1920, 2200-2205).  Per-ray geometry is set up in FP64:
 // the sample index position is affine in k, so each ray carries its FP64-derived
-// start and step (FP32) and its exact support interval [k_lo, k_hi) where any
-// channel can be non-zero; samples outside it are exact zeros in the reference
-// and are skipped.
+// start and step and its support interval [k_lo, k_hi) where any channel can
+// be non-zero; samples outside it are exact zeros in the reference and are
+// skipped.  The sample position itself stays in FP64 (FP64 FMA runs at the
+// unpacked FP32 rate on CDNA): an FP32 index near 512 cannot hold better than
+// 3e-5 voxel, which shows up as ~1e-4 relative error on grazing rays.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -112,67 +117,73 @@ template <typename T> void dfree(T *&p)
 
 // ------------------------------------------------------------- kernels ---
 
-constexpr int kVoxPadLo = 1; // zero voxels below index 0 on each axis
+constexpr int kVoxPadLo = 2; // zero voxels below index 0 on each axis
 constexpr int kVoxPadHi = 2; // zero voxels above index N-1 on each axis
 
-// Scatter n_mat FP32 volumes [m][z][y][x] into the padded interleaved layout.
-__global__ void pad_interleave_f32(const float *__restrict__ src, float *__restrict__ dst,
-                                   int n_mat, int64_t nx, int64_t ny, int64_t nz, int64_t px,
-                                   int64_t py, int64_t pz)
-{
-    const int64_t total = px * py * pz;
-    const int64_t nvox = nx * ny * nz;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t x = i % px - kVoxPadLo;
-        int64_t y = (i / px) % py - kVoxPadLo;
-        int64_t z = i / (px * py) - kVoxPadLo;
-        bool in = x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz;
-        int64_t s = (z * ny + y) * nx + x;
-        for (int m = 0; m < n_mat; m++) dst[i * n_mat + m] = in ? src[m * nvox + s] : 0.0f;
-    }
-}
-
-// Same from an array of per-material device pointers.
+// Per-material FP32 source volumes [z][y][x] (device pointers).
 struct VolPtrs {
     const float *p[4];
 };
-__global__ void pad_interleave_ptrs_f32(VolPtrs src, float *__restrict__ dst, int n_mat,
-                                        int64_t nx, int64_t ny, int64_t nz, int64_t px,
-                                        int64_t py, int64_t pz)
+
+__device__ __forceinline__ float src_voxel(const VolPtrs &src, int m, int64_t x, int64_t y, int64_t z,
+                                           int64_t nx, int64_t ny, int64_t nz)
+{
+    if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return 0.0f;
+    return src.p[m][(z * ny + y) * nx + x];
+}
+
+// FP32 resident layout: channels interleaved per voxel, zero padding.
+__global__ void layout_voxels_f32(VolPtrs src, float *__restrict__ dst, int n_mat, int64_t nx,
+                                  int64_t ny, int64_t nz, int64_t px, int64_t py, int64_t pz)
 {
     const int64_t total = px * py * pz;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
          i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t x = i % px - kVoxPadLo;
-        int64_t y = (i / px) % py - kVoxPadLo;
-        int64_t z = i / (px * py) - kVoxPadLo;
-        bool in = x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz;
-        int64_t s = (z * ny + y) * nx + x;
-        for (int m = 0; m < n_mat; m++) dst[i * n_mat + m] = in ? src.p[m][s] : 0.0f;
+        int64_t x = i % px - kVoxPadLo, y = (i / px) % py - kVoxPadLo, z = i / (px * py) - kVoxPadLo;
+        for (int m = 0; m < n_mat; m++) dst[i * n_mat + m] = src_voxel(src, m, x, y, z, nx, ny, nz);
     }
 }
 
-// Density channels straight from the label volume into the padded layout.
-struct Densities {
-    float d[4];
-};
-__global__ void labels_to_channels(const uint8_t *__restrict__ labels, float *__restrict__ dst,
-                                   int n_mat, int single_material, Densities rho, int64_t nx,
-                                   int64_t ny, int64_t nz, int64_t px, int64_t py, int64_t pz)
+// 16-bit resident layout: cell (x, y, z) holds, per channel, the pair
+// (v(x), v(x+1)) in one dword, so one row load serves both x corners.
+template <bool kBF16>
+__global__ void layout_pairs_16(VolPtrs src, uint32_t *__restrict__ dst, int n_mat, int64_t nx,
+                                int64_t ny, int64_t nz, int64_t px, int64_t py, int64_t pz)
 {
     const int64_t total = px * py * pz;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
          i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t x = i % px - kVoxPadLo;
-        int64_t y = (i / px) % py - kVoxPadLo;
-        int64_t z = i / (px * py) - kVoxPadLo;
-        bool in = x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz;
-        unsigned lab = in ? labels[(z * ny + y) * nx + x] : 0u;
+        int64_t x = i % px - kVoxPadLo, y = (i / px) % py - kVoxPadLo, z = i / (px * py) - kVoxPadLo;
+        for (int m = 0; m < n_mat; m++) {
+            float lo = src_voxel(src, m, x, y, z, nx, ny, nz);
+            float hi = src_voxel(src, m, x + 1, y, z, nx, ny, nz);
+            uint32_t bits;
+            if constexpr (kBF16) {
+                bits = (uint32_t)__bfloat16_as_ushort(__float2bfloat16(lo)) |
+                       ((uint32_t)__bfloat16_as_ushort(__float2bfloat16(hi)) << 16);
+            } else {
+                __half2 h = __floats2half2_rn(lo, hi);
+                bits = __builtin_bit_cast(uint32_t, h);
+            }
+            dst[i * n_mat + m] = bits;
+        }
+    }
+}
+
+// Density channels [m][z][y][x] from a label volume.
+struct Densities {
+    float d[4];
+};
+__global__ void labels_to_raw(const uint8_t *__restrict__ labels, float *__restrict__ raw, int n_mat,
+                              int single_material, Densities rho, int64_t nvox)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nvox;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        unsigned lab = labels[i];
         if (single_material) {
-            dst[i] = (float)lab * rho.d[0];
+            raw[i] = (float)lab * rho.d[0];
         } else {
-            for (int m = 0; m < n_mat; m++) dst[i * n_mat + m] = (lab == (unsigned)(m + 1)) ? rho.d[m] : 0.0f;
+            for (int m = 0; m < n_mat; m++) raw[m * nvox + i] = (lab == (unsigned)(m + 1)) ? rho.d[m] : 0.0f;
         }
     }
 }
@@ -207,7 +218,7 @@ __global__ void build_weights(WeightArgs a, float *__restrict__ W)
     }
 }
 
-// I0[v][u] = sum_e W[e][v][u] in bin order (FP32).
+// I0[v][u] = sum_e W[e][v][u] in bin order.
 __global__ void flat_field_kernel(const float *__restrict__ W, int n_act, int64_t stride_e,
                                   int stride_v, int64_t nu, int64_t nv, float *__restrict__ I0)
 {
@@ -215,9 +226,9 @@ __global__ void flat_field_kernel(const float *__restrict__ W, int n_act, int64_
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix;
          p += (int64_t)gridDim.x * blockDim.x) {
         int64_t u = p % nu, v = p / nu;
-        float acc = 0.0f;
-        for (int e = 0; e < n_act; e++) acc += W[e * stride_e + v * stride_v + u];
-        I0[p] = acc;
+        double acc = 0.0; // bin order, FP64 like the per-ray sums
+        for (int e = 0; e < n_act; e++) acc += (double)W[e * stride_e + v * stride_v + u];
+        I0[p] = (float)acc;
     }
 }
 
@@ -277,12 +288,6 @@ constexpr int kWaveU = 8, kWaveV = 8;  // pixels per wavefront
 constexpr int kTileU = 16, kTileV = 16; // pixels per workgroup (4 wavefronts)
 constexpr int kBlock = 256;
 
-template <int M> struct VoxTraits;
-template <> struct VoxTraits<1> { static constexpr int bytes = 4; };
-template <> struct VoxTraits<2> { static constexpr int bytes = 8; };
-template <> struct VoxTraits<3> { static constexpr int bytes = 12; };
-template <> struct VoxTraits<4> { static constexpr int bytes = 16; };
-
 template <int M>
 __device__ __forceinline__ void load_vox(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
                                          float (&c)[M])
@@ -322,7 +327,7 @@ __device__ __forceinline__ void clip_axis(double s, double d, double n, double &
 // Ray (u, v) of view `vw`: FP64 set-up of the index-space start (at k_lo) and
 // step, and the sample range [k_lo, k_hi).
 struct RaySetup {
-    float a0, b0, c0, da, db, dc;
+    double a0, b0, c0, da, db, dc;
     int k_lo, n;
 };
 
@@ -362,16 +367,69 @@ __device__ __forceinline__ RaySetup setup_ray(const MarchArgs &P, int u, int v, 
     }
     R.k_lo = k_lo;
     R.n = max(0, k_hi - k_lo);
-    R.a0 = (float)(sa + da * k_lo);
-    R.b0 = (float)(sb2 + db * k_lo);
-    R.c0 = (float)(sc + dc * k_lo);
-    R.da = (float)da;
-    R.db = (float)db;
-    R.dc = (float)dc;
+    R.a0 = sa + da * k_lo;
+    R.b0 = sb2 + db * k_lo;
+    R.c0 = sc + dc * k_lo;
+    R.da = da;
+    R.db = db;
+    R.dc = dc;
     return R;
 }
 
-template <int M, bool kSpectrum>
+// Storage of the resident volume: FP32 voxels, or FP16 / BF16 x-pairs.
+enum Storage { kStF32 = 0, kStF16 = 1, kStBF16 = 2 };
+
+template <int M>
+__device__ __forceinline__ void load_raw(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
+                                         uint32_t (&c)[M])
+{
+    if constexpr (M == 1) {
+        c[0] = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+    } else if constexpr (M == 2) {
+        auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        c[0] = v[0];
+        c[1] = v[1];
+    } else if constexpr (M == 3) {
+        auto v = __builtin_amdgcn_raw_buffer_load_b96(r, voff, soff, 0);
+        c[0] = v[0];
+        c[1] = v[1];
+        c[2] = v[2];
+    } else {
+        auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+        c[0] = v[0];
+        c[1] = v[1];
+        c[2] = v[2];
+        c[3] = v[3];
+    }
+}
+
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef short bf16x2_t __attribute__((ext_vector_type(2)));
+
+// acc += w.x * lo(d) + w.y * hi(d), FP32 accumulate (v_dot2_f32_f16 / _bf16).
+template <int ST>
+__device__ __forceinline__ float dot2_acc(uint32_t w, uint32_t d, float acc)
+{
+    if constexpr (ST == kStF16)
+        return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, w), __builtin_bit_cast(half2_t, d), acc,
+                                      false);
+    else
+        return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2_t, w),
+                                               __builtin_bit_cast(bf16x2_t, d), acc, false);
+}
+
+// Round-to-nearest pack of two FP32 weights into the storage's 16-bit pair.
+template <int ST> __device__ __forceinline__ uint32_t pack_w(float lo, float hi)
+{
+    if constexpr (ST == kStF16) {
+        return __builtin_bit_cast(uint32_t, __floats2half2_rn(lo, hi));
+    } else {
+        return (uint32_t)__bfloat16_as_ushort(__float2bfloat16(lo)) |
+               ((uint32_t)__bfloat16_as_ushort(__float2bfloat16(hi)) << 16);
+    }
+}
+
+template <int M, int ST, bool kSpectrum>
 __global__ __launch_bounds__(kBlock) void march_kernel(MarchArgs P)
 {
     // XCD-aware block order: consecutive logical blocks (adjacent tiles of one
@@ -393,53 +451,83 @@ __global__ __launch_bounds__(kBlock) void march_kernel(MarchArgs P)
     if (active) {
         R = setup_ray(P, u, v, vw);
     } else {
-        R.a0 = R.b0 = R.c0 = R.da = R.db = R.dc = 0.0f;
+        R.a0 = R.b0 = R.c0 = R.da = R.db = R.dc = 0.0;
         R.k_lo = 0;
         R.n = 0;
     }
 
     __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(P.vol), 0, (int)P.vol_bytes, 0x00020000);
-    constexpr uint32_t VB = VoxTraits<M>::bytes;
-    const uint32_t oy = P.row_y, oz = P.row_z, oyz = P.row_y + P.row_z;
-    const uint32_t ry = P.row_y / VB; // voxels per row
-    const uint32_t rz = P.row_z / P.row_y; // rows per slice
+    constexpr uint32_t CB = 4u * M; // bytes per voxel (FP32) or per x-pair cell (16-bit)
+    const uint32_t oy = P.row_y & 0xFFFFFFu, oz = P.row_z & 0xFFFFFFu, oyz = oy + oz;
 
     float acc[M];
 #pragma unroll
     for (int m = 0; m < M; m++) acc[m] = 0.0f;
 
-    float jf = 0.0f;
-    for (int j = 0; j < R.n; j++, jf += 1.0f) {
-        float fa = fminf(fmaxf(fmaf(jf, R.da, R.a0), -1.0f), P.nx);
-        float fb = fminf(fmaxf(fmaf(jf, R.db, R.b0), -1.0f), P.ny);
-        float fc = fminf(fmaxf(fmaf(jf, R.dc, R.c0), -1.0f), P.nz);
-        float ia = floorf(fa), ib = floorf(fb), ic = floorf(fc);
-        float ax = fa - ia, ay = fb - ib, az = fc - ic;
-        uint32_t xi = (uint32_t)((int)ia + kVoxPadLo);
-        uint32_t yi = (uint32_t)((int)ib + kVoxPadLo);
-        uint32_t zi = (uint32_t)((int)ic + kVoxPadLo);
-        // 24-bit multiplies (full rate); every factor is < 2^24 and the byte
-        // offset < 2^32 (checked when the volume is allocated).
-        uint32_t voff = __umul24(__umul24(__umul24(zi, rz) + yi, ry) + xi, VB);
-        float c000[M], c100[M], c010[M], c110[M], c001[M], c101[M], c011[M], c111[M];
-        load_vox<M>(rsrc, voff, 0, c000);
-        load_vox<M>(rsrc, voff + VB, 0, c100);
-        load_vox<M>(rsrc, voff, oy, c010);
-        load_vox<M>(rsrc, voff + VB, oy, c110);
-        load_vox<M>(rsrc, voff, oz, c001);
-        load_vox<M>(rsrc, voff + VB, oz, c101);
-        load_vox<M>(rsrc, voff, oyz, c011);
-        load_vox<M>(rsrc, voff + VB, oyz, c111);
+    const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
+    double jd = 0.0;
+    for (int j = 0; j < R.n; j++, jd += 1.0) {
+        // FP64 position; the integer cell is clamped to [-2, N] where every
+        // corner outside the volume is a padding zero, so clamped samples are 0.
+        double fa = fma(jd, R.da, R.a0), fb = fma(jd, R.db, R.b0), fc = fma(jd, R.dc, R.c0);
+        double ia = floor(fa), ib = floor(fb), ic = floor(fc);
+        float ax = (float)(fa - ia), ay = (float)(fb - ib), az = (float)(fc - ic);
+        int xi0 = min(max((int)ia, -2), nx), yi0 = min(max((int)ib, -2), ny), zi0 = min(max((int)ic, -2), nz);
+        // Byte offset with full-rate 24-bit multiply-adds: the masks (no-ops on
+        // the clamped cells) tell the compiler every factor is < 2^24; the
+        // strides are < 2^24 and the offset < 2^32 (checked at allocation).
+        uint32_t xi = (uint32_t)(xi0 + kVoxPadLo) & 0xFFFFFu;
+        uint32_t yi = (uint32_t)(yi0 + kVoxPadLo) & 0xFFFFFu;
+        uint32_t zi = (uint32_t)(zi0 + kVoxPadLo) & 0xFFFFFu;
+        uint32_t voff = zi * oz + (yi * oy + xi * CB);
+        const float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
+        const float w00 = by * bz, w10 = ay * bz, w01 = by * az, w11 = ay * az;
+        if constexpr (ST == kStF32) {
+            float c000[M], c100[M], c010[M], c110[M], c001[M], c101[M], c011[M], c111[M];
+            load_vox<M>(rsrc, voff, 0, c000);
+            load_vox<M>(rsrc, voff + CB, 0, c100);
+            load_vox<M>(rsrc, voff, oy, c010);
+            load_vox<M>(rsrc, voff + CB, oy, c110);
+            load_vox<M>(rsrc, voff, oz, c001);
+            load_vox<M>(rsrc, voff + CB, oz, c101);
+            load_vox<M>(rsrc, voff, oyz, c011);
+            load_vox<M>(rsrc, voff + CB, oyz, c111);
+            // trilinear weights, accumulated straight into the per-material sums
 #pragma unroll
-        for (int m = 0; m < M; m++) {
-            float e00 = fmaf(ax, c100[m] - c000[m], c000[m]);
-            float e10 = fmaf(ax, c110[m] - c010[m], c010[m]);
-            float e01 = fmaf(ax, c101[m] - c001[m], c001[m]);
-            float e11 = fmaf(ax, c111[m] - c011[m], c011[m]);
-            float e0 = fmaf(ay, e10 - e00, e00);
-            float e1 = fmaf(ay, e11 - e01, e01);
-            acc[m] += fmaf(az, e1 - e0, e0);
+            for (int m = 0; m < M; m++) {
+                float a = acc[m];
+                a = fmaf(bx * w00, c000[m], a);
+                a = fmaf(ax * w00, c100[m], a);
+                a = fmaf(bx * w10, c010[m], a);
+                a = fmaf(ax * w10, c110[m], a);
+                a = fmaf(bx * w01, c001[m], a);
+                a = fmaf(ax * w01, c101[m], a);
+                a = fmaf(bx * w11, c011[m], a);
+                a = fmaf(ax * w11, c111[m], a);
+                acc[m] = a;
+            }
+        } else {
+            // four row loads, each holding (v(x), v(x+1)) per channel; one
+            // dot2 per row and channel with the rounded weight pair
+            uint32_t r00[M], r10[M], r01[M], r11[M];
+            load_raw<M>(rsrc, voff, 0, r00);
+            load_raw<M>(rsrc, voff, oy, r10);
+            load_raw<M>(rsrc, voff, oz, r01);
+            load_raw<M>(rsrc, voff, oyz, r11);
+            const uint32_t p00 = pack_w<ST>(bx * w00, ax * w00);
+            const uint32_t p10 = pack_w<ST>(bx * w10, ax * w10);
+            const uint32_t p01 = pack_w<ST>(bx * w01, ax * w01);
+            const uint32_t p11 = pack_w<ST>(bx * w11, ax * w11);
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                float a = acc[m];
+                a = dot2_acc<ST>(p00, r00[m], a);
+                a = dot2_acc<ST>(p10, r10[m], a);
+                a = dot2_acc<ST>(p01, r01[m], a);
+                a = dot2_acc<ST>(p11, r11[m], a);
+                acc[m] = a;
+            }
         }
     }
     if (!active) return;
@@ -454,33 +542,47 @@ __global__ __launch_bounds__(kBlock) void march_kernel(MarchArgs P)
         for (int m = 0; m < M; m++) P.L[((int64_t)vw_local * M + m) * npix + pix] = Lm[m];
     }
     if constexpr (kSpectrum) {
-        // Per-ray spectrum: tau_e = sum_m A[e][m] L_m, then the energy-ordered
-        // FP32 sums of W e^-tau and W (1 - e^-tau).
+        // Per-ray spectrum: tau_e = sum_m A[e][m] L_m (FP32), e^-tau and
+        // 1 - e^-tau (expm1, no cancellation at small tau) in FP32, and the
+        // energy sums of W e^-tau and W (1 - e^-tau) in bin order in FP64 (an
+        // FP32 sum of ~100 terms would cost ~1e-6 of I, visible in ln(I0/I)
+        // of thin rays; FP64 FMA runs at the FP32 rate).
         const float *wp = P.W + (int64_t)v * P.w_stride_v + u;
-        float sI = 0.0f, sA = 0.0f;
+        double sI = 0.0, sA = 0.0;
         for (int e = 0; e < P.n_act; e++) {
             float tau = 0.0f;
 #pragma unroll
             for (int m = 0; m < M; m++) tau = fmaf(P.A[e * M + m], Lm[m], tau);
             float ex = expf(-tau);
-            float w = wp[(int64_t)e * P.w_stride_e];
-            sI = fmaf(w, ex, sI);
-            sA = fmaf(w, 1.0f - ex, sA);
+            float ab = -expm1f(-tau);
+            double w = (double)wp[(int64_t)e * P.w_stride_e];
+            sI = fma(w, (double)ex, sI);
+            sA = fma(w, (double)ab, sA);
         }
-        P.I[(int64_t)vw_local * npix + pix] = sI;
-        P.IA[(int64_t)vw_local * npix + pix] = sA;
+        P.I[(int64_t)vw_local * npix + pix] = (float)sI;
+        P.IA[(int64_t)vw_local * npix + pix] = (float)sA;
     }
 }
 
 using MarchFn = void (*)(MarchArgs);
 
-MarchFn pick_march(int n_mat, bool spectrum)
+template <int ST> MarchFn pick_march_st(int n_mat, bool spectrum)
 {
     switch (n_mat) {
-    case 1: return spectrum ? march_kernel<1, true> : march_kernel<1, false>;
-    case 2: return spectrum ? march_kernel<2, true> : march_kernel<2, false>;
-    case 3: return spectrum ? march_kernel<3, true> : march_kernel<3, false>;
-    case 4: return spectrum ? march_kernel<4, true> : march_kernel<4, false>;
+    case 1: return spectrum ? march_kernel<1, ST, true> : march_kernel<1, ST, false>;
+    case 2: return spectrum ? march_kernel<2, ST, true> : march_kernel<2, ST, false>;
+    case 3: return spectrum ? march_kernel<3, ST, true> : march_kernel<3, ST, false>;
+    case 4: return spectrum ? march_kernel<4, ST, true> : march_kernel<4, ST, false>;
+    default: return nullptr;
+    }
+}
+
+MarchFn pick_march(int n_mat, int dtype, bool spectrum)
+{
+    switch (dtype) {
+    case CTFP_F32: return pick_march_st<kStF32>(n_mat, spectrum);
+    case CTFP_F16: return pick_march_st<kStF16>(n_mat, spectrum);
+    case CTFP_BF16: return pick_march_st<kStBF16>(n_mat, spectrum);
     default: return nullptr;
     }
 }
@@ -506,6 +608,8 @@ int alloc_volume(ctfp_context *ctx, int n_mat)
     int64_t py = s.p_num_y + kVoxPadLo + kVoxPadHi;
     int64_t pz = s.p_num_z + kVoxPadLo + kVoxPadHi;
     size_t bytes = (size_t)(px * py * pz) * (size_t)n_mat * sizeof(float);
+    if ((size_t)(px * py) * n_mat * sizeof(float) >= (1u << 24))
+        return set_error(CTFP_ERR_UNSUPPORTED, "one padded z slice exceeds 16 MiB");
     if (bytes >= 0xFFFFFFFFull)
         return set_error(CTFP_ERR_UNSUPPORTED,
                          "padded volume of %zu bytes exceeds the 4 GiB buffer-load range", bytes);
@@ -636,48 +740,67 @@ int ctfp_get_system_info(const ctfp_context *ctx, ctfp_system_info *out)
     return CTFP_OK;
 }
 
+// Writes the resident layout for `dtype` from n_mat FP32 device volumes.
+static int write_layout(ctfp_context *ctx, const VolPtrs &vp, int n_mat, int dtype)
+{
+    int rc = alloc_volume(ctx, n_mat);
+    if (rc) return rc;
+    const ctfp_scan_params &s = ctx->scan;
+    const int64_t total = ctx->px * ctx->py * ctx->pz;
+    if (dtype == CTFP_F32)
+        layout_voxels_f32<<<grid_for(total), 256>>>(vp, (float *)ctx->vol, n_mat, s.p_num_x, s.p_num_y,
+                                                    s.p_num_z, ctx->px, ctx->py, ctx->pz);
+    else if (dtype == CTFP_F16)
+        layout_pairs_16<false><<<grid_for(total), 256>>>(vp, (uint32_t *)ctx->vol, n_mat, s.p_num_x,
+                                                         s.p_num_y, s.p_num_z, ctx->px, ctx->py, ctx->pz);
+    else
+        layout_pairs_16<true><<<grid_for(total), 256>>>(vp, (uint32_t *)ctx->vol, n_mat, s.p_num_x,
+                                                        s.p_num_y, s.p_num_z, ctx->px, ctx->py, ctx->pz);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    ctx->dtype = dtype;
+    return CTFP_OK;
+}
+
+static int check_dtype(int dtype)
+{
+    if (dtype != CTFP_F32 && dtype != CTFP_F16 && dtype != CTFP_BF16)
+        return set_error(CTFP_ERR_INVALID, "storage dtype %d is not CTFP_F32/F16/BF16", dtype);
+    return CTFP_OK;
+}
+
 int ctfp_set_volumes(ctfp_context *ctx, int n_mat, const float *const *volumes,
                      int volumes_on_device, int storage_dtype)
 {
     if (check_ctx(ctx)) return CTFP_ERR_INVALID;
     if (!volumes) return set_error(CTFP_ERR_INVALID, "null volume list");
-    if (storage_dtype != CTFP_F32)
-        return set_error(CTFP_ERR_UNSUPPORTED, "storage dtype %d not supported yet", storage_dtype);
+    if (check_dtype(storage_dtype)) return CTFP_ERR_INVALID;
+    if (n_mat < 1 || n_mat > 4)
+        return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d: 1..4 materials are supported", n_mat);
     for (int m = 0; m < n_mat; m++)
         if (!volumes[m]) return set_error(CTFP_ERR_INVALID, "volume %d is null", m);
     DeviceGuard g(ctx->device);
-    int rc = alloc_volume(ctx, n_mat);
-    if (rc) return rc;
     const ctfp_scan_params &s = ctx->scan;
     const int64_t nvox = s.p_num_x * s.p_num_y * s.p_num_z;
-    const int64_t total = ctx->px * ctx->py * ctx->pz;
+    VolPtrs vp{};
+    float *stage = nullptr;
     if (volumes_on_device) {
-        VolPtrs vp{};
         for (int m = 0; m < n_mat; m++) vp.p[m] = volumes[m];
-        pad_interleave_ptrs_f32<<<grid_for(total), 256>>>(vp, (float *)ctx->vol, n_mat, s.p_num_x,
-                                                         s.p_num_y, s.p_num_z, ctx->px, ctx->py,
-                                                         ctx->pz);
-        HIP_TRY(hipGetLastError());
     } else {
-        float *stage = nullptr;
         HIP_TRY(hipMalloc(&stage, (size_t)nvox * n_mat * sizeof(float)));
-        hipError_t e = hipSuccess;
-        for (int m = 0; m < n_mat && e == hipSuccess; m++)
-            e = hipMemcpy(stage + m * nvox, volumes[m], (size_t)nvox * sizeof(float),
-                          hipMemcpyHostToDevice);
-        if (e == hipSuccess) {
-            pad_interleave_f32<<<grid_for(total), 256>>>(stage, (float *)ctx->vol, n_mat, s.p_num_x,
-                                                        s.p_num_y, s.p_num_z, ctx->px, ctx->py,
-                                                        ctx->pz);
-            e = hipGetLastError();
+        for (int m = 0; m < n_mat; m++) {
+            hipError_t e = hipMemcpy(stage + m * nvox, volumes[m], (size_t)nvox * sizeof(float),
+                                     hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                (void)hipFree(stage);
+                return set_error(CTFP_ERR_DEVICE, "volume upload: %s", hipGetErrorString(e));
+            }
+            vp.p[m] = stage + m * nvox;
         }
-        if (e == hipSuccess) e = hipDeviceSynchronize();
-        (void)hipFree(stage);
-        if (e != hipSuccess) return set_error(CTFP_ERR_DEVICE, "volume upload: %s", hipGetErrorString(e));
     }
-    HIP_TRY(hipDeviceSynchronize());
-    ctx->dtype = storage_dtype;
-    return CTFP_OK;
+    int rc = write_layout(ctx, vp, n_mat, storage_dtype);
+    if (stage) (void)hipFree(stage);
+    return rc;
 }
 
 int ctfp_set_labels(ctfp_context *ctx, const uint8_t *labels, int labels_on_device, int n_mat,
@@ -685,39 +808,41 @@ int ctfp_set_labels(ctfp_context *ctx, const uint8_t *labels, int labels_on_devi
 {
     if (check_ctx(ctx)) return CTFP_ERR_INVALID;
     if (!labels || !density) return set_error(CTFP_ERR_INVALID, "null labels or densities");
-    if (storage_dtype != CTFP_F32)
-        return set_error(CTFP_ERR_UNSUPPORTED, "storage dtype %d not supported yet", storage_dtype);
+    if (check_dtype(storage_dtype)) return CTFP_ERR_INVALID;
     if (n_mat < 1 || n_mat > 4)
         return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d: 1..4 materials are supported", n_mat);
     DeviceGuard g(ctx->device);
-    int channels = single_material ? 1 : n_mat;
-    int rc = alloc_volume(ctx, channels);
-    if (rc) return rc;
+    const int channels = single_material ? 1 : n_mat;
     const ctfp_scan_params &s = ctx->scan;
     const int64_t nvox = s.p_num_x * s.p_num_y * s.p_num_z;
-    const uint8_t *dl = labels;
     uint8_t *stage = nullptr;
+    float *raw = nullptr;
+    const uint8_t *dl = labels;
+    hipError_t e = hipSuccess;
     if (!labels_on_device) {
-        HIP_TRY(hipMalloc(&stage, (size_t)nvox));
-        hipError_t e = hipMemcpy(stage, labels, (size_t)nvox, hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
-            (void)hipFree(stage);
-            return set_error(CTFP_ERR_DEVICE, "label upload: %s", hipGetErrorString(e));
-        }
+        e = hipMalloc(&stage, (size_t)nvox);
+        if (e == hipSuccess) e = hipMemcpy(stage, labels, (size_t)nvox, hipMemcpyHostToDevice);
         dl = stage;
     }
-    Densities rho{};
-    for (int m = 0; m < (single_material ? 1 : n_mat); m++) rho.d[m] = density[m];
-    const int64_t total = ctx->px * ctx->py * ctx->pz;
-    labels_to_channels<<<grid_for(total), 256>>>(dl, (float *)ctx->vol, channels, single_material,
-                                                 rho, s.p_num_x, s.p_num_y, s.p_num_z, ctx->px,
-                                                 ctx->py, ctx->pz);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMalloc(&raw, (size_t)nvox * channels * sizeof(float));
+    if (e == hipSuccess) {
+        Densities rho{};
+        for (int m = 0; m < channels; m++) rho.d[m] = density[m];
+        labels_to_raw<<<grid_for(nvox), 256>>>(dl, raw, channels, single_material, rho, nvox);
+        e = hipGetLastError();
+    }
+    int rc = CTFP_OK;
+    if (e == hipSuccess) {
+        VolPtrs vp{};
+        for (int m = 0; m < channels; m++) vp.p[m] = raw + m * nvox;
+        rc = write_layout(ctx, vp, channels, storage_dtype);
+    }
     if (stage) (void)hipFree(stage);
-    if (e != hipSuccess) return set_error(CTFP_ERR_DEVICE, "label channels: %s", hipGetErrorString(e));
-    ctx->dtype = storage_dtype;
-    return CTFP_OK;
+    if (raw) (void)hipFree(raw);
+    if (e != hipSuccess)
+        return set_error(e == hipErrorOutOfMemory ? CTFP_ERR_NOMEM : CTFP_ERR_DEVICE, "label channels: %s",
+                         hipGetErrorString(e));
+    return rc;
 }
 
 int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
@@ -835,7 +960,7 @@ static int launch_march(ctfp_context *ctx, int64_t view_begin, int64_t view_coun
     P.I = I;
     P.IA = IA;
     P.L = L;
-    MarchFn fn = pick_march(ctx->n_mat, spectrum);
+    MarchFn fn = pick_march(ctx->n_mat, ctx->dtype, spectrum);
     if (!fn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->n_mat);
     DeviceGuard g(ctx->device);
     dim3 grid(P.per_xcd * 8), block(kBlock);

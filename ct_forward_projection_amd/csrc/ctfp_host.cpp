@@ -126,7 +126,7 @@ int grid_layout(const ctfp_grid_params *g, GridLayout &L)
 {
     if (!g || g->d_num < 1) return set_error(CTFP_ERR_INVALID, "grid: d_num must be positive");
     // mGridAndDetectorSystem.dSize = dSize * 1000 (float), stored to size_t (Grid.h:126)
-    float d_um = g->d_size_mm * 1000;
+    float d_um = g->d_size_um;
     if (!(d_um >= 1.0f) || d_um > 1.0e6f) return set_error(CTFP_ERR_INVALID, "grid: pixel pitch out of range");
     L.D = (int64_t)d_um;
     L.G = g->d_num * L.D;

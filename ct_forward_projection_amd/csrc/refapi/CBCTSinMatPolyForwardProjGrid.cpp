@@ -1,0 +1,6 @@
+#include "CBCTSinMatPolyForwardProjGrid.h"
+
+CBCTSinMatPolyForwardProjGrid::~CBCTSinMatPolyForwardProjGrid() {}
+// CBCTSinMatPolyForwardProjGrid.cpp:13-145 / 147-267
+void CBCTSinMatPolyForwardProjGrid::computePolyForwProj() { runScan(true, true, true); }
+void CBCTSinMatPolyForwardProjGrid::computePolyForwProjNoResponse() { runScan(true, true, false); }

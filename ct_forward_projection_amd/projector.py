@@ -53,8 +53,10 @@ def det_response(factor: float, mu_c: float, thickness: np.ndarray) -> np.ndarra
 
 def grid_transmission(d_num: int, d_size_mm: float, interspace_width: int, strip_width: int,
                       height_mm: float, focal_mm: float, mu_strip, mu_inter) -> np.ndarray:
-    """T_E(u) [n_energy][d_num] of the focused grid (Grid.cpp:373-443)."""
-    g = L.GridParams(d_num, d_size_mm, interspace_width, strip_width, height_mm, focal_mm)
+    """T_E(u) [n_energy][d_num] of the focused grid (Grid.cpp:373-443).  The
+    pitch enters as FP32 dSize * 1000 um (CBCTPolyForwardProjGrid.cpp:291)."""
+    d_um = float(np.float32(d_size_mm) * np.float32(1000))
+    g = L.GridParams(d_num, d_um, interspace_width, strip_width, height_mm, focal_mm)
     ms = np.ascontiguousarray(np.atleast_1d(mu_strip), dtype=np.float32)
     mi = np.ascontiguousarray(np.atleast_1d(mu_inter), dtype=np.float32)
     if ms.shape != mi.shape:

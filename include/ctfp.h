@@ -96,7 +96,8 @@ typedef struct ctfp_spectral {
 /* Focused anti-scatter grid.  Mirrors GridAndDetectorSystem (Source/Grid.h:42-66). */
 typedef struct ctfp_grid_params {
     int64_t d_num;            /* pixels along u (dNums) */
-    float d_size_mm;          /* pixel pitch, mm; truncated to whole um as the reference does */
+    float d_size_um;          /* pixel pitch in um (dSize * 1000 in FP32); truncated to whole um
+                                 as the reference does (Grid.h:126) */
     int32_t interspace_width; /* um */
     int32_t strip_width;      /* um */
     float height_mm;          /* lamella height h */

@@ -58,10 +58,11 @@ def test_hip_path_matches_fixture(require_gpu, fx):
         Iu, IAu, pr = P.postprocess(I, IA, I0)
         torch.cuda.synchronize()
         I, L, I0, Iu, pr = (t.cpu().numpy() for t in (I, L, I0, Iu, pr))
+    # relative to the array maximum: the fixture's FP32 mirror carries the
+    # reference's FP32 sample-position noise, the HIP path sets geometry in FP64
     rel = lambda a, b: np.abs(a.astype(np.float64) - b).max() / np.abs(b).max()
-    assert rel(L, fx["L"]) < 1e-4
-    assert rel(I, fx["I"]) < 1e-4
+    assert rel(L, fx["L64"]) < 1e-5 and rel(L, fx["L"]) < 1e-4
+    assert rel(I, fx["I64"]) < 1e-5 and rel(I, fx["I"]) < 1e-4
     assert rel(I0, fx["I0"]) < 1e-6
-    assert rel(I, fx["I64"]) < 1e-4
     assert np.abs(Iu.view(np.uint32).astype(np.int64) - fx["Iu"].astype(np.int64)).max() <= 1
     assert rel(pr, fx["proj"]) < 1e-3

@@ -1,11 +1,21 @@
-"""Parity of the HIP path (through the C ABI) with the oracle's FP32 mirror of
-the reference method, on seeded inputs.
+"""Parity of the HIP path (through the C ABI) with the oracle, on seeded inputs.
 
-Tolerance (FP32 path vs FP32 mirror): 1e-4 relative, measured as
-max |d| / max(|ref|, 1e-3 max|ref|) on I, IAbsorb, the line integrals and the
-log projection p = ln(I0 / I) before the floor.  Full-size scans are checked
-through size-independent properties (I + IAbsorb = I0, exact linearity in
-density, view-range independence).
+Metric: max |d| / max(|ref|, f * max|ref|) on the line integrals L, I,
+IAbsorb (f = 1e-3) and the log projection p = ln(I0 / I) before the floor
+(f = 1e-2: p is a log ratio of two FP32 intensities of ~1e6 photons after a
+~100-term energy sum, so it is only resolved to ~1e-6 absolute; below
+1e-2 max|p| its relative error measures output rounding, not the projector).
+
+Two references: the FP64 evaluation of the reference method, and the oracle's
+FP32 mirror of the reference's own arithmetic.  The reference computes every
+sample position as t*d - SOD in FP32 at t ~ SOD, which costs ~5e-5 mm per
+sample; under the metric above that alone puts the FP32 mirror ~2e-4 away
+from FP64 on grazing rays.  The HIP path sets geometry up in FP64, so:
+  - vs FP64: error < 1e-4 (TOL);
+  - vs the FP32 mirror: error <= (mirror vs FP64) + 1e-4.
+Full-size scans are checked through size-independent properties
+(I + IAbsorb = I0, exact linearity in density, view-range independence).
+Set CTFP_PARITY_LOG=<file> to append the measured errors as JSON lines.
 """
 import numpy as np
 import pytest
@@ -18,10 +28,10 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-4
 
 
-def rel_err(got, ref):
+def rel_err(got, ref, floor_frac=1e-3):
     got = np.asarray(got, np.float64)
     ref = np.asarray(ref, np.float64)
-    floor = 1e-3 * np.abs(ref).max() if ref.size else 0.0
+    floor = floor_frac * np.abs(ref).max() if ref.size else 0.0
     if floor == 0.0:
         return float(np.abs(got - ref).max()) if ref.size else 0.0
     return float((np.abs(got - ref) / np.maximum(np.abs(ref), floor)).max())
@@ -31,9 +41,10 @@ def to_np(t):
     return t.detach().cpu().numpy()
 
 
-def run_projector(setup, vols, views=None, view_begin=0, spec_kw=None, want_L=True):
+def run_projector(setup, vols, views=None, view_begin=0, spec_kw=None, want_L=True,
+                  dtype=P.CTFP_F32):
     proj = P.Projector(P.scan_params(**setup.scan_kwargs()))
-    proj.set_volumes(vols)
+    proj.set_volumes(vols, storage_dtype=dtype)
     proj.set_spectral(**spec_kw)
     n = setup.views - view_begin if views is None else views
     I, IA, L = proj.forward(view_begin, n, want_L=want_L)
@@ -46,10 +57,11 @@ def run_projector(setup, vols, views=None, view_begin=0, spec_kw=None, want_L=Tr
 
 
 def oracle_expect(setup, vols, spec_kw, view_begin, views):
+    """(I, IA, L, I0) from the FP32 mirror and from the FP64 evaluation."""
     g = O.geom_from_setup(setup)
     p = O.paras(g)
     L = O.march_f32(g, p, vols, view_begin, views)
-    nu, nv = setup.det
+    L64 = O.march_f64(g, p, vols, view_begin, views)
     grid = spec_kw.get("grid")
     det = spec_kw.get("det")
     spectrum = np.asarray(spec_kw["spectrum"], np.float32)
@@ -60,19 +72,38 @@ def oracle_expect(setup, vols, spec_kw, view_begin, views):
         det = np.stack([O.det_response(spec_kw.get("det_factor", 1.0), float(mu[e]),
                                        spec_kw["scint_thickness"]) for e in range(mu.size)])
     I, IA, I0 = O.spectrum_f32(L, spec_kw["atten"], spectrum, spec_kw["i0_val"], grid, det)
-    return I, IA, L, I0
+    I64, IA64, I064 = O.spectrum_f64(L64, spec_kw["atten"], spectrum, spec_kw["i0_val"], grid, det)
+    return (I, IA, L, I0), (I64, IA64, L64, I064)
 
 
-def check_all(got, ref, tol=TOL):
+def _errors(got, ref):
     I, IA, L, I0 = got
     rI, rIA, rL, rI0 = ref
-    assert rel_err(L, rL) < tol, "line integrals"
-    assert rel_err(I, rI) < tol, "I"
-    assert rel_err(IA, rIA) < tol, "IAbsorb"
-    assert rel_err(I0, rI0) < 1e-6, "I0"
     p = np.log(I0[None].astype(np.float64) / I)
-    pr = np.log(rI0[None].astype(np.float64) / rI)
-    assert rel_err(p, pr) < tol, "log projection"
+    pr = np.log(np.asarray(rI0, np.float64)[None] / rI)
+    return {"L": rel_err(L, rL), "I": rel_err(I, rI), "IA": rel_err(IA, rIA),
+            "I0": rel_err(I0, rI0), "p": rel_err(p, pr, 1e-2)}
+
+
+def log_errors(name, **errs):
+    import json
+    import os
+    path = os.environ.get("CTFP_PARITY_LOG")
+    if path:
+        with open(path, "a") as f:
+            f.write(json.dumps({"test": name, **errs}) + "\n")
+
+
+def check_all(got, refs, tol=TOL, name=""):
+    ref32, ref64 = refs
+    e64 = _errors(got, ref64)
+    e32 = _errors(got, ref32)
+    m = _errors(ref32, ref64)
+    log_errors(name, vs_fp64=e64, vs_fp32_mirror=e32, mirror_vs_fp64=m)
+    for k in ("L", "I", "IA", "p"):
+        assert e64[k] < tol, (k, "vs FP64", e64, m)
+        assert e32[k] <= m[k] + tol, (k, "vs FP32 mirror", e32, m)
+    assert e32["I0"] < 1e-6 and e64["I0"] < 1e-6, ("I0", e32, e64)
 
 
 @pytest.fixture(scope="module")
@@ -88,7 +119,7 @@ def test_config1_single_material(require_gpu, cfg1_phantom):
     for vb in (0, 44):
         got = run_projector(s, [cfg1_phantom], views=4, view_begin=vb, spec_kw=kw)
         ref = oracle_expect(s, [cfg1_phantom], kw, vb, 4)
-        check_all(got, ref)
+        check_all(got, ref, name=f"config1_views_{vb}")
 
 
 def small_three_material(n=96, views=12, det=(160, 120), d_size=0.8, terr=0.05):
@@ -108,7 +139,7 @@ def test_three_material_grid_response(require_gpu):
     kw = P.spectral_inputs(s, *s.det)
     got = run_projector(s, vols, spec_kw=kw)
     ref = oracle_expect(s, vols, kw, 0, s.views)
-    check_all(got, ref)
+    check_all(got, ref, name="three_material_grid_response")
 
 
 def test_no_grid_no_response_variants(require_gpu):
@@ -117,7 +148,7 @@ def test_no_grid_no_response_variants(require_gpu):
         kw = P.spectral_inputs(s, *s.det, with_response=resp, with_grid=grid)
         got = run_projector(s, vols, spec_kw=kw)
         ref = oracle_expect(s, vols, kw, 0, s.views)
-        check_all(got, ref)
+        check_all(got, ref, name=f"variant_grid{grid}_resp{resp}")
 
 
 def test_ragged_detector_and_view_range(require_gpu):
@@ -125,7 +156,7 @@ def test_ragged_detector_and_view_range(require_gpu):
     kw = P.spectral_inputs(s, *s.det)
     got = run_projector(s, vols, views=3, view_begin=13, spec_kw=kw)
     ref = oracle_expect(s, vols, kw, 13, 3)
-    check_all(got, ref)
+    check_all(got, ref, name="ragged_detector_view_range")
 
 
 def test_single_view_and_negative_rotation(require_gpu):
@@ -134,7 +165,7 @@ def test_single_view_and_negative_rotation(require_gpu):
     kw = P.spectral_inputs(s, *s.det)
     got = run_projector(s, vols, spec_kw=kw)
     ref = oracle_expect(s, vols, kw, 0, s.views)
-    check_all(got, ref)
+    check_all(got, ref, name="negative_rotation")
 
 
 def test_labels_multi_material_literal_method(require_gpu):
@@ -161,9 +192,16 @@ def test_labels_multi_material_literal_method(require_gpu):
     det = np.stack([O.det_response(1.0, float(m), kw["scint_thickness"]) for m in kw["scint_mu"]])
     rI, rIA, rI0 = O.forward_literal_f32(g, p, labels, density, mass, kw["spectrum"],
                                          kw["i0_val"], kw["grid"], det)
-    assert rel_err(I, rI) < TOL
-    assert rel_err(IA, rIA) < 5 * TOL
-    assert rel_err(I0, rI0) < 1e-6
+    ch = O.labels_to_density_channels(labels, density)
+    L64 = O.march_f64(g, p, list(ch))
+    I64, IA64, I064 = O.spectrum_f64(L64, kw["atten"], kw["spectrum"], kw["i0_val"], kw["grid"], det)
+    e64 = {"I": rel_err(I, I64), "IA": rel_err(IA, IA64), "I0": rel_err(I0, I064)}
+    lit = {"I": rel_err(I, rI), "IA": rel_err(IA, rIA), "I0": rel_err(I0, rI0)}
+    m = {"I": rel_err(rI, I64), "IA": rel_err(rIA, IA64)}
+    log_errors("labels_multi_material_literal", vs_fp64=e64, vs_fp32_literal=lit, literal_vs_fp64=m)
+    assert e64["I"] < TOL and e64["IA"] < TOL, (e64, m)
+    assert lit["I"] <= m["I"] + TOL and lit["IA"] <= m["IA"] + TOL, (lit, m)
+    assert lit["I0"] < 1e-6
 
 
 def test_labels_single_material_density(require_gpu):
@@ -181,7 +219,7 @@ def test_labels_single_material_density(require_gpu):
     proj.close()
     dens = O.labels_to_density_single(labels, 1.05)
     ref = oracle_expect(s, [dens], kw, 0, s.views)
-    check_all(got, ref)
+    check_all(got, ref, name="labels_single_material")
 
 
 def test_empty_volume_is_flat_field(require_gpu):
@@ -258,7 +296,7 @@ def test_config2_full_size_view_parity(require_gpu, cfg2_volumes):
     kw = P.spectral_inputs(s, *s.det)
     got = run_projector(s, cfg2_volumes, views=1, view_begin=137, spec_kw=kw)
     ref = oracle_expect(s, cfg2_volumes, kw, 137, 1)
-    check_all(got, ref)
+    check_all(got, ref, name="config2_full_size_view137")
 
 
 def test_config2_full_scan_properties(require_gpu, cfg2_volumes):
@@ -287,3 +325,61 @@ def test_config2_full_scan_properties(require_gpu, cfg2_volumes):
     assert torch.equal(L2, 2 * L1)
     assert (L1[:, 0] > 0).float().mean().item() > 0.3
     proj.close()
+
+
+# ------------------------------------------------------- mixed precision
+
+MIXED_TOL = 1e-3
+
+
+def check_mixed(got, ref64, name):
+    e = _errors(got, ref64)
+    log_errors(name, vs_fp64=e)
+    for k in ("L", "I", "IA", "p"):
+        assert e[k] < MIXED_TOL, (k, e)
+
+
+@pytest.mark.parametrize("dtype,name", [(P.CTFP_F16, "f16"), (P.CTFP_BF16, "bf16")])
+def test_mixed_precision_small(require_gpu, dtype, name):
+    """16-bit resident volumes (x-paired layout, dot2 with FP32 accumulate)
+    against the FP64 evaluation: <= 1e-3 relative."""
+    s, vols = small_three_material()
+    kw = P.spectral_inputs(s, *s.det)
+    got = run_projector(s, vols, spec_kw=kw, dtype=dtype)
+    _, ref64 = oracle_expect(s, vols, kw, 0, s.views)
+    check_mixed(got, ref64, f"mixed_{name}_three_material")
+    s1 = S.config1()
+    vol1 = S.cylinder_phantom()
+    kw1 = P.spectral_inputs(s1, *s1.det)
+    got = run_projector(s1, [vol1], views=3, view_begin=30, spec_kw=kw1, dtype=dtype)
+    _, ref64 = oracle_expect(s1, [vol1], kw1, 30, 3)
+    check_mixed(got, ref64, f"mixed_{name}_config1")
+
+
+@pytest.mark.parametrize("dtype,name", [(P.CTFP_F16, "f16"), (P.CTFP_BF16, "bf16")])
+def test_mixed_precision_config2_view(require_gpu, cfg2_volumes, dtype, name):
+    s = S.config2()
+    kw = P.spectral_inputs(s, *s.det)
+    got = run_projector(s, cfg2_volumes, views=1, view_begin=251, spec_kw=kw, dtype=dtype)
+    _, ref64 = oracle_expect(s, cfg2_volumes, kw, 251, 1)
+    check_mixed(got, ref64, f"mixed_{name}_config2_view251")
+
+
+def test_mixed_layout_edges(require_gpu):
+    """x-paired cells at the volume faces: a volume that is non-zero only in
+    its first and last x / y / z slabs gives the same line integrals in every
+    storage type up to 16-bit rounding of the values."""
+    s, _ = small_three_material(n=32, views=6, det=(64, 48), d_size=1.0)
+    rng = np.random.default_rng(17)
+    vols = []
+    for m in range(3):
+        v = np.zeros((32, 32, 32), np.float32)
+        v[:, :, :2] = v[:, :, -2:] = rng.uniform(0.5, 1.5)
+        v[:, :2, :] = v[:, -2:, :] = rng.uniform(0.5, 1.5)
+        v[:2] = v[-2:] = rng.uniform(0.5, 1.5)
+        vols.append(v)
+    kw = P.spectral_inputs(s, *s.det)
+    L32 = run_projector(s, vols, spec_kw=kw)[2]
+    for dt in (P.CTFP_F16, P.CTFP_BF16):
+        L16 = run_projector(s, vols, spec_kw=kw, dtype=dt)[2]
+        assert rel_err(L16, L32) < 5e-3
