@@ -32,7 +32,11 @@ EXPORTED = (
     "ctfp_forward", "ctfp_line_integrals", "ctfp_flat_field", "ctfp_forward_host",
     "ctfp_postprocess", "ctfp_synchronize", "ctfp_scint_thickness", "ctfp_det_response",
     "ctfp_grid_transmission", "ctfp_grid_period_index", "ctfp_bench_copy",
+    "ctfp_set_tuning", "ctfp_get_tuning", "ctfp_backward",
 )
+
+CTFP_MARCH_GLOBAL = 0
+CTFP_MARCH_LDS = 1
 
 
 class ScanParams(C.Structure):
@@ -87,6 +91,11 @@ class GridParams(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    """ctfp_tuning: ray-march launch options (results do not depend on them)."""
+    _fields_ = [("march", C.c_int32), ("chunk", C.c_int32), ("lds_bytes", C.c_int32)]
+
+
 class CTFPError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"ctfp error {code}: {msg}")
@@ -120,6 +129,9 @@ def _declare(lib):
         "ctfp_grid_transmission": (i32, [P(GridParams), i32, _FP, _FP, _FP]),
         "ctfp_grid_period_index": (i32, [i64, _FP, P(i64), i64, P(i64)]),
         "ctfp_bench_copy": (i32, [vp, vp, i64, vp]),
+        "ctfp_set_tuning": (i32, [vp, P(Tuning)]),
+        "ctfp_backward": (i32, [vp, i64, i64, vp, vp, vp, vp, vp]),
+        "ctfp_get_tuning": (i32, [vp, P(Tuning)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -92,6 +93,8 @@ struct ctfp_context {
     // host-mode output staging
     float *dI = nullptr, *dIA = nullptr;
     int64_t staged_views = 0;
+    // launch tuning
+    ctfp_tuning tune{CTFP_MARCH_LDS, 16, 40 * 1024};
 };
 
 namespace {
@@ -282,6 +285,9 @@ struct MarchArgs {
     int32_t w_stride_v;
     // outputs
     float *I, *IA, *L;
+    // LDS staging (march_lds_kernel)
+    int32_t chunk;     // samples per staged chunk
+    int32_t lds_cells; // staged cells that fit the LDS budget
 };
 
 constexpr int kWaveU = 8, kWaveV = 8;  // pixels per wavefront
@@ -429,117 +435,153 @@ template <int ST> __device__ __forceinline__ uint32_t pack_w(float lo, float hi)
     }
 }
 
-template <int M, int ST, bool kSpectrum>
-__global__ __launch_bounds__(kBlock) void march_kernel(MarchArgs P)
+// One sample: integer cell (clamped to [-2, N]; every corner outside the
+// volume is then a padding zero) and FP32 interpolation fractions, from the
+// FP64 position start + j * step.
+struct Cell {
+    int x, y, z;
+    float ax, ay, az;
+};
+__device__ __forceinline__ Cell sample_cell(const RaySetup &R, double jd, int nx, int ny, int nz)
 {
-    // XCD-aware block order: consecutive logical blocks (adjacent tiles of one
-    // view) are kept on one XCD so they share its L2.
-    const uint32_t bid = blockIdx.x;
-    const uint32_t logical = (bid & 7u) * P.per_xcd + (bid >> 3);
-    if (logical >= P.n_blocks) return;
-    const uint32_t tiles = (uint32_t)(P.tiles_u * P.tiles_v);
-    const int vw_local = (int)(logical / tiles);
-    const uint32_t t = logical % tiles;
-    const int tile_v = (int)(t / P.tiles_u), tile_u = (int)(t % P.tiles_u);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int u = tile_u * kTileU + (wave & 1) * kWaveU + (lane % kWaveU);
-    const int v = tile_v * kTileV + (wave >> 1) * kWaveV + (lane / kWaveU);
-    const bool active = (u < P.nu) && (v < P.nv);
-    const int vw = (int)(P.view_begin + vw_local);
+    double fa = fma(jd, R.da, R.a0), fb = fma(jd, R.db, R.b0), fc = fma(jd, R.dc, R.c0);
+    double ia = floor(fa), ib = floor(fb), ic = floor(fc);
+    Cell c;
+    c.ax = (float)(fa - ia);
+    c.ay = (float)(fb - ib);
+    c.az = (float)(fc - ic);
+    c.x = min(max((int)ia, -2), nx);
+    c.y = min(max((int)ib, -2), ny);
+    c.z = min(max((int)ic, -2), nz);
+    return c;
+}
 
-    RaySetup R;
-    if (active) {
-        R = setup_ray(P, u, v, vw);
+// Trilinear accumulation of one sample into the M per-material sums from
+// 8 corners (FP32 voxels) or 4 rows of x-pairs (16-bit).  `fetch(row, out)`
+// returns the M dwords at corner row `row` (0: y0z0, 1: y1z0, 2: y0z1,
+// 3: y1z1) and, for FP32, `fetch(row + 4, out)` the x+1 corner.
+template <int M, int ST, typename Fetch>
+__device__ __forceinline__ void accumulate(const Cell &c, Fetch fetch, float (&acc)[M])
+{
+    const float bx = 1.0f - c.ax, by = 1.0f - c.ay, bz = 1.0f - c.az;
+    const float w00 = by * bz, w10 = c.ay * bz, w01 = by * c.az, w11 = c.ay * c.az;
+    if constexpr (ST == kStF32) {
+        uint32_t d[8][M];
+#pragma unroll
+        for (int r = 0; r < 8; r++) fetch(r, d[r]);
+        const float wr[4] = {w00, w10, w01, w11};
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            float a = acc[m];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                a = fmaf(bx * wr[r], __uint_as_float(d[r][m]), a);
+                a = fmaf(c.ax * wr[r], __uint_as_float(d[r + 4][m]), a);
+            }
+            acc[m] = a;
+        }
     } else {
-        R.a0 = R.b0 = R.c0 = R.da = R.db = R.dc = 0.0;
-        R.k_lo = 0;
-        R.n = 0;
-    }
-
-    __amdgpu_buffer_rsrc_t rsrc =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(P.vol), 0, (int)P.vol_bytes, 0x00020000);
-    constexpr uint32_t CB = 4u * M; // bytes per voxel (FP32) or per x-pair cell (16-bit)
-    const uint32_t oy = P.row_y & 0xFFFFFFu, oz = P.row_z & 0xFFFFFFu, oyz = oy + oz;
-
-    float acc[M];
+        uint32_t d[4][M];
 #pragma unroll
-    for (int m = 0; m < M; m++) acc[m] = 0.0f;
-
-    const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
-    double jd = 0.0;
-    for (int j = 0; j < R.n; j++, jd += 1.0) {
-        // FP64 position; the integer cell is clamped to [-2, N] where every
-        // corner outside the volume is a padding zero, so clamped samples are 0.
-        double fa = fma(jd, R.da, R.a0), fb = fma(jd, R.db, R.b0), fc = fma(jd, R.dc, R.c0);
-        double ia = floor(fa), ib = floor(fb), ic = floor(fc);
-        float ax = (float)(fa - ia), ay = (float)(fb - ib), az = (float)(fc - ic);
-        int xi0 = min(max((int)ia, -2), nx), yi0 = min(max((int)ib, -2), ny), zi0 = min(max((int)ic, -2), nz);
-        // Byte offset with full-rate 24-bit multiply-adds: the masks (no-ops on
-        // the clamped cells) tell the compiler every factor is < 2^24; the
-        // strides are < 2^24 and the offset < 2^32 (checked at allocation).
-        uint32_t xi = (uint32_t)(xi0 + kVoxPadLo) & 0xFFFFFu;
-        uint32_t yi = (uint32_t)(yi0 + kVoxPadLo) & 0xFFFFFu;
-        uint32_t zi = (uint32_t)(zi0 + kVoxPadLo) & 0xFFFFFu;
-        uint32_t voff = zi * oz + (yi * oy + xi * CB);
-        const float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
-        const float w00 = by * bz, w10 = ay * bz, w01 = by * az, w11 = ay * az;
-        if constexpr (ST == kStF32) {
-            float c000[M], c100[M], c010[M], c110[M], c001[M], c101[M], c011[M], c111[M];
-            load_vox<M>(rsrc, voff, 0, c000);
-            load_vox<M>(rsrc, voff + CB, 0, c100);
-            load_vox<M>(rsrc, voff, oy, c010);
-            load_vox<M>(rsrc, voff + CB, oy, c110);
-            load_vox<M>(rsrc, voff, oz, c001);
-            load_vox<M>(rsrc, voff + CB, oz, c101);
-            load_vox<M>(rsrc, voff, oyz, c011);
-            load_vox<M>(rsrc, voff + CB, oyz, c111);
-            // trilinear weights, accumulated straight into the per-material sums
+        for (int r = 0; r < 4; r++) fetch(r, d[r]);
+        const uint32_t p[4] = {pack_w<ST>(bx * w00, c.ax * w00), pack_w<ST>(bx * w10, c.ax * w10),
+                               pack_w<ST>(bx * w01, c.ax * w01), pack_w<ST>(bx * w11, c.ax * w11)};
 #pragma unroll
-            for (int m = 0; m < M; m++) {
-                float a = acc[m];
-                a = fmaf(bx * w00, c000[m], a);
-                a = fmaf(ax * w00, c100[m], a);
-                a = fmaf(bx * w10, c010[m], a);
-                a = fmaf(ax * w10, c110[m], a);
-                a = fmaf(bx * w01, c001[m], a);
-                a = fmaf(ax * w01, c101[m], a);
-                a = fmaf(bx * w11, c011[m], a);
-                a = fmaf(ax * w11, c111[m], a);
-                acc[m] = a;
-            }
-        } else {
-            // four row loads, each holding (v(x), v(x+1)) per channel; one
-            // dot2 per row and channel with the rounded weight pair
-            uint32_t r00[M], r10[M], r01[M], r11[M];
-            load_raw<M>(rsrc, voff, 0, r00);
-            load_raw<M>(rsrc, voff, oy, r10);
-            load_raw<M>(rsrc, voff, oz, r01);
-            load_raw<M>(rsrc, voff, oyz, r11);
-            const uint32_t p00 = pack_w<ST>(bx * w00, ax * w00);
-            const uint32_t p10 = pack_w<ST>(bx * w10, ax * w10);
-            const uint32_t p01 = pack_w<ST>(bx * w01, ax * w01);
-            const uint32_t p11 = pack_w<ST>(bx * w11, ax * w11);
+        for (int m = 0; m < M; m++) {
+            float a = acc[m];
 #pragma unroll
-            for (int m = 0; m < M; m++) {
-                float a = acc[m];
-                a = dot2_acc<ST>(p00, r00[m], a);
-                a = dot2_acc<ST>(p10, r10[m], a);
-                a = dot2_acc<ST>(p01, r01[m], a);
-                a = dot2_acc<ST>(p11, r11[m], a);
-                acc[m] = a;
-            }
+            for (int r = 0; r < 4; r++) a = dot2_acc<ST>(p[r], d[r][m], a);
+            acc[m] = a;
         }
     }
-    if (!active) return;
+}
 
+// Global-memory gather of one sample (raw buffer loads; one VGPR offset).
+template <int M, int ST>
+__device__ __forceinline__ void accumulate_global(__amdgpu_buffer_rsrc_t rsrc, const Cell &c, uint32_t oy,
+                                                  uint32_t oz, float (&acc)[M])
+{
+    constexpr uint32_t CB = 4u * M;
+    // Byte offset with full-rate 24-bit multiply-adds: the masks (no-ops on
+    // the clamped cells) tell the compiler every factor is < 2^24; the
+    // strides are < 2^24 and the offset < 2^32 (checked at allocation).
+    const uint32_t xi = (uint32_t)(c.x + kVoxPadLo) & 0xFFFFFu;
+    const uint32_t yi = (uint32_t)(c.y + kVoxPadLo) & 0xFFFFFu;
+    const uint32_t zi = (uint32_t)(c.z + kVoxPadLo) & 0xFFFFFu;
+    const uint32_t voff = zi * oz + (yi * oy + xi * CB);
+    const uint32_t soff[4] = {0u, oy, oz, oy + oz};
+    accumulate<M, ST>(c, [&](int r, uint32_t(&out)[M]) {
+        load_raw<M>(rsrc, r < 4 ? voff : voff + CB, soff[r & 3], out);
+    }, acc);
+}
+
+// Bytes of one staged cell in LDS: M dwords, three padded to four so every
+// cell read is one aligned ds_read.
+template <int M> constexpr uint32_t lds_cell_bytes() { return M == 3 ? 16u : 4u * M; }
+
+template <int M>
+__device__ __forceinline__ void lds_read(const char *lds, uint32_t off, uint32_t (&c)[M])
+{
+    if constexpr (M == 1) {
+        c[0] = *(const uint32_t *)(lds + off);
+    } else if constexpr (M == 2) {
+        uint2 v = *(const uint2 *)(lds + off);
+        c[0] = v.x;
+        c[1] = v.y;
+    } else {
+        uint4 v = *(const uint4 *)(lds + off);
+        c[0] = v.x;
+        c[1] = v.y;
+        c[2] = v.z;
+        if constexpr (M == 4) c[3] = v.w;
+    }
+}
+
+template <int M>
+__device__ __forceinline__ void lds_write(char *lds, uint32_t off, const uint32_t (&c)[M])
+{
+    if constexpr (M == 1) {
+        *(uint32_t *)(lds + off) = c[0];
+    } else if constexpr (M == 2) {
+        *(uint2 *)(lds + off) = make_uint2(c[0], c[1]);
+    } else if constexpr (M == 3) {
+        *(uint4 *)(lds + off) = make_uint4(c[0], c[1], c[2], 0u);
+    } else {
+        *(uint4 *)(lds + off) = make_uint4(c[0], c[1], c[2], c[3]);
+    }
+}
+
+// Ray geometry of the lane's pixel (or an empty range for lanes past the edge).
+struct TileLane {
+    int vw_local, u, v;
+    bool active;
+};
+__device__ __forceinline__ TileLane tile_lane(const MarchArgs &P, uint32_t logical)
+{
+    const uint32_t tiles = (uint32_t)(P.tiles_u * P.tiles_v);
+    TileLane t;
+    t.vw_local = (int)(logical / tiles);
+    const uint32_t q = logical % tiles;
+    const int tile_v = (int)(q / P.tiles_u), tile_u = (int)(q % P.tiles_u);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    t.u = tile_u * kTileU + (wave & 1) * kWaveU + (lane % kWaveU);
+    t.v = tile_v * kTileV + (wave >> 1) * kWaveV + (lane / kWaveU);
+    t.active = (t.u < P.nu) && (t.v < P.nv);
+    return t;
+}
+
+// Writes L and, with the spectrum, I and IAbsorb of one ray.
+template <int M, bool kSpectrum>
+__device__ __forceinline__ void finish_ray(const MarchArgs &P, const TileLane &t, const float (&acc)[M])
+{
     const int64_t npix = (int64_t)P.nu * P.nv;
-    const int64_t pix = (int64_t)v * P.nu + u;
+    const int64_t pix = (int64_t)t.v * P.nu + t.u;
     float Lm[M];
 #pragma unroll
     for (int m = 0; m < M; m++) Lm[m] = acc[m] * P.dx_f;
     if (P.L) {
 #pragma unroll
-        for (int m = 0; m < M; m++) P.L[((int64_t)vw_local * M + m) * npix + pix] = Lm[m];
+        for (int m = 0; m < M; m++) P.L[((int64_t)t.vw_local * M + m) * npix + pix] = Lm[m];
     }
     if constexpr (kSpectrum) {
         // Per-ray spectrum: tau_e = sum_m A[e][m] L_m (FP32), e^-tau and
@@ -547,7 +589,7 @@ __global__ __launch_bounds__(kBlock) void march_kernel(MarchArgs P)
         // energy sums of W e^-tau and W (1 - e^-tau) in bin order in FP64 (an
         // FP32 sum of ~100 terms would cost ~1e-6 of I, visible in ln(I0/I)
         // of thin rays; FP64 FMA runs at the FP32 rate).
-        const float *wp = P.W + (int64_t)v * P.w_stride_v + u;
+        const float *wp = P.W + (int64_t)t.v * P.w_stride_v + t.u;
         double sI = 0.0, sA = 0.0;
         for (int e = 0; e < P.n_act; e++) {
             float tau = 0.0f;
@@ -559,30 +601,382 @@ __global__ __launch_bounds__(kBlock) void march_kernel(MarchArgs P)
             sI = fma(w, (double)ex, sI);
             sA = fma(w, (double)ab, sA);
         }
-        P.I[(int64_t)vw_local * npix + pix] = (float)sI;
-        P.IA[(int64_t)vw_local * npix + pix] = (float)sA;
+        P.I[(int64_t)t.vw_local * npix + pix] = (float)sI;
+        P.IA[(int64_t)t.vw_local * npix + pix] = (float)sA;
     }
+}
+
+__device__ __forceinline__ uint32_t xcd_logical(const MarchArgs &P)
+{
+    // XCD-aware block order: consecutive logical blocks (adjacent tiles of one
+    // view) are kept on one XCD so they share its L2.
+    const uint32_t bid = blockIdx.x;
+    return (bid & 7u) * P.per_xcd + (bid >> 3);
+}
+
+// Ray-march with direct global gathers: 8 (FP32) or 4 (16-bit) loads per sample.
+template <int M, int ST, bool kSpectrum>
+__global__ __launch_bounds__(kBlock) void march_kernel(MarchArgs P)
+{
+    const uint32_t logical = xcd_logical(P);
+    if (logical >= P.n_blocks) return;
+    const TileLane t = tile_lane(P, logical);
+    RaySetup R;
+    if (t.active) {
+        R = setup_ray(P, t.u, t.v, (int)(P.view_begin + t.vw_local));
+    } else {
+        R.a0 = R.b0 = R.c0 = R.da = R.db = R.dc = 0.0;
+        R.k_lo = 0;
+        R.n = 0;
+    }
+    __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(P.vol), 0, (int)P.vol_bytes, 0x00020000);
+    const uint32_t oy = P.row_y & 0xFFFFFFu, oz = P.row_z & 0xFFFFFFu;
+    const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
+    float acc[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) acc[m] = 0.0f;
+    double jd = 0.0;
+    for (int j = 0; j < R.n; j++, jd += 1.0)
+        accumulate_global<M, ST>(rsrc, sample_cell(R, jd, nx, ny, nz), oy, oz, acc);
+    if (t.active) finish_ray<M, kSpectrum>(P, t, acc);
+}
+
+// Block-wide min / max of six ints (cell box of a chunk).
+__device__ __forceinline__ void block_box(int (&lo)[3], int (&hi)[3], int *scratch)
+{
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+#pragma unroll
+        for (int s = 32; s >= 1; s >>= 1) {
+            lo[d] = min(lo[d], __shfl_xor(lo[d], s));
+            hi[d] = max(hi[d], __shfl_xor(hi[d], s));
+        }
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            scratch[wave * 8 + d] = lo[d];
+            scratch[wave * 8 + 4 + d] = hi[d];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        int a = scratch[d], b = scratch[4 + d];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; w++) {
+            a = min(a, scratch[w * 8 + d]);
+            b = max(b, scratch[w * 8 + 4 + d]);
+        }
+        lo[d] = a;
+        hi[d] = b;
+    }
+}
+
+// Exact q = n / d for 0 <= n < 2^22, 1 <= d < 2^10, from a float reciprocal.
+__device__ __forceinline__ int small_div(int n, int d, float inv_d)
+{
+    int q = (int)((float)n * inv_d);
+    q -= (q * d > n);
+    q += ((q + 1) * d <= n);
+    return q;
+}
+
+// Ray-march of one block's rays with the volume staged in LDS per chunk of
+// samples: the rays move together along the common sample lattice; for each
+// chunk of `chunk` samples the box of cells they touch is loaded once,
+// row-coalesced, and all 8 (or 4) corner reads come from LDS.  A chunk whose
+// box exceeds the LDS budget reads from global memory instead.  Each ray still
+// accumulates in ascending k, so the result equals march_kernel's bit for bit.
+// Every thread of the block must call it (it synchronises the block).
+template <int M, int ST>
+__device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup &R,
+                                             __amdgpu_buffer_rsrc_t rsrc, char *lds, int *scratch,
+                                             float (&acc)[M])
+{
+    constexpr uint32_t LB = lds_cell_bytes<M>();
+    const uint32_t oy = P.row_y & 0xFFFFFFu, oz = P.row_z & 0xFFFFFFu;
+    const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
+    const int k_hi = R.k_lo + R.n;
+    int kr_lo[3] = {R.n > 0 ? R.k_lo : INT_MAX, 0, 0}, kr_hi[3] = {R.n > 0 ? k_hi : INT_MIN, 0, 0};
+    __syncthreads();
+    block_box(kr_lo, kr_hi, scratch);
+    const int kb_lo = kr_lo[0], kb_hi = kr_hi[0];
+    for (int k0 = kb_lo; k0 < kb_hi; k0 += P.chunk) {
+        const int j0 = max(k0, R.k_lo) - R.k_lo, j1 = min(k0 + P.chunk, k_hi) - R.k_lo;
+        int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+        if (j1 > j0) { // positions are monotone in j: the chunk's ends bound its cells
+            Cell a = sample_cell(R, (double)j0, nx, ny, nz), b = sample_cell(R, (double)(j1 - 1), nx, ny, nz);
+            lo[0] = min(a.x, b.x); lo[1] = min(a.y, b.y); lo[2] = min(a.z, b.z);
+            hi[0] = max(a.x, b.x) + 1; hi[1] = max(a.y, b.y) + 1; hi[2] = max(a.z, b.z) + 1;
+        }
+        __syncthreads(); // scratch and staged cells of the previous chunk are free
+        block_box(lo, hi, scratch);
+        const int sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
+        const bool staged = sx > 0 && (int64_t)sx * sy * sz <= (int64_t)P.lds_cells;
+        if (staged) {
+            // cooperative, row-coalesced fill of the box [lo, hi]
+            const int cells = sx * sy * sz;
+            const float inv_sx = 1.0f / (float)sx, inv_sy = 1.0f / (float)sy;
+            for (int i = threadIdx.x; i < cells; i += kBlock) {
+                const int row = small_div(i, sx, inv_sx), lx = i - row * sx;
+                const int lz = small_div(row, sy, inv_sy), ly = row - lz * sy;
+                const uint32_t xi = (uint32_t)(lo[0] + lx + kVoxPadLo) & 0xFFFFFu;
+                const uint32_t yi = (uint32_t)(lo[1] + ly + kVoxPadLo) & 0xFFFFFu;
+                const uint32_t zi = (uint32_t)(lo[2] + lz + kVoxPadLo) & 0xFFFFFu;
+                uint32_t d[M];
+                load_raw<M>(rsrc, zi * oz + (yi * oy + xi * (4u * M)), 0, d);
+                lds_write<M>(lds, (uint32_t)i * LB, d);
+            }
+            __syncthreads();
+            const uint32_t ry = (uint32_t)sx * LB, rz = (uint32_t)(sx * sy) * LB;
+            const uint32_t roff[4] = {0u, ry, rz, ry + rz};
+            double jd = (double)j0;
+            for (int j = j0; j < j1; j++, jd += 1.0) {
+                const Cell c = sample_cell(R, jd, nx, ny, nz);
+                const uint32_t base = (uint32_t)(((c.z - lo[2]) * sy + (c.y - lo[1])) * sx + (c.x - lo[0])) * LB;
+                accumulate<M, ST>(c, [&](int r, uint32_t(&out)[M]) {
+                    lds_read<M>(lds, base + roff[r & 3] + (r < 4 ? 0u : LB), out);
+                }, acc);
+            }
+        } else {
+            double jd = (double)j0;
+            for (int j = j0; j < j1; j++, jd += 1.0)
+                accumulate_global<M, ST>(rsrc, sample_cell(R, jd, nx, ny, nz), oy, oz, acc);
+        }
+    }
+}
+
+__device__ __forceinline__ RaySetup lane_ray(const MarchArgs &P, const TileLane &t)
+{
+    RaySetup R;
+    if (t.active) {
+        R = setup_ray(P, t.u, t.v, (int)(P.view_begin + t.vw_local));
+    } else {
+        R.a0 = R.b0 = R.c0 = R.da = R.db = R.dc = 0.0;
+        R.k_lo = 0;
+        R.n = 0;
+    }
+    return R;
+}
+
+template <int M, int ST, bool kSpectrum>
+__global__ __launch_bounds__(kBlock) void march_lds_kernel(MarchArgs P)
+{
+    extern __shared__ char lds_raw[];
+    const uint32_t logical = xcd_logical(P);
+    if (logical >= P.n_blocks) return;
+    const TileLane t = tile_lane(P, logical);
+    const RaySetup R = lane_ray(P, t);
+    __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(P.vol), 0, (int)P.vol_bytes, 0x00020000);
+    float acc[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) acc[m] = 0.0f;
+    march_chunks<M, ST>(P, R, rsrc, lds_raw + 256, (int *)lds_raw, acc);
+    if (t.active) finish_ray<M, kSpectrum>(P, t, acc);
+}
+
+// ------------------------------------------------------------ the adjoint ---
+
+struct AdjointArgs {
+    const float *gI, *gIA; // [views][nv][nu] or null (zero)
+    const float *L;        // [views][M][nv][nu] or null (recomputed)
+    float *grad;           // [M][nz][ny][nx], accumulated
+};
+
+// Backprojection of one block's rays: every sample adds g_m * w_corner to
+// the 8 corners of its cell.  Per chunk the block accumulates into an LDS copy
+// of the chunk's cell box (ds_add_f32), then adds the box's non-zero cells to
+// the gradient with one hardware FP32 atomic per cell and material.  A chunk
+// whose box exceeds the LDS budget adds straight to global memory.
+template <int M>
+__device__ __forceinline__ void scatter_chunks(const MarchArgs &P, const RaySetup &R, const float (&g)[M],
+                                               float *grad, char *lds, int *scratch)
+{
+    constexpr uint32_t LW = lds_cell_bytes<M>() / 4; // floats per staged cell
+    float *box = (float *)lds;
+    const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
+    const int64_t nvox = (int64_t)nx * ny * nz;
+    const int k_hi = R.k_lo + R.n;
+    int kr_lo[3] = {R.n > 0 ? R.k_lo : INT_MAX, 0, 0}, kr_hi[3] = {R.n > 0 ? k_hi : INT_MIN, 0, 0};
+    __syncthreads();
+    block_box(kr_lo, kr_hi, scratch);
+    const int kb_lo = kr_lo[0], kb_hi = kr_hi[0];
+    auto add_global = [&](int x, int y, int z, float w) {
+        if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz || w == 0.0f) return;
+        const int64_t q = ((int64_t)z * ny + y) * nx + x;
+#pragma unroll
+        for (int m = 0; m < M; m++)
+            if (g[m] != 0.0f) unsafeAtomicAdd(grad + m * nvox + q, g[m] * w);
+    };
+    for (int k0 = kb_lo; k0 < kb_hi; k0 += P.chunk) {
+        const int j0 = max(k0, R.k_lo) - R.k_lo, j1 = min(k0 + P.chunk, k_hi) - R.k_lo;
+        int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+        if (j1 > j0) {
+            Cell a = sample_cell(R, (double)j0, nx, ny, nz), b = sample_cell(R, (double)(j1 - 1), nx, ny, nz);
+            lo[0] = min(a.x, b.x); lo[1] = min(a.y, b.y); lo[2] = min(a.z, b.z);
+            hi[0] = max(a.x, b.x) + 1; hi[1] = max(a.y, b.y) + 1; hi[2] = max(a.z, b.z) + 1;
+        }
+        __syncthreads();
+        block_box(lo, hi, scratch);
+        const int sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
+        const bool staged = sx > 0 && (int64_t)sx * sy * sz <= (int64_t)P.lds_cells;
+        if (staged) {
+            const int cells = sx * sy * sz;
+            for (int i = threadIdx.x; i < cells * (int)LW; i += kBlock) box[i] = 0.0f;
+            __syncthreads();
+            double jd = (double)j0;
+            for (int j = j0; j < j1; j++, jd += 1.0) {
+                const Cell c = sample_cell(R, jd, nx, ny, nz);
+                const float bx = 1.0f - c.ax, by = 1.0f - c.ay, bz = 1.0f - c.az;
+                const float wr[4] = {by * bz, c.ay * bz, by * c.az, c.ay * c.az};
+                const int base = ((c.z - lo[2]) * sy + (c.y - lo[1])) * sx + (c.x - lo[0]);
+                const int roff[4] = {0, sx, sx * sy, sx * sy + sx};
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    float *p0 = box + (uint32_t)(base + roff[r]) * LW;
+#pragma unroll
+                    for (int m = 0; m < M; m++) {
+                        atomicAdd(p0 + m, g[m] * (bx * wr[r]));
+                        atomicAdd(p0 + LW + m, g[m] * (c.ax * wr[r]));
+                    }
+                }
+            }
+            __syncthreads();
+            const float inv_sx = 1.0f / (float)sx, inv_sy = 1.0f / (float)sy;
+            for (int i = threadIdx.x; i < cells; i += kBlock) {
+                const int row = small_div(i, sx, inv_sx), lx = i - row * sx;
+                const int lz = small_div(row, sy, inv_sy), ly = row - lz * sy;
+                const int x = lo[0] + lx, y = lo[1] + ly, z = lo[2] + lz;
+                if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) continue;
+                const int64_t q = ((int64_t)z * ny + y) * nx + x;
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    const float v = box[(uint32_t)i * LW + m];
+                    if (v != 0.0f) unsafeAtomicAdd(grad + m * nvox + q, v);
+                }
+            }
+        } else {
+            double jd = (double)j0;
+            for (int j = j0; j < j1; j++, jd += 1.0) {
+                const Cell c = sample_cell(R, jd, nx, ny, nz);
+                const float bx = 1.0f - c.ax, by = 1.0f - c.ay, bz = 1.0f - c.az;
+                const float wr[4] = {by * bz, c.ay * bz, by * c.az, c.ay * c.az};
+                const int dy[4] = {0, 1, 0, 1}, dz[4] = {0, 0, 1, 1};
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    add_global(c.x, c.y + dy[r], c.z + dz[r], bx * wr[r]);
+                    add_global(c.x + 1, c.y + dy[r], c.z + dz[r], c.ax * wr[r]);
+                }
+            }
+        }
+    }
+}
+
+// Adjoint of ctfp_forward for upstream gradients gI, gIA: per ray, the line
+// integrals (read or recomputed), then dl/dL_m = (gIA - gI) sum_e A[e][m]
+// W_e e^-tau_e, then the backprojection of dx * dl/dL_m.
+template <int M, int ST>
+__global__ __launch_bounds__(kBlock) void adjoint_kernel(MarchArgs P, AdjointArgs G)
+{
+    extern __shared__ char lds_raw[];
+    const uint32_t logical = xcd_logical(P);
+    if (logical >= P.n_blocks) return;
+    const TileLane t = tile_lane(P, logical);
+    const RaySetup R = lane_ray(P, t);
+    const int64_t npix = (int64_t)P.nu * P.nv;
+    const int64_t pix = (int64_t)t.v * P.nu + t.u;
+    float Lm[M];
+    if (G.L) {
+#pragma unroll
+        for (int m = 0; m < M; m++)
+            Lm[m] = t.active ? G.L[((int64_t)t.vw_local * M + m) * npix + pix] : 0.0f;
+    } else {
+        __amdgpu_buffer_rsrc_t rsrc =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(P.vol), 0, (int)P.vol_bytes, 0x00020000);
+        float acc[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) acc[m] = 0.0f;
+        march_chunks<M, ST>(P, R, rsrc, lds_raw + 256, (int *)lds_raw, acc);
+#pragma unroll
+        for (int m = 0; m < M; m++) Lm[m] = acc[m] * P.dx_f;
+    }
+    float g[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) g[m] = 0.0f;
+    if (t.active) {
+        const int64_t q = (int64_t)t.vw_local * npix + pix;
+        const float c = (G.gIA ? G.gIA[q] : 0.0f) - (G.gI ? G.gI[q] : 0.0f);
+        if (c != 0.0f) {
+            const float *wp = P.W + (int64_t)t.v * P.w_stride_v + t.u;
+            double s[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) s[m] = 0.0;
+            for (int e = 0; e < P.n_act; e++) {
+                float tau = 0.0f;
+#pragma unroll
+                for (int m = 0; m < M; m++) tau = fmaf(P.A[e * M + m], Lm[m], tau);
+                const double we = (double)wp[(int64_t)e * P.w_stride_e] * (double)expf(-tau);
+#pragma unroll
+                for (int m = 0; m < M; m++) s[m] = fma((double)P.A[e * M + m], we, s[m]);
+            }
+#pragma unroll
+            for (int m = 0; m < M; m++) g[m] = (float)((double)c * s[m] * (double)P.dx_f);
+        }
+    }
+    scatter_chunks<M>(P, R, g, G.grad, lds_raw + 256, (int *)lds_raw);
 }
 
 using MarchFn = void (*)(MarchArgs);
 
-template <int ST> MarchFn pick_march_st(int n_mat, bool spectrum)
+template <int ST> MarchFn pick_march_st(int n_mat, bool spectrum, bool lds)
+{
+#define CTFP_PICK(Mv)                                                                   \
+    case Mv:                                                                            \
+        if (lds) return spectrum ? march_lds_kernel<Mv, ST, true> : march_lds_kernel<Mv, ST, false>; \
+        return spectrum ? march_kernel<Mv, ST, true> : march_kernel<Mv, ST, false>;
+    switch (n_mat) {
+        CTFP_PICK(1)
+        CTFP_PICK(2)
+        CTFP_PICK(3)
+        CTFP_PICK(4)
+    default: return nullptr;
+    }
+#undef CTFP_PICK
+}
+
+using AdjointFn = void (*)(MarchArgs, AdjointArgs);
+
+template <int ST> AdjointFn pick_adjoint_st(int n_mat)
 {
     switch (n_mat) {
-    case 1: return spectrum ? march_kernel<1, ST, true> : march_kernel<1, ST, false>;
-    case 2: return spectrum ? march_kernel<2, ST, true> : march_kernel<2, ST, false>;
-    case 3: return spectrum ? march_kernel<3, ST, true> : march_kernel<3, ST, false>;
-    case 4: return spectrum ? march_kernel<4, ST, true> : march_kernel<4, ST, false>;
+    case 1: return adjoint_kernel<1, ST>;
+    case 2: return adjoint_kernel<2, ST>;
+    case 3: return adjoint_kernel<3, ST>;
+    case 4: return adjoint_kernel<4, ST>;
     default: return nullptr;
     }
 }
 
-MarchFn pick_march(int n_mat, int dtype, bool spectrum)
+AdjointFn pick_adjoint(int n_mat, int dtype)
 {
     switch (dtype) {
-    case CTFP_F32: return pick_march_st<kStF32>(n_mat, spectrum);
-    case CTFP_F16: return pick_march_st<kStF16>(n_mat, spectrum);
-    case CTFP_BF16: return pick_march_st<kStBF16>(n_mat, spectrum);
+    case CTFP_F32: return pick_adjoint_st<kStF32>(n_mat);
+    case CTFP_F16: return pick_adjoint_st<kStF16>(n_mat);
+    case CTFP_BF16: return pick_adjoint_st<kStBF16>(n_mat);
+    default: return nullptr;
+    }
+}
+
+MarchFn pick_march(int n_mat, int dtype, bool spectrum, bool lds)
+{
+    switch (dtype) {
+    case CTFP_F32: return pick_march_st<kStF32>(n_mat, spectrum, lds);
+    case CTFP_F16: return pick_march_st<kStF16>(n_mat, spectrum, lds);
+    case CTFP_BF16: return pick_march_st<kStBF16>(n_mat, spectrum, lds);
     default: return nullptr;
     }
 }
@@ -960,12 +1354,40 @@ static int launch_march(ctfp_context *ctx, int64_t view_begin, int64_t view_coun
     P.I = I;
     P.IA = IA;
     P.L = L;
-    MarchFn fn = pick_march(ctx->n_mat, ctx->dtype, spectrum);
+    const bool lds = ctx->tune.march == CTFP_MARCH_LDS;
+    MarchFn fn = pick_march(ctx->n_mat, ctx->dtype, spectrum, lds);
     if (!fn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->n_mat);
+    size_t shmem = 0;
+    if (lds) {
+        const uint32_t lb = ctx->n_mat == 3 ? 16u : 4u * (uint32_t)ctx->n_mat;
+        P.chunk = ctx->tune.chunk;
+        P.lds_cells = (int32_t)((ctx->tune.lds_bytes - 256) / lb);
+        shmem = (size_t)ctx->tune.lds_bytes;
+    }
     DeviceGuard g(ctx->device);
     dim3 grid(P.per_xcd * 8), block(kBlock);
-    hipLaunchKernelGGL(fn, grid, block, 0, (hipStream_t)stream, P);
+    hipLaunchKernelGGL(fn, grid, block, shmem, (hipStream_t)stream, P);
     HIP_TRY(hipGetLastError());
+    return CTFP_OK;
+}
+
+int ctfp_set_tuning(ctfp_context *ctx, const ctfp_tuning *t)
+{
+    if (check_ctx(ctx)) return CTFP_ERR_INVALID;
+    if (!t) return set_error(CTFP_ERR_INVALID, "null tuning");
+    if (t->march != CTFP_MARCH_GLOBAL && t->march != CTFP_MARCH_LDS)
+        return set_error(CTFP_ERR_INVALID, "tuning: march mode %d", t->march);
+    if (t->chunk < 1 || t->chunk > 4096) return set_error(CTFP_ERR_INVALID, "tuning: chunk %d", t->chunk);
+    if (t->lds_bytes < 1024 || t->lds_bytes > 64 * 1024)
+        return set_error(CTFP_ERR_INVALID, "tuning: lds_bytes %d outside [1 KiB, 64 KiB]", t->lds_bytes);
+    ctx->tune = *t;
+    return CTFP_OK;
+}
+
+int ctfp_get_tuning(const ctfp_context *ctx, ctfp_tuning *t)
+{
+    if (check_ctx(ctx) || !t) return set_error(CTFP_ERR_INVALID, "null argument");
+    *t = ctx->tune;
     return CTFP_OK;
 }
 
@@ -973,6 +1395,36 @@ int ctfp_forward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, floa
                  float *IAbsorb, float *L, void *stream)
 {
     return launch_march(ctx, view_begin, view_count, I, IAbsorb, L, stream, true);
+}
+
+int ctfp_backward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, const float *gI,
+                  const float *gIAbsorb, const float *L, float *grad, void *stream)
+{
+    if (check_ctx(ctx)) return CTFP_ERR_INVALID;
+    MarchArgs P;
+    int rc = fill_march_args(ctx, view_begin, view_count, P);
+    if (rc) return rc;
+    if (!ctx->W || !ctx->A) return set_error(CTFP_ERR_STATE, "spectral weights not set");
+    if (ctx->spec_n_mat != ctx->n_mat)
+        return set_error(CTFP_ERR_STATE, "spectral tables are for %d materials, volumes have %d",
+                         ctx->spec_n_mat, ctx->n_mat);
+    if (!grad) return set_error(CTFP_ERR_INVALID, "null gradient output");
+    P.n_act = ctx->n_act;
+    P.A = ctx->A;
+    P.W = ctx->W;
+    P.w_stride_e = ctx->w_stride_e;
+    P.w_stride_v = ctx->w_stride_v;
+    const uint32_t lb = ctx->n_mat == 3 ? 16u : 4u * (uint32_t)ctx->n_mat;
+    P.chunk = ctx->tune.chunk;
+    P.lds_cells = (int32_t)((ctx->tune.lds_bytes - 256) / lb);
+    AdjointArgs G{gI, gIAbsorb, L, grad};
+    AdjointFn fn = pick_adjoint(ctx->n_mat, ctx->dtype);
+    if (!fn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->n_mat);
+    DeviceGuard g(ctx->device);
+    hipLaunchKernelGGL(fn, dim3(P.per_xcd * 8), dim3(kBlock), (size_t)ctx->tune.lds_bytes,
+                       (hipStream_t)stream, P, G);
+    HIP_TRY(hipGetLastError());
+    return CTFP_OK;
 }
 
 int ctfp_line_integrals(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *L,

@@ -128,6 +128,26 @@ class Projector:
     def __exit__(self, *exc):
         self.close()
 
+    def set_tuning(self, march: str | None = None, chunk: int | None = None,
+                   lds_bytes: int | None = None):
+        """Launch options of the march ("global" or "lds", samples per staged
+        chunk, LDS bytes per workgroup); results are identical for all."""
+        t = L.Tuning()
+        L.check(L.lib().ctfp_get_tuning(self._ctx, C.byref(t)))
+        if march is not None:
+            t.march = {"global": L.CTFP_MARCH_GLOBAL, "lds": L.CTFP_MARCH_LDS}[march]
+        if chunk is not None:
+            t.chunk = chunk
+        if lds_bytes is not None:
+            t.lds_bytes = lds_bytes
+        L.check(L.lib().ctfp_set_tuning(self._ctx, C.byref(t)))
+
+    def get_tuning(self) -> dict:
+        t = L.Tuning()
+        L.check(L.lib().ctfp_get_tuning(self._ctx, C.byref(t)))
+        return {"march": "lds" if t.march == L.CTFP_MARCH_LDS else "global", "chunk": t.chunk,
+                "lds_bytes": t.lds_bytes}
+
     @property
     def samples_per_view(self) -> int:
         return int(self.nu * self.nv * self.info["int_num"])
@@ -238,6 +258,33 @@ class Projector:
         L.check(L.lib().ctfp_line_integrals(self._ctx, vb, vc, out.data_ptr(),
                                             _stream_handle(stream)))
         return out
+
+    def backward(self, gI=None, gIA=None, L_in=None, view_begin: int = 0,
+                 view_count: int | None = None, grad=None, stream=None):
+        """Adjoint of forward: adds dl/d(density) for upstream gradients
+        gI = dl/dI and gIA = dl/dIAbsorb ([views][nv][nu] CUDA tensors, either
+        may be None) to `grad` ([n_mat][nz][ny][nx], zero-initialised when not
+        given) and returns it.  L_in (from forward(want_L=True)) skips the
+        per-ray recomputation of the line integrals."""
+        torch = _torch()
+        vb, vc = self._views(view_begin, view_count)
+        if grad is None:
+            grad = torch.zeros((self.n_mat, self.scan.p_num_z, self.scan.p_num_y, self.scan.p_num_x),
+                               dtype=torch.float32, device=torch.device("cuda", self.device))
+        shape = (vc, self.nv, self.nu)
+        for t in (gI, gIA):
+            if t is not None and (tuple(t.shape) != shape or t.dtype != torch.float32
+                                  or not t.is_contiguous()):
+                raise ValueError(f"upstream gradients must be contiguous float32 {shape}")
+        if L_in is not None and L_in.numel() != vc * self.n_mat * self.nv * self.nu:
+            raise ValueError("L_in does not match the view range")
+        if grad.numel() != self.n_mat * self.scan.p_num_x * self.scan.p_num_y * self.scan.p_num_z \
+                or not grad.is_contiguous() or grad.dtype != torch.float32:
+            raise ValueError("grad must be a contiguous float32 [n_mat][nz][ny][nx] tensor")
+        ptr = lambda t: None if t is None else t.data_ptr()
+        L.check(L.lib().ctfp_backward(self._ctx, vb, vc, ptr(gI), ptr(gIA), ptr(L_in), grad.data_ptr(),
+                                      _stream_handle(stream)))
+        return grad
 
     def flat_field(self, stream=None):
         torch = _torch()

@@ -104,6 +104,17 @@ typedef struct ctfp_grid_params {
     float focal_mm;           /* focal distance FD */
 } ctfp_grid_params;
 
+/* Launch tuning of the ray-march (no reference counterpart: the reference
+ * launches fixed 8x8x8 blocks, CBCTPolyProjGPU.cuh:22-24).  Tuning never
+ * changes results: every setting accumulates each ray in the same order. */
+#define CTFP_MARCH_GLOBAL 0 /* corners gathered from global memory */
+#define CTFP_MARCH_LDS 1    /* per-chunk volume boxes staged in LDS (default) */
+typedef struct ctfp_tuning {
+    int32_t march;     /* CTFP_MARCH_GLOBAL or CTFP_MARCH_LDS */
+    int32_t chunk;     /* samples per staged chunk (LDS mode), 1..4096 */
+    int32_t lds_bytes; /* LDS per workgroup (LDS mode), 1 KiB..64 KiB */
+} ctfp_tuning;
+
 typedef struct ctfp_context ctfp_context;
 
 /* ---- library ------------------------------------------------------------ */
@@ -142,6 +153,9 @@ int ctfp_set_labels(ctfp_context *ctx, const uint8_t *labels, int labels_on_devi
 /* Replaces the per-energy host uploads and scalars; see ctfp_spectral. */
 int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *spec);
 
+int ctfp_set_tuning(ctfp_context *ctx, const ctfp_tuning *tuning);
+int ctfp_get_tuning(const ctfp_context *ctx, ctfp_tuning *tuning);
+
 /* Polyenergetic forward projection of views [view_begin, view_begin+view_count):
  * ray-march of every material once, then the whole spectrum per ray.
  * Replaces forward{Proj,SinMatProj,SinMatNoResponseProj}{Grid,NoGrid}GPU
@@ -157,6 +171,17 @@ int ctfp_forward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, floa
  * L: device [view_count][n_mat][d_num_v][d_num_u]. */
 int ctfp_line_integrals(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *L,
                         void *stream);
+
+/* Adjoint of ctfp_forward (no reference counterpart: the reference has no
+ * gradient).  For upstream gradients gI = dl/dI and gIAbsorb = dl/dIAbsorb
+ * (device [view_count][d_num_v][d_num_u]; either may be NULL = zero), adds
+ * dl/d(density) to `grad` (device FP32 [n_mat][p_num_z][p_num_y][p_num_x];
+ * the caller zeroes it).  L (device [view_count][n_mat][d_num_v][d_num_u],
+ * from ctfp_forward) may be NULL, then the line integrals are recomputed per
+ * ray.  Accumulation uses FP32 atomics: the summation order, hence the last
+ * bits, can differ between runs. */
+int ctfp_backward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, const float *gI,
+                  const float *gIAbsorb, const float *L, float *grad, void *stream);
 
 /* Flat field I0 = sum_E I0Val*S_E*T_E*eta_E in bin order.  Replaces addI0 /
  * addI0NoResponse (CBCTPolyForwardProjGrid.cpp:330-350,

@@ -647,3 +647,88 @@ int orc_num_threads(void)
     return 1;
 #endif
 }
+
+/* ------------------------------------------------------------- adjoint --- */
+
+/* Exact adjoint of orc_march_f64's linear map rho_m -> L_m: every sample k of
+ * ray (view, v, u) adds dx * gL * w_corner to the 8 corners of its cell
+ * (zero-padded border: corners outside the volume are dropped).
+ * gL: [n_views][n_mat][nv][nu]; grad: [n_mat][nz][ny][nx], accumulated.
+ * Serial (the order of additions is fixed). */
+void orc_backproject_f64(const orc_geom *g, const orc_paras *s, int n_mat, int64_t view0,
+                         int64_t n_views, const double *gL, double *grad)
+{
+    const int64_t nvox = g->nx * g->ny * g->nz;
+    const int64_t npix = g->nu * g->nv;
+    const double t0 = (double)g->sod - (double)s->fov_r + (double)s->dx / 2;
+    for (int64_t iv = 0; iv < n_views; iv++) {
+        double th = (double)orc_view_angle(g, s, view0 + iv);
+        double cth = cos(th), sth = sin(th);
+        for (int64_t v = 0; v < g->nv; v++) {
+            for (int64_t u = 0; u < g->nu; u++) {
+                ray_f64 d = ray_dir_f64(g, s, u, v);
+                double gm[16];
+                int any = 0;
+                for (int m = 0; m < n_mat; m++) {
+                    gm[m] = gL[(iv * n_mat + m) * npix + v * g->nu + u] * (double)s->dx;
+                    any |= gm[m] != 0.0;
+                }
+                if (!any) continue;
+                for (int64_t k = 0; k < s->n_steps; k++) {
+                    double t = t0 + (double)k * (double)s->dx;
+                    double X = d.cgcb * t - (double)g->sod, Y = d.cgsb * t, Z = d.sg * t;
+                    double a = (X * cth - Y * sth + s->half_x) / s->px;
+                    double b = (-(X * sth + Y * cth) + s->half_y) / s->py;
+                    double c = (-Z + s->half_z) / s->pz;
+                    if (!(a > -1.0 && a < (double)g->nx && b > -1.0 && b < (double)g->ny && c > -1.0 &&
+                          c < (double)g->nz))
+                        continue;
+                    double fx = floor(a), fy = floor(b), fz = floor(c);
+                    double w[2][3] = {{1.0 - (a - fx), 1.0 - (b - fy), 1.0 - (c - fz)},
+                                      {a - fx, b - fy, c - fz}};
+                    int64_t ix = (int64_t)fx, iy = (int64_t)fy, iz = (int64_t)fz;
+                    for (int dz = 0; dz < 2; dz++)
+                        for (int dy = 0; dy < 2; dy++)
+                            for (int dx_ = 0; dx_ < 2; dx_++) {
+                                int64_t x = ix + dx_, y = iy + dy, z = iz + dz;
+                                if (x < 0 || y < 0 || z < 0 || x >= g->nx || y >= g->ny || z >= g->nz)
+                                    continue;
+                                double wc = w[dx_][0] * w[dy][1] * w[dz][2];
+                                int64_t q = (z * g->ny + y) * g->nx + x;
+                                for (int m = 0; m < n_mat; m++) grad[m * nvox + q] += gm[m] * wc;
+                            }
+                }
+            }
+        }
+    }
+}
+
+/* Chain rule of the spectrum, per ray: with tau_e = sum_m atten[e][m] L_m,
+ * I = sum_e W_e e^-tau_e and IA = sum_e W_e (1 - e^-tau_e),
+ * dl/dL_m = (gIA - gI) * sum_e atten[e][m] W_e e^-tau_e,
+ * W_e = i0val * S_e * T_e(u) * eta_e(v,u) (grid / det may be NULL). */
+void orc_spectrum_grad_f64(int64_t nu, int64_t nv, int64_t n_views, int n_mat, int n_energy,
+                           const double *L, const float *atten, const float *spectrum, double i0val,
+                           const float *grid, const float *det, const double *gI, const double *gIA,
+                           double *gL)
+{
+    const int64_t npix = nu * nv;
+    for (int64_t iv = 0; iv < n_views; iv++) {
+        for (int64_t p = 0; p < npix; p++) {
+            int64_t u = p % nu;
+            double s[16] = {0};
+            for (int e = 0; e < n_energy; e++) {
+                double tau = 0.0;
+                for (int m = 0; m < n_mat; m++)
+                    tau += (double)atten[e * n_mat + m] * L[(iv * n_mat + m) * npix + p];
+                double w = i0val * (double)spectrum[e];
+                if (grid) w *= (double)grid[e * nu + u];
+                if (det) w *= (double)det[(int64_t)e * npix + p];
+                double we = w * exp(-tau);
+                for (int m = 0; m < n_mat; m++) s[m] += (double)atten[e * n_mat + m] * we;
+            }
+            double c = (gIA ? gIA[iv * npix + p] : 0.0) - (gI ? gI[iv * npix + p] : 0.0);
+            for (int m = 0; m < n_mat; m++) gL[(iv * n_mat + m) * npix + p] = c * s[m];
+        }
+    }
+}

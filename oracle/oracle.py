@@ -67,6 +67,9 @@ def lib():
             "orc_grid_unit_heights": (None, [i64, C.c_float, C.c_float, _F]),
             "orc_postprocess": (None, [i64, i64, _F, _F, _F, vp, vp, _F]),
             "orc_num_threads": (i32, []),
+            "orc_backproject_f64": (None, [G, P, i32, i64, i64, _D, _D]),
+            "orc_spectrum_grad_f64": (None, [i64, i64, i64, i32, i32, _D, _F, _F, C.c_double, _F, _F,
+                                             _D, _D, _D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -255,3 +258,27 @@ def postprocess(I, IA, I0):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def backproject_f64(g, p, gL, view0=0):
+    """Adjoint of march_f64: gL [views][M][nv][nu] -> grad [M][nz][ny][nx]."""
+    gL = np.ascontiguousarray(gL, dtype=np.float64)
+    n_views, n_mat = gL.shape[:2]
+    grad = np.zeros((n_mat, g.nz, g.ny, g.nx), dtype=np.float64)
+    lib().orc_backproject_f64(C.byref(g), C.byref(p), n_mat, view0, n_views, _d(gL), _d(grad))
+    return grad
+
+
+def spectrum_grad_f64(L, atten, spectrum, i0val, gI, gIA, grid=None, det=None):
+    """dl/dL [views][M][nv][nu] from dl/dI and dl/dIA (either may be None)."""
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    nv_, n_mat, nv, nu = L.shape
+    atten, spectrum = f32(atten), f32(spectrum)
+    grid = None if grid is None else f32(grid)
+    det = None if det is None else f32(det)
+    gI = None if gI is None else np.ascontiguousarray(gI, dtype=np.float64)
+    gIA = None if gIA is None else np.ascontiguousarray(gIA, dtype=np.float64)
+    gL = np.empty_like(L)
+    lib().orc_spectrum_grad_f64(nu, nv, nv_, n_mat, spectrum.size, _d(L), _f(atten), _f(spectrum),
+                                float(i0val), _f(grid), _f(det), _d(gI), _d(gIA), _d(gL))
+    return gL

@@ -329,31 +329,35 @@ def test_config2_full_scan_properties(require_gpu, cfg2_volumes):
 
 # ------------------------------------------------------- mixed precision
 
-MIXED_TOL = 1e-3
+# FP16 volumes meet the 1e-3 target.  BF16 keeps 8 mantissa bits: a bone or
+# iodine insert's density is quantised by up to 2^-9 (0.2%) as one systematic
+# offset per insert, which reaches ~1e-3 of p and ~4e-3 of L on the small
+# phantoms here; it is held to 5e-3 and reported, not claimed at 1e-3.
+MIXED_TOL = {"f16": 1e-3, "bf16": 5e-3}
 
 
-def check_mixed(got, ref64, name):
+def check_mixed(got, ref64, name, kind):
     e = _errors(got, ref64)
     log_errors(name, vs_fp64=e)
     for k in ("L", "I", "IA", "p"):
-        assert e[k] < MIXED_TOL, (k, e)
+        assert e[k] < MIXED_TOL[kind], (k, e)
 
 
 @pytest.mark.parametrize("dtype,name", [(P.CTFP_F16, "f16"), (P.CTFP_BF16, "bf16")])
 def test_mixed_precision_small(require_gpu, dtype, name):
     """16-bit resident volumes (x-paired layout, dot2 with FP32 accumulate)
-    against the FP64 evaluation: <= 1e-3 relative."""
+    against the FP64 evaluation."""
     s, vols = small_three_material()
     kw = P.spectral_inputs(s, *s.det)
     got = run_projector(s, vols, spec_kw=kw, dtype=dtype)
     _, ref64 = oracle_expect(s, vols, kw, 0, s.views)
-    check_mixed(got, ref64, f"mixed_{name}_three_material")
+    check_mixed(got, ref64, f"mixed_{name}_three_material", name)
     s1 = S.config1()
     vol1 = S.cylinder_phantom()
     kw1 = P.spectral_inputs(s1, *s1.det)
     got = run_projector(s1, [vol1], views=3, view_begin=30, spec_kw=kw1, dtype=dtype)
     _, ref64 = oracle_expect(s1, [vol1], kw1, 30, 3)
-    check_mixed(got, ref64, f"mixed_{name}_config1")
+    check_mixed(got, ref64, f"mixed_{name}_config1", name)
 
 
 @pytest.mark.parametrize("dtype,name", [(P.CTFP_F16, "f16"), (P.CTFP_BF16, "bf16")])
@@ -362,7 +366,7 @@ def test_mixed_precision_config2_view(require_gpu, cfg2_volumes, dtype, name):
     kw = P.spectral_inputs(s, *s.det)
     got = run_projector(s, cfg2_volumes, views=1, view_begin=251, spec_kw=kw, dtype=dtype)
     _, ref64 = oracle_expect(s, cfg2_volumes, kw, 251, 1)
-    check_mixed(got, ref64, f"mixed_{name}_config2_view251")
+    check_mixed(got, ref64, f"mixed_{name}_config2_view251", name)
 
 
 def test_mixed_layout_edges(require_gpu):
@@ -383,3 +387,28 @@ def test_mixed_layout_edges(require_gpu):
     for dt in (P.CTFP_F16, P.CTFP_BF16):
         L16 = run_projector(s, vols, spec_kw=kw, dtype=dt)[2]
         assert rel_err(L16, L32) < 5e-3
+
+
+# -------------------------------------------------------- launch tuning
+
+@pytest.mark.parametrize("dtype", [P.CTFP_F32, P.CTFP_F16])
+def test_lds_staging_equals_global_gather(require_gpu, dtype):
+    """The LDS-staged march reads the same corners in the same order as the
+    global-gather march: outputs are bit-identical for every chunk length and
+    LDS budget, including budgets so small that chunks fall back to global."""
+    import torch
+    s, vols = small_three_material(n=80, views=6, det=(150, 110), d_size=0.8)
+    kw = P.spectral_inputs(s, *s.det)
+    with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
+        proj.set_volumes(vols, storage_dtype=dtype)
+        proj.set_spectral(**kw)
+        proj.set_tuning(march="global")
+        ref = proj.forward(want_L=True)
+        for chunk, lds in [(16, 40 * 1024), (1, 40 * 1024), (7, 8 * 1024), (64, 64 * 1024), (32, 2048)]:
+            proj.set_tuning(march="lds", chunk=chunk, lds_bytes=lds)
+            got = proj.forward(want_L=True)
+            torch.cuda.synchronize()
+            for a, b in zip(got, ref):
+                assert torch.equal(a, b), (chunk, lds)
+        with pytest.raises(P.CTFPError):
+            proj.set_tuning(chunk=0)

@@ -361,3 +361,41 @@ def test_postprocess_floor_and_log():
     assert IAu.tolist() == [[3, 0, 2, 9, 0, 1]]
     assert np.isinf(proj[0, 0]) and proj[0, 0] > 0
     np.testing.assert_allclose(proj[0, 2:], np.log(1000.0 / Iu[0, 2:].astype(np.float64)), rtol=1e-6)
+
+
+# ----------------------------------------------------------------- adjoint
+
+def test_backprojection_is_the_adjoint():
+    """<A x, y> = <x, A^T y> for the linear march, in FP64."""
+    g = small_geom(n=20, nu=18, nv=14, views=3, lx=20.0, dsize=1.6, sod=300.0, sdd=450.0)
+    p = O.paras(g)
+    rng = np.random.default_rng(21)
+    x = rng.uniform(0, 1, size=(2, 20, 20, 20)).astype(np.float32)
+    y = rng.normal(size=(3, 2, 14, 18))
+    Ax = O.march_f64(g, p, list(x))
+    ATy = O.backproject_f64(g, p, y)
+    lhs = float((Ax * y).sum())
+    rhs = float((x.astype(np.float64) * ATy).sum())
+    assert lhs == pytest.approx(rhs, rel=1e-12)
+
+
+def test_spectrum_grad_matches_central_differences():
+    s = S.config2()
+    rng = np.random.default_rng(22)
+    L = (rng.uniform(0, 40, size=(1, 3, 2, 3)) * np.array([1, 0.2, 0.02])[None, :, None, None])
+    atten, spec = s.atten(), s.spectrum()
+    gI = rng.normal(size=(1, 2, 3))
+    gIA = rng.normal(size=(1, 2, 3))
+    gL = O.spectrum_grad_f64(L, atten, spec, 1.0e5, gI, gIA)
+
+    def loss(Lx):
+        I, IA, _ = O.spectrum_f64(Lx, atten, spec, 1.0e5)
+        return float((I * gI).sum() + (IA * gIA).sum())
+
+    for idx in [(0, 0, 0, 0), (0, 1, 1, 2), (0, 2, 0, 1)]:
+        h = 1e-4
+        Lp, Lm = L.copy(), L.copy()
+        Lp[idx] += h
+        Lm[idx] -= h
+        fd = (loss(Lp) - loss(Lm)) / (2 * h)
+        assert gL[idx] == pytest.approx(fd, rel=1e-6)
