@@ -7,6 +7,12 @@ applied per ray; outputs I and IAbsorb stay in HBM.  Inputs are synthetic
 (seeded phantom, Kramers spectrum, model attenuation tables) and resident in
 HBM before timing starts.
 
+The headline is the mixed-precision projector (volumes stored in FP16,
+interpolation, line integrals and exp in FP32, energy sums in FP64); the
+same scan through the FP32 projector with the same launch options and
+through the plain FP32 projector (corners gathered from HBM/L2, no LDS
+staging) is timed beside it.  --dtype f32 / bf16 time the other storage types.
+
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -83,11 +89,18 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--views", type=int, default=0, help="views per step (0 = the full 360)")
-    ap.add_argument("--dtype", choices=["f32", "f16", "bf16"], default="f32",
-                    help="storage precision of the resident volumes")
+    ap.add_argument("--dtype", choices=["f32", "f16", "bf16"], default="f16",
+                    help="storage precision of the resident volumes (f16: the mixed-precision "
+                         "projector; f32 and the global-gather FP32 projector are timed beside it)")
+    ap.add_argument("--march", choices=["lds", "global"], default="lds",
+                    help="corner gathers from LDS-staged chunks or straight from HBM/L2")
+    ap.add_argument("--chunk", type=int, default=0, help="samples per staged chunk (0 = library default)")
+    ap.add_argument("--lds-kib", type=int, default=0, help="LDS per workgroup in KiB (0 = default)")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the one-view error check against the FP64 oracle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true",
+                    help="do not time the FP32 projectors beside a 16-bit run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also launch the copy kernel over this many MiB (counter calibration)")
@@ -112,6 +125,12 @@ def main():
     proj = P.Projector(P.scan_params(**setup.scan_kwargs()))
     proj.set_volumes(vols, storage_dtype=dtypes[args.dtype])
     proj.set_spectral(**kw)
+    tune = {"march": args.march}
+    if args.chunk > 0:
+        tune["chunk"] = args.chunk
+    if args.lds_kib > 0:
+        tune["lds_bytes"] = args.lds_kib * 1024
+    proj.set_tuning(**tune)
     views = setup.views if args.views <= 0 else min(args.views, setup.views)
     nu, nv = setup.det
     dev = torch.device("cuda", 0)
@@ -161,30 +180,39 @@ def main():
     tfile = os.path.join(ROOT, "profiles", "march_config2_traffic.json")
     if os.path.exists(tfile) and views == setup.views:
         with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            entry = json.load(f).get(f"{args.dtype}_{args.march}")
+        if entry:
+            traffic = entry.get("bytes_per_launch")
 
     result = {
         "metric": METRIC, "value": round(gsps, 3), "unit": "Gsamples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.dtype,
         "data": "synthetic",
-        "config": {"workload": "3-material 512^3 FP32 (water/bone/iodine, 0.5 mm), 1024x768 "
+        "config": {"workload": "3-material 512^3 (water/bone/iodine, 0.5 mm), 1024x768 "
                                "@ 0.4 mm, 360 views, K=1082, 120 kVp 101 bins, Pb grid 10:1 "
                                "40 l/cm, CsI response",
                    "views_per_step": views, "samples_per_step": samples,
-                   "materials": n_mat, "volume_dtype": args.dtype},
+                   "materials": n_mat, "volume_dtype": args.dtype, "tuning": proj.get_tuning()},
         "views_per_s": round(views / (ms_per_step / 1e3), 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
     }
-    if args.dtype != "f32":
+    if args.dtype != "f32" and not args.no_fp32:
+        # the FP32 projector with the same launch options, and the plain FP32
+        # projector (every corner gathered from HBM/L2, no LDS staging)
         ref = P.Projector(P.scan_params(**setup.scan_kwargs()))
         ref.set_volumes(vols, storage_dtype=P.CTFP_F32)
         ref.set_spectral(**kw)
+        ref.set_tuning(**tune)
         wall32, _ = timed(ref, "f32")
         result["fp32_value"] = round(samples / (wall32 / args.steps) / 1e9, 3)
         result["speedup_vs_fp32"] = round(result["value"] / result["fp32_value"], 3)
+        ref.set_tuning(march="global")
+        wallg, _ = timed(ref, "f32 global")
+        result["plain_fp32_value"] = round(samples / (wallg / args.steps) / 1e9, 3)
+        result["speedup_vs_plain_fp32"] = round(result["value"] / result["plain_fp32_value"], 3)
         ref.close()
     if not args.no_accuracy:
         result["rel_err_vs_fp64"] = accuracy(proj, setup, vols, kw)

@@ -32,7 +32,8 @@ EXPORTED = (
     "ctfp_forward", "ctfp_line_integrals", "ctfp_flat_field", "ctfp_forward_host",
     "ctfp_postprocess", "ctfp_synchronize", "ctfp_scint_thickness", "ctfp_det_response",
     "ctfp_grid_transmission", "ctfp_grid_period_index", "ctfp_bench_copy",
-    "ctfp_set_tuning", "ctfp_get_tuning", "ctfp_backward",
+    "ctfp_set_tuning", "ctfp_get_tuning", "ctfp_backward", "ctfp_set_volumes_async",
+    "ctfp_get_scan", "ctfp_get_device",
 )
 
 CTFP_MARCH_GLOBAL = 0
@@ -132,6 +133,9 @@ def _declare(lib):
         "ctfp_set_tuning": (i32, [vp, P(Tuning)]),
         "ctfp_backward": (i32, [vp, i64, i64, vp, vp, vp, vp, vp]),
         "ctfp_get_tuning": (i32, [vp, P(Tuning)]),
+        "ctfp_set_volumes_async": (i32, [vp, i32, P(vp), i32, vp]),
+        "ctfp_get_scan": (i32, [vp, P(ScanParams)]),
+        "ctfp_get_device": (i32, [vp, P(i32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

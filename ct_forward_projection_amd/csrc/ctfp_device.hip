@@ -714,6 +714,7 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
         }
         __syncthreads(); // scratch and staged cells of the previous chunk are free
         block_box(lo, hi, scratch);
+        if (hi[0] < lo[0]) continue; // no lane of the block samples this chunk (block-uniform)
         const int sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
         const bool staged = sx > 0 && (int64_t)sx * sy * sz <= (int64_t)P.lds_cells;
         if (staged) {
@@ -822,6 +823,7 @@ __device__ __forceinline__ void scatter_chunks(const MarchArgs &P, const RaySetu
         }
         __syncthreads();
         block_box(lo, hi, scratch);
+        if (hi[0] < lo[0]) continue; // no lane of the block samples this chunk (block-uniform)
         const int sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
         const bool staged = sx > 0 && (int64_t)sx * sy * sz <= (int64_t)P.lds_cells;
         if (staged) {
@@ -1134,24 +1136,26 @@ int ctfp_get_system_info(const ctfp_context *ctx, ctfp_system_info *out)
     return CTFP_OK;
 }
 
-// Writes the resident layout for `dtype` from n_mat FP32 device volumes.
-static int write_layout(ctfp_context *ctx, const VolPtrs &vp, int n_mat, int dtype)
+// Writes the resident layout for `dtype` from n_mat FP32 device volumes on
+// `stream`; waits for it unless `async`.
+static int write_layout(ctfp_context *ctx, const VolPtrs &vp, int n_mat, int dtype,
+                        hipStream_t stream = nullptr, bool async = false)
 {
     int rc = alloc_volume(ctx, n_mat);
     if (rc) return rc;
     const ctfp_scan_params &s = ctx->scan;
     const int64_t total = ctx->px * ctx->py * ctx->pz;
     if (dtype == CTFP_F32)
-        layout_voxels_f32<<<grid_for(total), 256>>>(vp, (float *)ctx->vol, n_mat, s.p_num_x, s.p_num_y,
+        layout_voxels_f32<<<grid_for(total), 256, 0, stream>>>(vp, (float *)ctx->vol, n_mat, s.p_num_x, s.p_num_y,
                                                     s.p_num_z, ctx->px, ctx->py, ctx->pz);
     else if (dtype == CTFP_F16)
-        layout_pairs_16<false><<<grid_for(total), 256>>>(vp, (uint32_t *)ctx->vol, n_mat, s.p_num_x,
+        layout_pairs_16<false><<<grid_for(total), 256, 0, stream>>>(vp, (uint32_t *)ctx->vol, n_mat, s.p_num_x,
                                                          s.p_num_y, s.p_num_z, ctx->px, ctx->py, ctx->pz);
     else
-        layout_pairs_16<true><<<grid_for(total), 256>>>(vp, (uint32_t *)ctx->vol, n_mat, s.p_num_x,
+        layout_pairs_16<true><<<grid_for(total), 256, 0, stream>>>(vp, (uint32_t *)ctx->vol, n_mat, s.p_num_x,
                                                         s.p_num_y, s.p_num_z, ctx->px, ctx->py, ctx->pz);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipDeviceSynchronize());
+    if (!async) HIP_TRY(hipDeviceSynchronize());
     ctx->dtype = dtype;
     return CTFP_OK;
 }
@@ -1195,6 +1199,37 @@ int ctfp_set_volumes(ctfp_context *ctx, int n_mat, const float *const *volumes,
     int rc = write_layout(ctx, vp, n_mat, storage_dtype);
     if (stage) (void)hipFree(stage);
     return rc;
+}
+
+int ctfp_set_volumes_async(ctfp_context *ctx, int n_mat, const float *const *volumes,
+                           int storage_dtype, void *stream)
+{
+    if (check_ctx(ctx)) return CTFP_ERR_INVALID;
+    if (!volumes) return set_error(CTFP_ERR_INVALID, "null volume list");
+    if (check_dtype(storage_dtype)) return CTFP_ERR_INVALID;
+    if (n_mat < 1 || n_mat > 4)
+        return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d: 1..4 materials are supported", n_mat);
+    VolPtrs vp{};
+    for (int m = 0; m < n_mat; m++) {
+        if (!volumes[m]) return set_error(CTFP_ERR_INVALID, "volume %d is null", m);
+        vp.p[m] = volumes[m];
+    }
+    DeviceGuard g(ctx->device);
+    return write_layout(ctx, vp, n_mat, storage_dtype, (hipStream_t)stream, true);
+}
+
+int ctfp_get_scan(const ctfp_context *ctx, ctfp_scan_params *out)
+{
+    if (check_ctx(ctx) || !out) return set_error(CTFP_ERR_INVALID, "null argument");
+    *out = ctx->scan;
+    return CTFP_OK;
+}
+
+int ctfp_get_device(const ctfp_context *ctx, int *device)
+{
+    if (check_ctx(ctx) || !device) return set_error(CTFP_ERR_INVALID, "null argument");
+    *device = ctx->device;
+    return CTFP_OK;
 }
 
 int ctfp_set_labels(ctfp_context *ctx, const uint8_t *labels, int labels_on_device, int n_mat,
@@ -1415,14 +1450,17 @@ int ctfp_backward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, con
     P.w_stride_e = ctx->w_stride_e;
     P.w_stride_v = ctx->w_stride_v;
     const uint32_t lb = ctx->n_mat == 3 ? 16u : 4u * (uint32_t)ctx->n_mat;
+    // CTFP_MARCH_GLOBAL: no chunk is staged; every sample gathers and
+    // scatters through global memory (the LDS then only holds the box scratch)
+    const bool global = ctx->tune.march == CTFP_MARCH_GLOBAL;
+    const size_t shmem = global ? 256 : (size_t)ctx->tune.lds_bytes;
     P.chunk = ctx->tune.chunk;
-    P.lds_cells = (int32_t)((ctx->tune.lds_bytes - 256) / lb);
+    P.lds_cells = global ? 0 : (int32_t)((ctx->tune.lds_bytes - 256) / lb);
     AdjointArgs G{gI, gIAbsorb, L, grad};
     AdjointFn fn = pick_adjoint(ctx->n_mat, ctx->dtype);
     if (!fn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->n_mat);
     DeviceGuard g(ctx->device);
-    hipLaunchKernelGGL(fn, dim3(P.per_xcd * 8), dim3(kBlock), (size_t)ctx->tune.lds_bytes,
-                       (hipStream_t)stream, P, G);
+    hipLaunchKernelGGL(fn, dim3(P.per_xcd * 8), dim3(kBlock), shmem, (hipStream_t)stream, P, G);
     HIP_TRY(hipGetLastError());
     return CTFP_OK;
 }

@@ -142,6 +142,13 @@ class Projector:
             t.lds_bytes = lds_bytes
         L.check(L.lib().ctfp_set_tuning(self._ctx, C.byref(t)))
 
+    @property
+    def handle(self) -> int:
+        """The ctfp_context pointer as an integer (for the PyTorch extension)."""
+        if not self._ctx:
+            raise L.CTFPError(L.CTFP_ERR_STATE, "the projector is closed")
+        return int(self._ctx.value)
+
     def get_tuning(self) -> dict:
         t = L.Tuning()
         L.check(L.lib().ctfp_get_tuning(self._ctx, C.byref(t)))

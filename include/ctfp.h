@@ -140,6 +140,18 @@ int ctfp_get_system_info(const ctfp_context *ctx, ctfp_system_info *out);
 int ctfp_set_volumes(ctfp_context *ctx, int n_mat, const float *const *volumes,
                      int volumes_on_device, int storage_dtype);
 
+/* Stream-ordered form of ctfp_set_volumes for FP32 device volumes (no
+ * reference counterpart: the reference copies host volumes into its texture
+ * arrays synchronously, initTexture3D / updateTex, CBCTPolyProjGPU.cu:2175-2221).  The layout pass is queued on `stream` and
+ * the call returns without waiting; the volumes must stay valid until it has
+ * run.  The first call, or a call with another n_mat, allocates. */
+int ctfp_set_volumes_async(ctfp_context *ctx, int n_mat, const float *const *volumes,
+                           int storage_dtype, void *stream);
+
+/* The scan parameters and device of a context (accessors for bindings). */
+int ctfp_get_scan(const ctfp_context *ctx, ctfp_scan_params *out);
+int ctfp_get_device(const ctfp_context *ctx, int *device);
+
 /* Build the density channels from a uint8 label volume on the device.
  * single_material = 0: channel m = density[m] where label == m+1, else 0
  *   (PhantomObject::updataLineAtten, PhantomObject.cpp:38-58, with (mu/rho)/10

@@ -28,7 +28,8 @@ def setup_case(n=40, views=6, det=(72, 56), d_size=1.0, seed=5):
 
 
 def det_table(kw):
-    return np.stack([O.det_response(1.0, float(m), kw["scint_thickness"]) for m in kw["scint_mu"]])
+    return np.stack([O.det_response(kw["det_factor"], float(m), kw["scint_thickness"])
+                     for m in kw["scint_mu"]])
 
 
 @pytest.mark.parametrize("with_L", [False, True])
@@ -58,10 +59,15 @@ def test_adjoint_matches_fp64_cpu_adjoint(require_gpu, with_L):
 
 
 def test_adjoint_finite_differences(require_gpu):
-    """dl/d(rho) at 64 random voxels against central differences of the GPU
-    forward, l = sum(c * I) + sum(d * IAbsorb).  The projector is linear in
-    the densities and smooth in L, so an h of 0.1 g/cc leaves the truncation
-    error far below the FP32 resolution of the outputs."""
+    """dl/d(rho) at 64 random voxels against finite differences of the GPU
+    forward, l = sum(c * I) + sum(d * IAbsorb).  The step per material is
+    h_m = 0.4 / max_E A[E][m] over the bins with spectral weight (one voxel
+    then moves any weighted tau by at most ~0.2), and the difference quotient
+    is Richardson-extrapolated from steps h and h/2, which leaves a truncation
+    error ~(A dL)^4 / 100 < 1e-5 while keeping dL large against the FP32
+    rounding of the marched sums.  The denominators are the perturbations as
+    stored in float32, not h itself (at h ~ 1e-3 the rounding of rho + h
+    alone would be a 1e-4 error)."""
     import torch
     s, vols = setup_case(n=32, views=4, det=(56, 44))
     kw = P.spectral_inputs(s, *s.det)
@@ -79,32 +85,40 @@ def test_adjoint_finite_differences(require_gpu):
         return float((I.cpu().numpy().astype(np.float64) * c).sum()
                      + (IA.cpu().numpy().astype(np.float64) * d).sum())
 
+    def quotient(m, z, y, x, h):
+        vp = [v.copy() for v in vols]
+        vm = [v.copy() for v in vols]
+        vp[m][z, y, x] += h
+        vm[m][z, y, x] -= h
+        return (loss(vp) - loss(vm)) / (float(vp[m][z, y, x]) - float(vm[m][z, y, x]))
+
     proj.set_volumes(vols)
     grad = proj.backward(torch.from_numpy(c.astype(np.float32)).to(dev),
                          torch.from_numpy(d.astype(np.float32)).to(dev))
     torch.cuda.synchronize()
     grad = grad.cpu().numpy().astype(np.float64)
+    active = np.asarray(kw["spectrum"]) > 0
+    step = 0.4 / np.asarray(kw["atten"], np.float64)[active].max(axis=0)
     n = vols[0].shape[0]
-    h = 0.1
     errs = []
-    picked = 0
-    while picked < 64:
+    while len(errs) < 64:
         m = int(rng.integers(0, 3))
         z, y, x = (int(v) for v in rng.integers(4, n - 4, size=3))
         if grad[m, z, y, x] == 0.0:
             continue
-        vp = [v.copy() for v in vols]
-        vm = [v.copy() for v in vols]
-        vp[m][z, y, x] += h
-        vm[m][z, y, x] -= h
-        fd = (loss(vp) - loss(vm)) / (2 * h)
-        errs.append(abs(fd - grad[m, z, y, x]) / max(abs(fd), 1e-3 * np.abs(grad).max()))
-        picked += 1
+        h = float(np.float32(step[m]))
+        fd = (4 * quotient(m, z, y, x, h / 2) - quotient(m, z, y, x, h)) / 3
+        errs.append(abs(fd - grad[m, z, y, x]) / max(abs(fd), 1e-3 * np.abs(grad[m]).max()))
     proj.close()
-    assert max(errs) < 1e-3, max(errs)
+    assert max(errs) < 1e-3, (max(errs), np.median(errs))
 
 
 def test_adjoint_lds_equals_global_atomics(require_gpu):
+    """LDS-accumulated chunks against the all-global-atomics fallback (an LDS
+    budget too small for any chunk box).  Both sum in FP32 atomics in an
+    order that varies from run to run, and random-sign upstream gradients
+    make the sums cancel, so they agree to 1e-4 of the floor 1e-3 max|grad|,
+    not bitwise."""
     import torch
     s, vols = setup_case(n=48, views=5, det=(90, 70), d_size=0.9)
     kw = P.spectral_inputs(s, *s.det)
@@ -114,8 +128,10 @@ def test_adjoint_lds_equals_global_atomics(require_gpu):
     with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
         proj.set_volumes(vols)
         proj.set_spectral(**kw)
+        proj.set_tuning(march="lds", chunk=64, lds_bytes=1024)
         ref = proj.backward(gI, None).cpu().numpy()
-        for chunk, lds in [(16, 40 * 1024), (5, 12 * 1024), (32, 2048)]:
-            proj.set_tuning(chunk=chunk, lds_bytes=lds)
+        for march, chunk, lds in [("lds", 16, 40 * 1024), ("lds", 5, 12 * 1024), ("lds", 32, 2048),
+                                  ("global", 16, 40 * 1024)]:
+            proj.set_tuning(march=march, chunk=chunk, lds_bytes=lds)
             got = proj.backward(gI, None).cpu().numpy()
-            assert rel_err(got, ref) < 1e-5, (chunk, lds)
+            assert rel_err(got, ref) < 1e-4, (march, chunk, lds)
