@@ -83,6 +83,78 @@ def accuracy(proj, setup, vols, kw):
     return {"p": rel(pg, pr, 1e-2), "I": rel(I, I64, 1e-3), "sample": "view 0 of 360 vs FP64 oracle"}
 
 
+def run_adjoint(args):
+    """Differentiable projector (3 x 256^3, 512x512, 180 views, grid + CsI):
+    one weighted-least-squares step on log projections against a Poisson-noisy
+    target (1e5 photons per pixel).  Forward = layout + projection launch;
+    backward = layout + adjoint launch + the loss's elementwise ops.  Prints
+    one JSON line with both times, their ratio and peak device memory."""
+    import numpy as np
+    import torch
+
+    import ct_forward_projection_amd as P
+    from ct_forward_projection_amd import autograd as A
+    from ct_forward_projection_amd import synthetic as S
+
+    setup = S.config4()
+    truth = np.stack(S.three_material_phantom(setup.n_vox[0], seed=41))
+    kw = P.spectral_inputs(setup, *setup.det)
+    dev = torch.device("cuda", 0)
+    proj = P.Projector(P.scan_params(**setup.scan_kwargs()))
+    proj.set_spectral(**kw)
+    I0 = proj.flat_field().double()
+    truth_t = torch.from_numpy(truth).to(dev)
+    with torch.no_grad():
+        It, _ = A.project(truth_t, proj)
+    counts = A.poisson_target(It, seed=3).double()
+    p_t = A.log_projection(counts, I0)
+    rng = np.random.default_rng(5)
+    est = (truth_t * torch.from_numpy(rng.uniform(0.9, 1.1, truth.shape).astype(np.float32)).to(dev))
+    est.requires_grad_(True)
+    del It, truth_t
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+
+    def step(timed):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        est.grad = None
+        ev[0].record(stream)
+        I, _ = A.project(est, proj)
+        ev[1].record(stream)
+        loss = A.wls_loss(A.log_projection(I.double(), I0), p_t, counts)
+        loss.backward()
+        ev[2].record(stream)
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), float(loss)
+
+    for i in range(args.warmup):
+        step(False)
+        log(f"adjoint: warmup {i + 1}/{args.warmup} done")
+    torch.cuda.reset_peak_memory_stats(dev)
+    fw, bw, losses = [], [], []
+    for i in range(args.steps):
+        f, b, l = step(True)
+        fw.append(f)
+        bw.append(b)
+        losses.append(l)
+        log(f"adjoint: step {i + 1}: forward {f:.2f} ms, backward {b:.2f} ms, loss {l:.6g}")
+    peak = torch.cuda.max_memory_allocated(dev) + proj.device_bytes
+    f, b = sum(fw) / len(fw), sum(bw) / len(bw)
+    samples = setup.views * proj.samples_per_view
+    out = {"metric": "differentiable projector: backward / forward time", "value": round(b / f, 3),
+           "unit": "ratio", "higher_is_better": False, "steps": args.steps, "warmup": args.warmup,
+           "forward_ms": round(f, 2), "backward_ms": round(b, 2),
+           "forward_gsamples_per_s": round(samples / (f / 1e3) / 1e9, 2),
+           "peak_device_bytes": int(peak), "projector_device_bytes": proj.device_bytes,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "3-material 256^3 (1 mm), 512x512 @ 0.8 mm, 180 views, 120 kVp "
+                                  "101 bins, Pb grid 10:1, CsI; WLS on log projections vs "
+                                  "Poisson target at 1e5 photons/pixel",
+                      "tuning": proj.get_tuning()}}
+    proj.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,11 +176,18 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also launch the copy kernel over this many MiB (counter calibration)")
+    ap.add_argument("--workload", choices=["scan", "adjoint"], default="scan",
+                    help="scan: the 512^3 forward projection (headline); adjoint: one "
+                         "gradient step of the differentiable 256^3 projector")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "views shard with no exchange, see the sharding design notes", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "adjoint":
+        run_adjoint(args)
+        return
 
     import numpy as np
     import torch

@@ -33,7 +33,7 @@ EXPORTED = (
     "ctfp_postprocess", "ctfp_synchronize", "ctfp_scint_thickness", "ctfp_det_response",
     "ctfp_grid_transmission", "ctfp_grid_period_index", "ctfp_bench_copy",
     "ctfp_set_tuning", "ctfp_get_tuning", "ctfp_backward", "ctfp_set_volumes_async",
-    "ctfp_get_scan", "ctfp_get_device",
+    "ctfp_get_scan", "ctfp_get_device", "ctfp_device_bytes",
 )
 
 CTFP_MARCH_GLOBAL = 0
@@ -136,6 +136,7 @@ def _declare(lib):
         "ctfp_set_volumes_async": (i32, [vp, i32, P(vp), i32, vp]),
         "ctfp_get_scan": (i32, [vp, P(ScanParams)]),
         "ctfp_get_device": (i32, [vp, P(i32)]),
+        "ctfp_device_bytes": (i32, [vp, P(i64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -435,25 +435,90 @@ template <int ST> __device__ __forceinline__ uint32_t pack_w(float lo, float hi)
     }
 }
 
-// One sample: integer cell (clamped to [-2, N]; every corner outside the
-// volume is then a padding zero) and FP32 interpolation fractions, from the
-// FP64 position start + j * step.
+// Sample positions.  Every kAnchor-th sample of the ray (global sample index
+// k a multiple of kAnchor) is an anchor whose index-space position is
+// evaluated in FP64 and split into an integer cell and an FP32 offset; the
+// kAnchor samples from it on are offset + i * step in FP32 (|error| <
+// ~3e-6 * max(1, |step|) voxel, against ~1e-4 for the reference's FP32
+// t * d - SOD).  The anchor grid is fixed, so every launch option sees the
+// same positions.
+constexpr int kAnchor = 16;
+
+struct Anchor {
+    int ia, ib, ic;       // cell of the anchor
+    float fa, fb, fc;     // offset of the anchor inside its cell, [0, 1)
+    float da, db, dc;     // per-sample step (FP32)
+};
+__device__ __forceinline__ Anchor anchor_at(const RaySetup &R, int ka)
+{
+    const double j = (double)(ka - R.k_lo);
+    const double pa = fma(j, R.da, R.a0), pb = fma(j, R.db, R.b0), pc = fma(j, R.dc, R.c0);
+    const double ga = floor(pa), gb = floor(pb), gc = floor(pc);
+    Anchor A;
+    A.ia = (int)ga;
+    A.ib = (int)gb;
+    A.ic = (int)gc;
+    A.fa = (float)(pa - ga);
+    A.fb = (float)(pb - gb);
+    A.fc = (float)(pc - gc);
+    A.da = (float)R.da;
+    A.db = (float)R.db;
+    A.dc = (float)R.dc;
+    return A;
+}
+
+// Sample `fi` (0..kAnchor-1) after an anchor: cell relative to the anchor's
+// cell and the FP32 interpolation fractions.
 struct Cell {
     int x, y, z;
     float ax, ay, az;
 };
-__device__ __forceinline__ Cell sample_cell(const RaySetup &R, double jd, int nx, int ny, int nz)
+// floor(x) as an int in one instruction (v_cvt_flr_i32_f32; |x| < 2^24 here).
+__device__ __forceinline__ int floor_to_int(float x)
 {
-    double fa = fma(jd, R.da, R.a0), fb = fma(jd, R.db, R.b0), fc = fma(jd, R.dc, R.c0);
-    double ia = floor(fa), ib = floor(fb), ic = floor(fc);
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+__device__ __forceinline__ Cell offset_cell(const Anchor &A, float fi)
+{
+    const float x = fmaf(fi, A.da, A.fa), y = fmaf(fi, A.db, A.fb), z = fmaf(fi, A.dc, A.fc);
     Cell c;
-    c.ax = (float)(fa - ia);
-    c.ay = (float)(fb - ib);
-    c.az = (float)(fc - ic);
-    c.x = min(max((int)ia, -2), nx);
-    c.y = min(max((int)ib, -2), ny);
-    c.z = min(max((int)ic, -2), nz);
+    // fract(x) = x - floor(x), held below 1.0 (v_fract_f32)
+    c.ax = __builtin_amdgcn_fractf(x);
+    c.ay = __builtin_amdgcn_fractf(y);
+    c.az = __builtin_amdgcn_fractf(z);
+    c.x = floor_to_int(x);
+    c.y = floor_to_int(y);
+    c.z = floor_to_int(z);
     return c;
+}
+
+// Absolute cell clamped to [-2, N] (every corner outside the volume is then a
+// padding zero, as is every corner of an unclamped cell outside [-1, N)).
+__device__ __forceinline__ Cell absolute_cell(const Anchor &A, const Cell &o, int nx, int ny, int nz)
+{
+    Cell c = o;
+    c.x = min(max(A.ia + o.x, -2), nx);
+    c.y = min(max(A.ib + o.y, -2), ny);
+    c.z = min(max(A.ic + o.z, -2), nz);
+    return c;
+}
+
+// Cells that samples [j0, j1) of a ray can touch, from the FP64 positions of
+// the two ends widened by 1e-4 voxel (far above the FP32 offset error), as
+// [lo, hi] with the +1 corner included.
+__device__ __forceinline__ void sample_box(const RaySetup &R, int j0, int j1, int (&lo)[3], int (&hi)[3])
+{
+    const double ja = (double)j0, jb = (double)(j1 - 1);
+    const double a[3] = {fma(ja, R.da, R.a0), fma(ja, R.db, R.b0), fma(ja, R.dc, R.c0)};
+    const double b[3] = {fma(jb, R.da, R.a0), fma(jb, R.db, R.b0), fma(jb, R.dc, R.c0)};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        lo[d] = (int)floor(fmin(a[d], b[d]) - 1e-4);
+        hi[d] = (int)floor(fmax(a[d], b[d]) + 1e-4) + 1;
+    }
 }
 
 // Trilinear accumulation of one sample into the M per-material sums from
@@ -636,9 +701,14 @@ __global__ __launch_bounds__(kBlock) void march_kernel(MarchArgs P)
     float acc[M];
 #pragma unroll
     for (int m = 0; m < M; m++) acc[m] = 0.0f;
-    double jd = 0.0;
-    for (int j = 0; j < R.n; j++, jd += 1.0)
-        accumulate_global<M, ST>(rsrc, sample_cell(R, jd, nx, ny, nz), oy, oz, acc);
+    const int k_hi = R.k_lo + R.n;
+    for (int ka = R.k_lo & ~(kAnchor - 1); ka < k_hi; ka += kAnchor) {
+        const Anchor A = anchor_at(R, ka);
+        const int i1 = min(ka + kAnchor, k_hi) - ka;
+        int i = max(ka, R.k_lo) - ka;
+        for (float fi = (float)i; i < i1; i++, fi += 1.0f)
+            accumulate_global<M, ST>(rsrc, absolute_cell(A, offset_cell(A, fi), nx, ny, nz), oy, oz, acc);
+    }
     if (t.active) finish_ray<M, kSpectrum>(P, t, acc);
 }
 
@@ -703,49 +773,75 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
     int kr_lo[3] = {R.n > 0 ? R.k_lo : INT_MAX, 0, 0}, kr_hi[3] = {R.n > 0 ? k_hi : INT_MIN, 0, 0};
     __syncthreads();
     block_box(kr_lo, kr_hi, scratch);
-    const int kb_lo = kr_lo[0], kb_hi = kr_hi[0];
-    for (int k0 = kb_lo; k0 < kb_hi; k0 += P.chunk) {
-        const int j0 = max(k0, R.k_lo) - R.k_lo, j1 = min(k0 + P.chunk, k_hi) - R.k_lo;
+    const int kb_hi = kr_hi[0];
+    // chunks start on the anchor grid (chunk is a multiple of kAnchor)
+    for (int k0 = kr_lo[0] & ~(kAnchor - 1); k0 < kb_hi; k0 += P.chunk) {
+        const int c_lo = max(k0, R.k_lo), c_hi = min(k0 + P.chunk, k_hi);
         int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-        if (j1 > j0) { // positions are monotone in j: the chunk's ends bound its cells
-            Cell a = sample_cell(R, (double)j0, nx, ny, nz), b = sample_cell(R, (double)(j1 - 1), nx, ny, nz);
-            lo[0] = min(a.x, b.x); lo[1] = min(a.y, b.y); lo[2] = min(a.z, b.z);
-            hi[0] = max(a.x, b.x) + 1; hi[1] = max(a.y, b.y) + 1; hi[2] = max(a.z, b.z) + 1;
-        }
+        if (c_hi > c_lo) sample_box(R, c_lo - R.k_lo, c_hi - R.k_lo, lo, hi);
         __syncthreads(); // scratch and staged cells of the previous chunk are free
         block_box(lo, hi, scratch);
         if (hi[0] < lo[0]) continue; // no lane of the block samples this chunk (block-uniform)
         const int sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
-        const bool staged = sx > 0 && (int64_t)sx * sy * sz <= (int64_t)P.lds_cells;
+        const bool staged = (int64_t)sx * sy * sz <= (int64_t)P.lds_cells;
         if (staged) {
-            // cooperative, row-coalesced fill of the box [lo, hi]
+            // cooperative, row-coalesced fill of the box [lo, hi]; cells
+            // outside the padded volume are zeros
             const int cells = sx * sy * sz;
             const float inv_sx = 1.0f / (float)sx, inv_sy = 1.0f / (float)sy;
             for (int i = threadIdx.x; i < cells; i += kBlock) {
                 const int row = small_div(i, sx, inv_sx), lx = i - row * sx;
                 const int lz = small_div(row, sy, inv_sy), ly = row - lz * sy;
-                const uint32_t xi = (uint32_t)(lo[0] + lx + kVoxPadLo) & 0xFFFFFu;
-                const uint32_t yi = (uint32_t)(lo[1] + ly + kVoxPadLo) & 0xFFFFFu;
-                const uint32_t zi = (uint32_t)(lo[2] + lz + kVoxPadLo) & 0xFFFFFu;
+                const int x = lo[0] + lx, y = lo[1] + ly, z = lo[2] + lz;
                 uint32_t d[M];
-                load_raw<M>(rsrc, zi * oz + (yi * oy + xi * (4u * M)), 0, d);
+                if (x >= -kVoxPadLo && x < nx + kVoxPadHi && y >= -kVoxPadLo && y < ny + kVoxPadHi &&
+                    z >= -kVoxPadLo && z < nz + kVoxPadHi) {
+                    const uint32_t xi = (uint32_t)(x + kVoxPadLo) & 0xFFFFFu;
+                    const uint32_t yi = (uint32_t)(y + kVoxPadLo) & 0xFFFFFu;
+                    const uint32_t zi = (uint32_t)(z + kVoxPadLo) & 0xFFFFFu;
+                    load_raw<M>(rsrc, zi * oz + (yi * oy + xi * (4u * M)), 0, d);
+                } else {
+#pragma unroll
+                    for (int m = 0; m < M; m++) d[m] = 0u;
+                }
                 lds_write<M>(lds, (uint32_t)i * LB, d);
             }
             __syncthreads();
-            const uint32_t ry = (uint32_t)sx * LB, rz = (uint32_t)(sx * sy) * LB;
-            const uint32_t roff[4] = {0u, ry, rz, ry + rz};
-            double jd = (double)j0;
-            for (int j = j0; j < j1; j++, jd += 1.0) {
-                const Cell c = sample_cell(R, jd, nx, ny, nz);
-                const uint32_t base = (uint32_t)(((c.z - lo[2]) * sy + (c.y - lo[1])) * sx + (c.x - lo[0])) * LB;
-                accumulate<M, ST>(c, [&](int r, uint32_t(&out)[M]) {
-                    lds_read<M>(lds, base + roff[r & 3] + (r < 4 ? 0u : LB), out);
-                }, acc);
+            const int ry = sx * (int)LB, rz = sx * sy * (int)LB;
+            const uint32_t roff[4] = {0u, (uint32_t)ry, (uint32_t)rz, (uint32_t)(ry + rz)};
+            for (int ka = k0; ka < k0 + P.chunk; ka += kAnchor) {
+                const int i1 = min(ka + kAnchor, c_hi) - ka;
+                int i = max(ka, c_lo) - ka;
+                if (i >= i1) continue;
+                const Anchor A = anchor_at(R, ka);
+                // byte offset of the anchor's cell in the box; samples add
+                // their (small, signed) cell offsets with 24-bit multiplies
+                const char *pa =
+                    lds + (((A.ic - lo[2]) * sy + (A.ib - lo[1])) * sx + (A.ia - lo[0])) * (int)LB;
+                auto sample = [&](float fi) {
+                    const Cell c = offset_cell(A, fi);
+                    const char *pc = pa + (c.x * (int)LB + __mul24(c.y, ry) + __mul24(c.z, rz));
+                    accumulate<M, ST>(c, [&](int r, uint32_t(&out)[M]) {
+                        lds_read<M>(pc, roff[r & 3] + (r < 4 ? 0u : LB), out);
+                    }, acc);
+                };
+                if (i == 0 && i1 == kAnchor) { // a whole segment (most of them)
+#pragma unroll
+                    for (int q = 0; q < kAnchor; q++) sample((float)q);
+                } else {
+                    for (float fi = (float)i; i < i1; i++, fi += 1.0f) sample(fi);
+                }
             }
         } else {
-            double jd = (double)j0;
-            for (int j = j0; j < j1; j++, jd += 1.0)
-                accumulate_global<M, ST>(rsrc, sample_cell(R, jd, nx, ny, nz), oy, oz, acc);
+            for (int ka = k0; ka < k0 + P.chunk; ka += kAnchor) {
+                const int i1 = min(ka + kAnchor, c_hi) - ka;
+                int i = max(ka, c_lo) - ka;
+                if (i >= i1) continue;
+                const Anchor A = anchor_at(R, ka);
+                for (float fi = (float)i; i < i1; i++, fi += 1.0f)
+                    accumulate_global<M, ST>(rsrc, absolute_cell(A, offset_cell(A, fi), nx, ny, nz), oy,
+                                             oz, acc);
+            }
         }
     }
 }
@@ -805,7 +901,7 @@ __device__ __forceinline__ void scatter_chunks(const MarchArgs &P, const RaySetu
     int kr_lo[3] = {R.n > 0 ? R.k_lo : INT_MAX, 0, 0}, kr_hi[3] = {R.n > 0 ? k_hi : INT_MIN, 0, 0};
     __syncthreads();
     block_box(kr_lo, kr_hi, scratch);
-    const int kb_lo = kr_lo[0], kb_hi = kr_hi[0];
+    const int kb_hi = kr_hi[0];
     auto add_global = [&](int x, int y, int z, float w) {
         if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz || w == 0.0f) return;
         const int64_t q = ((int64_t)z * ny + y) * nx + x;
@@ -813,37 +909,40 @@ __device__ __forceinline__ void scatter_chunks(const MarchArgs &P, const RaySetu
         for (int m = 0; m < M; m++)
             if (g[m] != 0.0f) unsafeAtomicAdd(grad + m * nvox + q, g[m] * w);
     };
-    for (int k0 = kb_lo; k0 < kb_hi; k0 += P.chunk) {
-        const int j0 = max(k0, R.k_lo) - R.k_lo, j1 = min(k0 + P.chunk, k_hi) - R.k_lo;
+    for (int k0 = kr_lo[0] & ~(kAnchor - 1); k0 < kb_hi; k0 += P.chunk) {
+        const int c_lo = max(k0, R.k_lo), c_hi = min(k0 + P.chunk, k_hi);
         int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-        if (j1 > j0) {
-            Cell a = sample_cell(R, (double)j0, nx, ny, nz), b = sample_cell(R, (double)(j1 - 1), nx, ny, nz);
-            lo[0] = min(a.x, b.x); lo[1] = min(a.y, b.y); lo[2] = min(a.z, b.z);
-            hi[0] = max(a.x, b.x) + 1; hi[1] = max(a.y, b.y) + 1; hi[2] = max(a.z, b.z) + 1;
-        }
+        if (c_hi > c_lo) sample_box(R, c_lo - R.k_lo, c_hi - R.k_lo, lo, hi);
         __syncthreads();
         block_box(lo, hi, scratch);
         if (hi[0] < lo[0]) continue; // no lane of the block samples this chunk (block-uniform)
         const int sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
-        const bool staged = sx > 0 && (int64_t)sx * sy * sz <= (int64_t)P.lds_cells;
+        const bool staged = (int64_t)sx * sy * sz <= (int64_t)P.lds_cells;
         if (staged) {
             const int cells = sx * sy * sz;
             for (int i = threadIdx.x; i < cells * (int)LW; i += kBlock) box[i] = 0.0f;
             __syncthreads();
-            double jd = (double)j0;
-            for (int j = j0; j < j1; j++, jd += 1.0) {
-                const Cell c = sample_cell(R, jd, nx, ny, nz);
-                const float bx = 1.0f - c.ax, by = 1.0f - c.ay, bz = 1.0f - c.az;
-                const float wr[4] = {by * bz, c.ay * bz, by * c.az, c.ay * c.az};
-                const int base = ((c.z - lo[2]) * sy + (c.y - lo[1])) * sx + (c.x - lo[0]);
-                const int roff[4] = {0, sx, sx * sy, sx * sy + sx};
+            const int ry = sx * (int)LW, rz = sx * sy * (int)LW;
+            const int roff[4] = {0, ry, rz, ry + rz};
+            for (int ka = k0; ka < k0 + P.chunk; ka += kAnchor) {
+                const int i1 = min(ka + kAnchor, c_hi) - ka;
+                int i = max(ka, c_lo) - ka;
+                if (i >= i1) continue;
+                const Anchor A = anchor_at(R, ka);
+                const int base = (((A.ic - lo[2]) * sy + (A.ib - lo[1])) * sx + (A.ia - lo[0])) * (int)LW;
+                for (float fi = (float)i; i < i1; i++, fi += 1.0f) {
+                    const Cell c = offset_cell(A, fi);
+                    const float bx = 1.0f - c.ax, by = 1.0f - c.ay, bz = 1.0f - c.az;
+                    const float wr[4] = {by * bz, c.ay * bz, by * c.az, c.ay * c.az};
+                    float *p = box + (base + c.x * (int)LW + __mul24(c.y, ry) + __mul24(c.z, rz));
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    float *p0 = box + (uint32_t)(base + roff[r]) * LW;
+                    for (int r = 0; r < 4; r++) {
+                        float *p0 = p + roff[r];
 #pragma unroll
-                    for (int m = 0; m < M; m++) {
-                        atomicAdd(p0 + m, g[m] * (bx * wr[r]));
-                        atomicAdd(p0 + LW + m, g[m] * (c.ax * wr[r]));
+                        for (int m = 0; m < M; m++) {
+                            atomicAdd(p0 + m, g[m] * (bx * wr[r]));
+                            atomicAdd(p0 + LW + m, g[m] * (c.ax * wr[r]));
+                        }
                     }
                 }
             }
@@ -862,16 +961,22 @@ __device__ __forceinline__ void scatter_chunks(const MarchArgs &P, const RaySetu
                 }
             }
         } else {
-            double jd = (double)j0;
-            for (int j = j0; j < j1; j++, jd += 1.0) {
-                const Cell c = sample_cell(R, jd, nx, ny, nz);
-                const float bx = 1.0f - c.ax, by = 1.0f - c.ay, bz = 1.0f - c.az;
-                const float wr[4] = {by * bz, c.ay * bz, by * c.az, c.ay * c.az};
-                const int dy[4] = {0, 1, 0, 1}, dz[4] = {0, 0, 1, 1};
+            for (int ka = k0; ka < k0 + P.chunk; ka += kAnchor) {
+                const int i1 = min(ka + kAnchor, c_hi) - ka;
+                int i = max(ka, c_lo) - ka;
+                if (i >= i1) continue;
+                const Anchor A = anchor_at(R, ka);
+                for (float fi = (float)i; i < i1; i++, fi += 1.0f) {
+                    const Cell o = offset_cell(A, fi);
+                    const int x = A.ia + o.x, y = A.ib + o.y, z = A.ic + o.z;
+                    const float bx = 1.0f - o.ax, by = 1.0f - o.ay, bz = 1.0f - o.az;
+                    const float wr[4] = {by * bz, o.ay * bz, by * o.az, o.ay * o.az};
+                    const int dy[4] = {0, 1, 0, 1}, dz[4] = {0, 0, 1, 1};
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    add_global(c.x, c.y + dy[r], c.z + dz[r], bx * wr[r]);
-                    add_global(c.x + 1, c.y + dy[r], c.z + dz[r], c.ax * wr[r]);
+                    for (int r = 0; r < 4; r++) {
+                        add_global(x, y + dy[r], z + dz[r], bx * wr[r]);
+                        add_global(x + 1, y + dy[r], z + dz[r], o.ax * wr[r]);
+                    }
                 }
             }
         }
@@ -1225,6 +1330,20 @@ int ctfp_get_scan(const ctfp_context *ctx, ctfp_scan_params *out)
     return CTFP_OK;
 }
 
+int ctfp_device_bytes(const ctfp_context *ctx, int64_t *bytes)
+{
+    if (check_ctx(ctx) || !bytes) return set_error(CTFP_ERR_INVALID, "null argument");
+    const int64_t npix = ctx->scan.d_num_u * ctx->scan.d_num_v;
+    int64_t b = ctx->vol ? (int64_t)ctx->vol_bytes : 0;
+    if (ctx->A) b += (int64_t)ctx->n_act * ctx->spec_n_mat * (int64_t)sizeof(float);
+    if (ctx->W) b += (int64_t)std::max(ctx->n_act, 1) * ctx->w_stride_e * (int64_t)sizeof(float);
+    if (ctx->I0) b += npix * (int64_t)sizeof(float);
+    if (ctx->view_cs) b += ctx->scan.proj_num * (int64_t)sizeof(double2);
+    if (ctx->dI) b += 2 * ctx->staged_views * npix * (int64_t)sizeof(float);
+    *bytes = b;
+    return CTFP_OK;
+}
+
 int ctfp_get_device(const ctfp_context *ctx, int *device)
 {
     if (check_ctx(ctx) || !device) return set_error(CTFP_ERR_INVALID, "null argument");
@@ -1412,7 +1531,9 @@ int ctfp_set_tuning(ctfp_context *ctx, const ctfp_tuning *t)
     if (!t) return set_error(CTFP_ERR_INVALID, "null tuning");
     if (t->march != CTFP_MARCH_GLOBAL && t->march != CTFP_MARCH_LDS)
         return set_error(CTFP_ERR_INVALID, "tuning: march mode %d", t->march);
-    if (t->chunk < 1 || t->chunk > 4096) return set_error(CTFP_ERR_INVALID, "tuning: chunk %d", t->chunk);
+    if (t->chunk < kAnchor || t->chunk > 4096 || t->chunk % kAnchor != 0)
+        return set_error(CTFP_ERR_INVALID, "tuning: chunk %d is not a multiple of %d in [%d, 4096]",
+                         t->chunk, kAnchor, kAnchor);
     if (t->lds_bytes < 1024 || t->lds_bytes > 64 * 1024)
         return set_error(CTFP_ERR_INVALID, "tuning: lds_bytes %d outside [1 KiB, 64 KiB]", t->lds_bytes);
     ctx->tune = *t;

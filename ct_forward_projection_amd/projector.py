@@ -149,6 +149,13 @@ class Projector:
             raise L.CTFPError(L.CTFP_ERR_STATE, "the projector is closed")
         return int(self._ctx.value)
 
+    @property
+    def device_bytes(self) -> int:
+        """Device memory held by the projector itself (volume layout, tables)."""
+        b = C.c_int64()
+        L.check(L.lib().ctfp_device_bytes(self._ctx, C.byref(b)))
+        return int(b.value)
+
     def get_tuning(self) -> dict:
         t = L.Tuning()
         L.check(L.lib().ctfp_get_tuning(self._ctx, C.byref(t)))

@@ -279,3 +279,15 @@ def config2() -> ScanSetup:
                      grid=dict(strip_width=36, interspace_width=214, height_mm=2.14,
                                focal_mm=1500.0, strip_density=11.35),
                      scint_thickness=0.6)
+
+
+def config4() -> ScanSetup:
+    """Differentiable projector: 3 x 256^3 water/bone/iodine, 1 mm voxels;
+    512x512 detector, 0.8 mm; 180 views; 120 kVp, 101 bins (20-120 keV);
+    the config-2 Pb grid and 0.6 mm CsI response."""
+    s = config2()
+    s.name = "config4"
+    s.n_vox, s.length_mm = (256, 256, 256), (256.0, 256.0, 256.0)
+    s.det, s.d_size, s.views = (512, 512), 0.8, 180
+    return s
+

@@ -111,7 +111,7 @@ typedef struct ctfp_grid_params {
 #define CTFP_MARCH_LDS 1    /* per-chunk volume boxes staged in LDS (default) */
 typedef struct ctfp_tuning {
     int32_t march;     /* CTFP_MARCH_GLOBAL or CTFP_MARCH_LDS */
-    int32_t chunk;     /* samples per staged chunk (LDS mode), 1..4096 */
+    int32_t chunk;     /* samples per staged chunk (LDS mode): a multiple of 16 in [16, 4096] */
     int32_t lds_bytes; /* LDS per workgroup (LDS mode), 1 KiB..64 KiB */
 } ctfp_tuning;
 
@@ -151,6 +151,8 @@ int ctfp_set_volumes_async(ctfp_context *ctx, int n_mat, const float *const *vol
 /* The scan parameters and device of a context (accessors for bindings). */
 int ctfp_get_scan(const ctfp_context *ctx, ctfp_scan_params *out);
 int ctfp_get_device(const ctfp_context *ctx, int *device);
+/* Device memory the context holds (volume layout, spectral tables, staging). */
+int ctfp_device_bytes(const ctfp_context *ctx, int64_t *bytes);
 
 /* Build the density channels from a uint8 label volume on the device.
  * single_material = 0: channel m = density[m] where label == m+1, else 0

@@ -55,7 +55,12 @@ def test_adjoint_matches_fp64_cpu_adjoint(require_gpu, with_L):
     gL = O.spectrum_grad_f64(L64, kw["atten"], kw["spectrum"], kw["i0_val"], gI, gIA, kw["grid"],
                              det_table(kw))
     ref = O.backproject_f64(g, p, gL)
-    assert rel_err(grad, ref) < 1e-4
+    # Each voxel's gradient sums hundreds of rays' contributions of both
+    # signs, so its rounding scales with the sum of their magnitudes: the
+    # backprojection of |dl/dL|.
+    scale = O.backproject_f64(g, p, np.abs(gL))
+    err = np.abs(grad - ref) / np.maximum(scale, 1e-3 * scale.max())
+    assert err.max() < 1e-4, err.max()
 
 
 def test_adjoint_finite_differences(require_gpu):
@@ -130,7 +135,7 @@ def test_adjoint_lds_equals_global_atomics(require_gpu):
         proj.set_spectral(**kw)
         proj.set_tuning(march="lds", chunk=64, lds_bytes=1024)
         ref = proj.backward(gI, None).cpu().numpy()
-        for march, chunk, lds in [("lds", 16, 40 * 1024), ("lds", 5, 12 * 1024), ("lds", 32, 2048),
+        for march, chunk, lds in [("lds", 16, 40 * 1024), ("lds", 48, 12 * 1024), ("lds", 32, 2048),
                                   ("global", 16, 40 * 1024)]:
             proj.set_tuning(march=march, chunk=chunk, lds_bytes=lds)
             got = proj.backward(gI, None).cpu().numpy()

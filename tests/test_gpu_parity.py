@@ -406,11 +406,12 @@ def test_lds_staging_equals_global_gather(require_gpu, dtype):
         proj.set_spectral(**kw)
         proj.set_tuning(march="global")
         ref = proj.forward(want_L=True)
-        for chunk, lds in [(16, 40 * 1024), (1, 40 * 1024), (7, 8 * 1024), (64, 64 * 1024), (32, 2048)]:
+        for chunk, lds in [(16, 40 * 1024), (48, 40 * 1024), (16, 8 * 1024), (64, 64 * 1024), (32, 2048)]:
             proj.set_tuning(march="lds", chunk=chunk, lds_bytes=lds)
             got = proj.forward(want_L=True)
             torch.cuda.synchronize()
             for a, b in zip(got, ref):
                 assert torch.equal(a, b), (chunk, lds)
-        with pytest.raises(P.CTFPError):
-            proj.set_tuning(chunk=0)
+        for bad in (0, 1, 7, 24, 4112):
+            with pytest.raises(P.CTFPError):
+                proj.set_tuning(chunk=bad)
