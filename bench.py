@@ -155,6 +155,113 @@ def run_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def batch_specs(seed: int = 2024):
+    """The eight uneven scans of the batched workload (seeded): cubic phantoms
+    of {128, 192, 256, 384, 512} voxels at 0.5 mm with 1-3 materials,
+    detectors from 256x256 to 1024x768, 60-360 views, 80-140 kVp spectra
+    (61-121 active 1-keV bins from 20 keV), grid on or off."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    dets = [(256, 256), (512, 384), (512, 512), (768, 576), (1024, 768)]
+    out = []
+    for k in range(8):
+        n = int(rng.choice([128, 192, 256, 384, 512]))
+        out.append(dict(n=n, n_mat=int(rng.integers(1, 4)), det=dets[int(rng.integers(0, len(dets)))],
+                        views=int(rng.integers(60, 361)), kvp=float(rng.choice(np.arange(80, 141, 10))),
+                        grid=bool(rng.random() < 0.5), seed=100 + k))
+    return out
+
+
+def batch_setup(spec):
+    from ct_forward_projection_amd import synthetic as S
+    s = S.config2()
+    n = spec["n"]
+    s.name = f"batch{spec['seed']}"
+    s.n_vox, s.length_mm = (n, n, n), (n * 0.5,) * 3
+    s.det, s.views, s.kvp = spec["det"], spec["views"], spec["kvp"]
+    # the detector spans the phantom's diagonal at 1.5x magnification
+    s.d_size = float(n * 0.5 * 1.45 * 1.5 / min(spec["det"]))
+    s.materials = s.materials[:spec["n_mat"]]
+    if not spec["grid"]:
+        s.grid = None
+    vols = list(S.three_material_phantom(n, seed=spec["seed"]))[:spec["n_mat"]]
+    return s, vols
+
+
+def run_batch(args):
+    """Eight uneven scans in one persistent, graph-captured launch (FP16
+    storage, each scan autotuned), against the same eight run one after the
+    other through the plain FP32 projector.  Prints one JSON line."""
+    import torch
+
+    import ct_forward_projection_amd as P
+
+    dev = torch.device("cuda", 0)
+    specs = batch_specs()
+    projs, plain, items = [], [], []
+    tuned = []
+    for sp in specs:
+        s, vols = batch_setup(sp)
+        kw = P.spectral_inputs(s, *s.det)
+        pr = P.Projector(P.scan_params(**s.scan_kwargs()))
+        pr.set_volumes(vols, storage_dtype=P.CTFP_F16)
+        pr.set_spectral(**kw)
+        # one LDS budget serves every scan of the persistent launch
+        tuned.append(pr.autotune(max_lds_bytes=40 * 1024))
+        I = torch.empty((s.views, s.det[1], s.det[0]), device=dev)
+        items.append((pr, 0, s.views, I, torch.empty_like(I)))
+        projs.append(pr)
+        pl = P.Projector(P.scan_params(**s.scan_kwargs()))
+        pl.set_volumes(vols, storage_dtype=P.CTFP_F32)
+        pl.set_spectral(**kw)
+        pl.set_tuning(march="global")
+        plain.append(pl)
+        log(f"scan {len(projs)}: {sp} tuned {tuned[-1]}")
+    batch = P.Batch(items)
+    samples = batch.samples
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, label):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        log(f"{label}: {ms:.2f} ms per pass")
+        return ms
+
+    ms_batch = timed(lambda: batch.run(stream), "batch (FP16, persistent graph)")
+    outs = [(torch.empty_like(I), torch.empty_like(IA)) for _, _, _, I, IA in items]
+
+    def sequential(ps):
+        for pr, (I, IA), it in zip(ps, outs, items):
+            pr.forward(0, it[2], I, IA, stream=stream)
+
+    ms_plain = timed(lambda: sequential(plain), "sequential plain FP32")
+    ms_seq16 = timed(lambda: sequential(projs), "sequential FP16 (tuned, one launch each)")
+    # the batch and the per-scan FP16 launches compute the same projections
+    same = all(torch.equal(I, o[0]) and torch.equal(IA, o[1]) for (_, _, _, I, IA), o in zip(items, outs))
+    out = {"metric": "batched uneven workload: aggregate Gsamples/s", "value": round(samples / ms_batch / 1e6, 2),
+           "unit": "Gsamples/s", "higher_is_better": True, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_pass": round(ms_batch, 2), "samples_per_pass": samples,
+           "plain_fp32_sequential_value": round(samples / ms_plain / 1e6, 2),
+           "speedup_vs_plain_fp32_sequential": round(ms_plain / ms_batch, 3),
+           "fp16_sequential_value": round(samples / ms_seq16 / 1e6, 2),
+           "batch_equals_per_scan_launches": same, "dtype": "f16", "data": "synthetic",
+           "config": {"workload": "8 seeded scans: cubic phantoms 128-512 (0.5 mm), 1-3 materials, "
+                                  "detectors 256x256-1024x768, 60-360 views, 80-140 kVp, grid on/off",
+                      "scans": specs, "tuning": tuned}}
+    batch.close()
+    for p in projs + plain:
+        p.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,9 +283,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also launch the copy kernel over this many MiB (counter calibration)")
-    ap.add_argument("--workload", choices=["scan", "adjoint"], default="scan",
+    ap.add_argument("--workload", choices=["scan", "adjoint", "batch"], default="scan",
                     help="scan: the 512^3 forward projection (headline); adjoint: one "
-                         "gradient step of the differentiable 256^3 projector")
+                         "gradient step of the differentiable 256^3 projector; batch: eight "
+                         "uneven scans in one persistent launch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -187,6 +295,9 @@ def main():
 
     if args.workload == "adjoint":
         run_adjoint(args)
+        return
+    if args.workload == "batch":
+        run_batch(args)
         return
 
     import numpy as np

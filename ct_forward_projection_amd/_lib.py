@@ -33,11 +33,14 @@ EXPORTED = (
     "ctfp_postprocess", "ctfp_synchronize", "ctfp_scint_thickness", "ctfp_det_response",
     "ctfp_grid_transmission", "ctfp_grid_period_index", "ctfp_bench_copy",
     "ctfp_set_tuning", "ctfp_get_tuning", "ctfp_backward", "ctfp_set_volumes_async",
-    "ctfp_get_scan", "ctfp_get_device", "ctfp_device_bytes",
+    "ctfp_get_scan", "ctfp_get_device", "ctfp_device_bytes", "ctfp_autotune",
+    "ctfp_batch_create", "ctfp_batch_run", "ctfp_batch_samples", "ctfp_batch_destroy",
 )
 
 CTFP_MARCH_GLOBAL = 0
 CTFP_MARCH_LDS = 1
+CTFP_ADJOINT_GATHER = 0
+CTFP_ADJOINT_SCATTER = 1
 
 
 class ScanParams(C.Structure):
@@ -94,7 +97,14 @@ class GridParams(C.Structure):
 
 class Tuning(C.Structure):
     """ctfp_tuning: ray-march launch options (results do not depend on them)."""
-    _fields_ = [("march", C.c_int32), ("chunk", C.c_int32), ("lds_bytes", C.c_int32)]
+    _fields_ = [("march", C.c_int32), ("chunk", C.c_int32), ("lds_bytes", C.c_int32),
+                ("wave_u", C.c_int32), ("block_u", C.c_int32), ("adjoint", C.c_int32)]
+
+
+class BatchItem(C.Structure):
+    """ctfp_batch_item: one scan of a batched launch."""
+    _fields_ = [("ctx", C.c_void_p), ("view_begin", C.c_int64), ("view_count", C.c_int64),
+                ("I", C.c_void_p), ("IAbsorb", C.c_void_p)]
 
 
 class CTFPError(RuntimeError):
@@ -137,6 +147,11 @@ def _declare(lib):
         "ctfp_get_scan": (i32, [vp, P(ScanParams)]),
         "ctfp_get_device": (i32, [vp, P(i32)]),
         "ctfp_device_bytes": (i32, [vp, P(i64)]),
+        "ctfp_autotune": (i32, [vp, i32, P(Tuning)]),
+        "ctfp_batch_create": (i32, [P(BatchItem), i32, P(vp)]),
+        "ctfp_batch_run": (i32, [vp, vp]),
+        "ctfp_batch_samples": (i32, [vp, P(i64)]),
+        "ctfp_batch_destroy": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -94,7 +94,12 @@ struct ctfp_context {
     float *dI = nullptr, *dIA = nullptr;
     int64_t staged_views = 0;
     // launch tuning
-    ctfp_tuning tune{CTFP_MARCH_LDS, 16, 40 * 1024};
+    ctfp_tuning tune{CTFP_MARCH_LDS, 16, 40 * 1024, 8, 2, CTFP_ADJOINT_GATHER};
+    // dl/dL workspace of the gather adjoint
+    float *adj_ws = nullptr;
+    size_t adj_ws_bytes = 0;
+    // bumped by every change of volumes or spectral tables (batches check it)
+    uint64_t generation = 0;
 };
 
 namespace {
@@ -288,11 +293,12 @@ struct MarchArgs {
     // LDS staging (march_lds_kernel)
     int32_t chunk;     // samples per staged chunk
     int32_t lds_cells; // staged cells that fit the LDS budget
+    // ray tile: wavefronts of (1 << wu_log2) x (64 >> wu_log2) pixels,
+    // (1 << bu_log2) x (4 >> bu_log2) wavefronts per workgroup
+    int32_t wu_log2, bu_log2;
 };
 
-constexpr int kWaveU = 8, kWaveV = 8;  // pixels per wavefront
-constexpr int kTileU = 16, kTileV = 16; // pixels per workgroup (4 wavefronts)
-constexpr int kBlock = 256;
+constexpr int kBlock = 256; // 4 wavefronts per workgroup
 
 template <int M>
 __device__ __forceinline__ void load_vox(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
@@ -337,7 +343,13 @@ struct RaySetup {
     int k_lo, n;
 };
 
-__device__ __forceinline__ RaySetup setup_ray(const MarchArgs &P, int u, int v, int vw)
+// Index-space line of ray (u, v) in the view with rotation cs = (cos, sin):
+// position(t) = src + t * dir, t in mm from the source (FP64).
+struct RayLine {
+    double sa, sb, sc; // source (t = 0)
+    double da, db, dc; // d(index)/dt
+};
+__device__ __forceinline__ RayLine ray_line(const MarchArgs &P, int u, int v, double2 cs)
 {
     // computeIntPointCoordinatesKernel (CBCTPolyProjGPU.cu:1813-1829), FP64.
     double detU = P.half_u - P.dsize * 0.5 - (double)u * P.dsize;
@@ -351,15 +363,39 @@ __device__ __forceinline__ RaySetup setup_ray(const MarchArgs &P, int u, int v, 
     double cb = sqrt(sdd2 + detV * detV) * inv_r;
     double dX = cg * cb, dY = cg * sb, dZ = sg;
     // transformKernel (CBCTPolyProjGPU.cu:1915-1917) is affine in t.
-    double2 cs = P.view_cs[vw];
-    double ca = (dX * cs.x - dY * cs.y) * P.inv_px;         // d(a)/dt
-    double cb_ = -(dX * cs.y + dY * cs.x) * P.inv_py;       // d(b)/dt
-    double cc = -dZ * P.inv_pz;                             // d(c)/dt
-    double a_off = (P.half_x - P.sod * cs.x) * P.inv_px;    // a at t = 0
-    double b_off = (P.half_y + P.sod * cs.y) * P.inv_py;
-    double c_off = P.half_z * P.inv_pz;
-    double sa = a_off + ca * P.t0, sb2 = b_off + cb_ * P.t0, sc = c_off + cc * P.t0; // k = 0
-    double da = ca * P.dx, db = cb_ * P.dx, dc = cc * P.dx;
+    RayLine L;
+    L.da = (dX * cs.x - dY * cs.y) * P.inv_px;
+    L.db = -(dX * cs.y + dY * cs.x) * P.inv_py;
+    L.dc = -dZ * P.inv_pz;
+    L.sa = (P.half_x - P.sod * cs.x) * P.inv_px;
+    L.sb = (P.half_y + P.sod * cs.y) * P.inv_py;
+    L.sc = P.half_z * P.inv_pz;
+    return L;
+}
+
+// Continuous detector coordinates (u, v) of the ray through the index-space
+// point (a, b, c) in the view cs: the inverse of ray_line's direction model
+// (azimuth tan = detU / sqrt(SDD^2 + detV^2), elevation sin = detV / r).
+__device__ __forceinline__ void project_point(const MarchArgs &P, double2 cs, double a, double b, double c,
+                                              double &u, double &v)
+{
+    const double qx = a / P.inv_px - P.half_x + P.sod * cs.x; // world, relative to the source
+    const double qy = b / P.inv_py - P.half_y - P.sod * cs.y;
+    const double qz = c / P.inv_pz - P.half_z;
+    const double X = qx * cs.x - qy * cs.y, Y = -qx * cs.y - qy * cs.x, Z = -qz;
+    const double al = Z / sqrt(X * X + Y * Y), be = Y / X;
+    const double den = 1.0 - al * al * be * be;
+    const double detV = al * P.sdd * sqrt((1.0 + be * be) / den);
+    const double detU = be * P.sdd * sqrt((1.0 + al * al) / den);
+    u = (P.half_u - P.dsize * 0.5 - detU) / P.dsize;
+    v = (P.half_v - P.dsize * 0.5 - detV) / P.dsize;
+}
+
+__device__ __forceinline__ RaySetup setup_ray(const MarchArgs &P, int u, int v, int vw)
+{
+    const RayLine Ln = ray_line(P, u, v, P.view_cs[vw]);
+    double sa = Ln.sa + Ln.da * P.t0, sb2 = Ln.sb + Ln.db * P.t0, sc = Ln.sc + Ln.dc * P.t0; // k = 0
+    double da = Ln.da * P.dx, db = Ln.db * P.dx, dc = Ln.dc * P.dx;
     double lo = 0.0, hi = (double)P.K - 1.0;
     clip_axis(sa, da, (double)P.nx, lo, hi);
     clip_axis(sb2, db, (double)P.ny, lo, hi);
@@ -629,10 +665,30 @@ __device__ __forceinline__ TileLane tile_lane(const MarchArgs &P, uint32_t logic
     const uint32_t q = logical % tiles;
     const int tile_v = (int)(q / P.tiles_u), tile_u = (int)(q % P.tiles_u);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    t.u = tile_u * kTileU + (wave & 1) * kWaveU + (lane % kWaveU);
-    t.v = tile_v * kTileV + (wave >> 1) * kWaveV + (lane / kWaveU);
+    const int wu = P.wu_log2, bu = P.bu_log2;
+    const int tile_pu = 1 << (wu + bu), tile_pv = (64 >> wu) * (4 >> bu);
+    t.u = tile_u * tile_pu + ((wave & ((1 << bu) - 1)) << wu) + (lane & ((1 << wu) - 1));
+    t.v = tile_v * tile_pv + (wave >> bu) * (64 >> wu) + (lane >> wu);
     t.active = (t.u < P.nu) && (t.v < P.nv);
     return t;
+}
+
+// e^-tau and 1 - e^-tau in FP32.  e^-tau is one v_exp_f32 of -tau log2(e)
+// (1 ulp, plus the rounding of the product: relative error < 1e-6 up to
+// tau = 20).  1 - e^-tau is 1 - e^-tau for |tau| >= 0.35 (relative error
+// < 2e-7 there) and below it the Taylor series to tau^7 in Horner form
+// (remainder < tau^8 / 8! : 2e-8 relative at 0.35), so thin rays keep their
+// absorbed fraction without cancellation.
+__device__ __forceinline__ void attenuate(float tau, float &ex, float &ab)
+{
+    ex = __builtin_amdgcn_exp2f(-tau * 1.44269504088896341f);
+    float q = fmaf(tau, 1.0f / 5040.0f, -1.0f / 720.0f);
+    q = fmaf(tau, q, 1.0f / 120.0f);
+    q = fmaf(tau, q, -1.0f / 24.0f);
+    q = fmaf(tau, q, 1.0f / 6.0f);
+    q = fmaf(tau, q, -0.5f);
+    q = fmaf(tau, q, 1.0f);
+    ab = fabsf(tau) < 0.35f ? tau * q : 1.0f - ex;
 }
 
 // Writes L and, with the spectrum, I and IAbsorb of one ray.
@@ -650,7 +706,7 @@ __device__ __forceinline__ void finish_ray(const MarchArgs &P, const TileLane &t
     }
     if constexpr (kSpectrum) {
         // Per-ray spectrum: tau_e = sum_m A[e][m] L_m (FP32), e^-tau and
-        // 1 - e^-tau (expm1, no cancellation at small tau) in FP32, and the
+        // 1 - e^-tau (attenuate: no cancellation at small tau) in FP32, and the
         // energy sums of W e^-tau and W (1 - e^-tau) in bin order in FP64 (an
         // FP32 sum of ~100 terms would cost ~1e-6 of I, visible in ln(I0/I)
         // of thin rays; FP64 FMA runs at the FP32 rate).
@@ -660,8 +716,8 @@ __device__ __forceinline__ void finish_ray(const MarchArgs &P, const TileLane &t
             float tau = 0.0f;
 #pragma unroll
             for (int m = 0; m < M; m++) tau = fmaf(P.A[e * M + m], Lm[m], tau);
-            float ex = expf(-tau);
-            float ab = -expm1f(-tau);
+            float ex, ab;
+            attenuate(tau, ex, ab);
             double w = (double)wp[(int64_t)e * P.w_stride_e];
             sI = fma(w, (double)ex, sI);
             sA = fma(w, (double)ab, sA);
@@ -859,12 +915,11 @@ __device__ __forceinline__ RaySetup lane_ray(const MarchArgs &P, const TileLane 
     return R;
 }
 
+// One ray tile (logical block `logical` of P) through the LDS-staged march.
+// The first 256 bytes of `lds_raw` are block scratch, the rest staged cells.
 template <int M, int ST, bool kSpectrum>
-__global__ __launch_bounds__(kBlock) void march_lds_kernel(MarchArgs P)
+__device__ __forceinline__ void march_tile(const MarchArgs &P, uint32_t logical, char *lds_raw)
 {
-    extern __shared__ char lds_raw[];
-    const uint32_t logical = xcd_logical(P);
-    if (logical >= P.n_blocks) return;
     const TileLane t = tile_lane(P, logical);
     const RaySetup R = lane_ray(P, t);
     __amdgpu_buffer_rsrc_t rsrc =
@@ -875,6 +930,77 @@ __global__ __launch_bounds__(kBlock) void march_lds_kernel(MarchArgs P)
     march_chunks<M, ST>(P, R, rsrc, lds_raw + 256, (int *)lds_raw, acc);
     if (t.active) finish_ray<M, kSpectrum>(P, t, acc);
 }
+
+template <int M, int ST, bool kSpectrum>
+__global__ __launch_bounds__(kBlock) void march_lds_kernel(MarchArgs P)
+{
+    extern __shared__ char lds_raw[];
+    const uint32_t logical = xcd_logical(P);
+    if (logical >= P.n_blocks) return;
+    march_tile<M, ST, kSpectrum>(P, logical, lds_raw);
+}
+
+// ------------------------------------------------------------ batches ---
+//
+// Several scans (contexts, each with its own volume, detector, views and
+// spectrum) in one persistent launch.  The work units are ray tiles; unit
+// queue x (one per XCD) holds, for every item, the tiles of the views
+// v = x (mod 8) in item order, so the tiles of one view stay on one XCD
+// (one L2).  Each workgroup drains the queue of its own XCD, then the other
+// seven in turn; it leaves when every queue is empty, so all waves end.
+
+constexpr int kXcds = 8;
+
+struct BatchItem {
+    MarchArgs P;
+    int32_t n_mat;
+};
+
+struct BatchQueues {
+    uint32_t *taken;            // [kXcds] units handed out per queue (zeroed before a launch)
+    const uint32_t *unit_base;  // [kXcds][n_items + 1] first unit of each item in queue x
+    int32_t n_items;
+};
+
+__global__ void batch_reset(uint32_t *taken)
+{
+    if (threadIdx.x < kXcds) taken[threadIdx.x] = 0u;
+}
+
+template <int ST>
+__global__ __launch_bounds__(kBlock) void batch_kernel(const BatchItem *__restrict__ items, BatchQueues Q)
+{
+    extern __shared__ char lds_raw[];
+    uint32_t *slot = (uint32_t *)lds_raw + 60; // beside the block_box scratch (ints 0..31)
+    const uint32_t home = blockIdx.x & (kXcds - 1);
+    const int stride = Q.n_items + 1;
+    for (uint32_t qi = 0; qi < (uint32_t)kXcds; qi++) {
+        const uint32_t x = (home + qi) & (kXcds - 1);
+        const uint32_t *base = Q.unit_base + x * stride;
+        const uint32_t total = base[Q.n_items];
+        for (;;) {
+            __syncthreads(); // the previous unit has finished with the LDS
+            if (threadIdx.x == 0) *slot = atomicAdd(&Q.taken[x], 1u);
+            __syncthreads();
+            const uint32_t u = *slot;
+            if (u >= total) break;
+            int i = 0;
+            while (u >= base[i + 1]) i++;
+            const BatchItem &it = items[i];
+            const uint32_t r = u - base[i];
+            const uint32_t tiles = (uint32_t)(it.P.tiles_u * it.P.tiles_v);
+            const uint32_t logical = ((r / tiles) * kXcds + x) * tiles + r % tiles;
+            switch (it.n_mat) {
+            case 1: march_tile<1, ST, true>(it.P, logical, lds_raw); break;
+            case 2: march_tile<2, ST, true>(it.P, logical, lds_raw); break;
+            case 3: march_tile<3, ST, true>(it.P, logical, lds_raw); break;
+            default: march_tile<4, ST, true>(it.P, logical, lds_raw); break;
+            }
+        }
+    }
+}
+
+using BatchFn = void (*)(const BatchItem *, BatchQueues);
 
 // ------------------------------------------------------------ the adjoint ---
 
@@ -986,14 +1112,13 @@ __device__ __forceinline__ void scatter_chunks(const MarchArgs &P, const RaySetu
 // Adjoint of ctfp_forward for upstream gradients gI, gIA: per ray, the line
 // integrals (read or recomputed), then dl/dL_m = (gIA - gI) sum_e A[e][m]
 // W_e e^-tau_e, then the backprojection of dx * dl/dL_m.
+// dl/dL_m * dx of the lane's ray: the line integrals (read from G.L or
+// recomputed by the march), then (gIA - gI) sum_e A[e][m] W_e e^-tau_e dx.
+// Every thread of the block must call it (the march synchronises).
 template <int M, int ST>
-__global__ __launch_bounds__(kBlock) void adjoint_kernel(MarchArgs P, AdjointArgs G)
+__device__ __forceinline__ void ray_gradient(const MarchArgs &P, const AdjointArgs &G, const TileLane &t,
+                                             const RaySetup &R, char *lds_raw, float (&g)[M])
 {
-    extern __shared__ char lds_raw[];
-    const uint32_t logical = xcd_logical(P);
-    if (logical >= P.n_blocks) return;
-    const TileLane t = tile_lane(P, logical);
-    const RaySetup R = lane_ray(P, t);
     const int64_t npix = (int64_t)P.nu * P.nv;
     const int64_t pix = (int64_t)t.v * P.nu + t.u;
     float Lm[M];
@@ -1011,7 +1136,6 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(MarchArgs P, AdjointArg
 #pragma unroll
         for (int m = 0; m < M; m++) Lm[m] = acc[m] * P.dx_f;
     }
-    float g[M];
 #pragma unroll
     for (int m = 0; m < M; m++) g[m] = 0.0f;
     if (t.active) {
@@ -1019,21 +1143,213 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(MarchArgs P, AdjointArg
         const float c = (G.gIA ? G.gIA[q] : 0.0f) - (G.gI ? G.gI[q] : 0.0f);
         if (c != 0.0f) {
             const float *wp = P.W + (int64_t)t.v * P.w_stride_v + t.u;
-            double s[M];
+            double sm[M];
 #pragma unroll
-            for (int m = 0; m < M; m++) s[m] = 0.0;
+            for (int m = 0; m < M; m++) sm[m] = 0.0;
             for (int e = 0; e < P.n_act; e++) {
                 float tau = 0.0f;
 #pragma unroll
                 for (int m = 0; m < M; m++) tau = fmaf(P.A[e * M + m], Lm[m], tau);
-                const double we = (double)wp[(int64_t)e * P.w_stride_e] * (double)expf(-tau);
+                float ex, ab;
+                attenuate(tau, ex, ab);
+                const double we = (double)wp[(int64_t)e * P.w_stride_e] * (double)ex;
 #pragma unroll
-                for (int m = 0; m < M; m++) s[m] = fma((double)P.A[e * M + m], we, s[m]);
+                for (int m = 0; m < M; m++) sm[m] = fma((double)P.A[e * M + m], we, sm[m]);
             }
 #pragma unroll
-            for (int m = 0; m < M; m++) g[m] = (float)((double)c * s[m] * (double)P.dx_f);
+            for (int m = 0; m < M; m++) g[m] = (float)((double)c * sm[m] * (double)P.dx_f);
         }
     }
+}
+
+// Ray pass of the gather adjoint: writes dl/dL_m * dx per ray into
+// gL [view - view_begin][m][v][u].
+template <int M, int ST>
+__global__ __launch_bounds__(kBlock) void ray_grad_kernel(MarchArgs P, AdjointArgs G, float *gL)
+{
+    extern __shared__ char lds_raw[];
+    const uint32_t logical = xcd_logical(P);
+    if (logical >= P.n_blocks) return;
+    const TileLane t = tile_lane(P, logical);
+    const RaySetup R = lane_ray(P, t);
+    float g[M];
+    ray_gradient<M, ST>(P, G, t, R, lds_raw, g);
+    if (t.active) {
+        const int64_t npix = (int64_t)P.nu * P.nv;
+        const int64_t pix = (int64_t)t.v * P.nu + t.u;
+#pragma unroll
+        for (int m = 0; m < M; m++) gL[((int64_t)t.vw_local * M + m) * npix + pix] = g[m];
+    }
+}
+
+// Voxel pass of the gather adjoint.  Each thread owns one voxel of an
+// 8 x 8 x 4 brick and sums, over the views, every sample of every ray whose
+// trilinear support (the open cube of side 2 around the voxel) holds it:
+//   grad_m(c) += sum_rays gL_m(ray) sum_k prod_axes max(0, 1 - |p_k - c|).
+// Per view the block stages, for each detector pixel of the brick's shadow,
+// the ray's closest approach to the brick centre (FP64-derived offset e and
+// sample index k*), its per-sample step and the ray's gL, so that the voxel
+// loop runs in FP32 on small numbers (positions good to ~1e-6 voxel).  The
+// voxel writes its own gradient: no atomics, and the result is deterministic.
+struct GatherArgs {
+    MarchArgs P;       // geometry of the scan (views [view_begin, +n_views))
+    const float *gL;   // [n_views][M][nv][nu]
+    float *grad;       // [M][nz][ny][nx], accumulated
+    int32_t bricks_x, bricks_y;
+    int32_t cap;       // staged pixels that fit the LDS budget
+};
+
+constexpr int kBrickX = 8, kBrickY = 8, kBrickZ = 4;
+
+template <int M>
+__global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
+{
+    extern __shared__ char lds_raw[];
+    const MarchArgs &P = A.P;
+    const int cap = A.cap;
+    // staged arrays (structure of arrays, `cap` entries each)
+    float *s_e = (float *)(lds_raw + 256);      // 3 x cap: offset of the closest approach
+    float *s_d = s_e + 3 * cap;                  // 3 x cap: step per sample
+    float *s_id = s_d + 3 * cap;                 // 3 x cap: 1 / step
+    int *s_kf = (int *)(s_id + 3 * cap);         // cap: floor(k*)
+    float *s_kr = (float *)(s_kf + cap);         // cap: k* - floor(k*)
+    float *s_g = s_kr + cap;                     // M x cap: dl/dL_m * dx
+    int *s_rect = (int *)lds_raw;                // scratch: the brick's pixel rectangle
+    const int bx = (int)(blockIdx.x % (uint32_t)A.bricks_x);
+    const int by = (int)((blockIdx.x / (uint32_t)A.bricks_x) % (uint32_t)A.bricks_y);
+    const int bz = (int)(blockIdx.x / ((uint32_t)A.bricks_x * (uint32_t)A.bricks_y));
+    const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
+    const int x = bx * kBrickX + lx, y = by * kBrickY + ly, z = bz * kBrickZ + lz;
+    const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
+    const bool inside = x < nx && y < ny && z < nz;
+    // brick centre (index space) and the voxel's offset from it
+    const double ca = bx * kBrickX + 0.5 * (kBrickX - 1), cb = by * kBrickY + 0.5 * (kBrickY - 1);
+    const double cc = bz * kBrickZ + 0.5 * (kBrickZ - 1);
+    const float ox = (float)(lx - 0.5 * (kBrickX - 1)), oy = (float)(ly - 0.5 * (kBrickY - 1));
+    const float oz = (float)(lz - 0.5 * (kBrickZ - 1));
+    const int64_t npix = (int64_t)P.nu * P.nv;
+    float acc[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) acc[m] = 0.0f;
+    for (int vw = 0; vw < P.n_views; vw++) {
+        const double2 cs = P.view_cs[P.view_begin + vw];
+        // shadow of the brick: centre projection and the Jacobian per voxel
+        double u0, v0, ua, va, ub, vb, uc, vc;
+        project_point(P, cs, ca, cb, cc, u0, v0);
+        project_point(P, cs, ca + 1.0, cb, cc, ua, va);
+        project_point(P, cs, ca, cb + 1.0, cc, ub, vb);
+        project_point(P, cs, ca, cb, cc + 1.0, uc, vc);
+        const float Jua = (float)(ua - u0), Jub = (float)(ub - u0), Juc = (float)(uc - u0);
+        const float Jva = (float)(va - v0), Jvb = (float)(vb - v0), Jvc = (float)(vc - v0);
+        // a voxel's support cube spans +-1 voxel: +-h pixels around its centre
+        const float hu = fabsf(Jua) + fabsf(Jub) + fabsf(Juc) + 0.25f;
+        const float hv = fabsf(Jva) + fabsf(Jvb) + fabsf(Jvc) + 0.25f;
+        const float eu = fabsf(Jua) * (0.5f * kBrickX + 0.5f) + fabsf(Jub) * (0.5f * kBrickY + 0.5f) +
+                         fabsf(Juc) * (0.5f * kBrickZ + 0.5f) + 0.5f;
+        const float ev = fabsf(Jva) * (0.5f * kBrickX + 0.5f) + fabsf(Jvb) * (0.5f * kBrickY + 0.5f) +
+                         fabsf(Jvc) * (0.5f * kBrickZ + 0.5f) + 0.5f;
+        const int ru0 = max(0, (int)ceil(u0 - eu)), ru1 = min(P.nu - 1, (int)floor(u0 + eu));
+        const int rv0 = max(0, (int)ceil(v0 - ev)), rv1 = min(P.nv - 1, (int)floor(v0 + ev));
+        if (ru0 > ru1 || rv0 > rv1) continue; // block-uniform
+        // the voxel's own pixel window (inside the brick's)
+        const float cu = (float)u0 + Jua * ox + Jub * oy + Juc * oz;
+        const float cv = (float)v0 + Jva * ox + Jvb * oy + Jvc * oz;
+        const int pu0 = max(ru0, (int)ceilf(cu - hu)), pu1 = min(ru1, (int)floorf(cu + hu));
+        const int pv0 = max(rv0, (int)ceilf(cv - hv)), pv1 = min(rv1, (int)floorf(cv + hv));
+        // the brick's shadow in pieces of at most `cap` pixels: column
+        // segments no wider than cap, each in bands of whole rows
+        for (int c0 = ru0; c0 <= ru1; c0 += cap) {
+            const int c1 = min(ru1, c0 + cap - 1);
+            const int rw = c1 - c0 + 1;
+            const int band = max(1, cap / rw);
+            for (int b0 = rv0; b0 <= rv1; b0 += band) {
+                const int b1 = min(rv1, b0 + band - 1);
+                const int count = (b1 - b0 + 1) * rw;
+                __syncthreads(); // the previous band's entries are consumed
+                for (int i = threadIdx.x; i < count; i += kBlock) {
+                    const int pv = b0 + i / rw, pu = c0 + i % rw;
+                    const RayLine Ln = ray_line(P, pu, pv, cs);
+                    // position at sample k: q0 + k * d, relative to the brick centre
+                    const double dxa = Ln.da * P.dx, dxb = Ln.db * P.dx, dxc = Ln.dc * P.dx;
+                    const double qa = Ln.sa + Ln.da * P.t0 - ca, qb = Ln.sb + Ln.db * P.t0 - cb;
+                    const double qc = Ln.sc + Ln.dc * P.t0 - cc;
+                    const double ks = -(qa * dxa + qb * dxb + qc * dxc) / (dxa * dxa + dxb * dxb + dxc * dxc);
+                    const double kf = floor(ks);
+                    s_e[i] = (float)(qa + ks * dxa);
+                    s_e[cap + i] = (float)(qb + ks * dxb);
+                    s_e[2 * cap + i] = (float)(qc + ks * dxc);
+                    const float da = (float)dxa, db = (float)dxb, dc = (float)dxc;
+                    s_d[i] = da;
+                    s_d[cap + i] = db;
+                    s_d[2 * cap + i] = dc;
+                    // a zero step leaves that axis unconstrained (or empty) through +-huge bounds
+                    s_id[i] = 1.0f / (fabsf(da) > 1e-30f ? da : 1e-30f);
+                    s_id[cap + i] = 1.0f / (fabsf(db) > 1e-30f ? db : 1e-30f);
+                    s_id[2 * cap + i] = 1.0f / (fabsf(dc) > 1e-30f ? dc : 1e-30f);
+                    s_kf[i] = (int)kf;
+                    s_kr[i] = (float)(ks - kf);
+                    const int64_t pix = (int64_t)pv * P.nu + pu;
+    #pragma unroll
+                    for (int m = 0; m < M; m++) s_g[m * cap + i] = A.gL[((int64_t)vw * M + m) * npix + pix];
+                }
+                __syncthreads();
+                if (!inside) continue;
+                const int q0 = max(pv0, b0), q1 = min(pv1, b1);
+                const int w0 = max(pu0, c0), w1 = min(pu1, c1);
+                for (int pv = q0; pv <= q1; pv++) {
+                    for (int pu = w0; pu <= w1; pu++) {
+                        const int i = (pv - b0) * rw + (pu - c0);
+                        const float fa = s_e[i] - ox, fb = s_e[cap + i] - oy, fc = s_e[2 * cap + i] - oz;
+                        const float da = s_d[i], db = s_d[cap + i], dc = s_d[2 * cap + i];
+                        const float ia = s_id[i], ib = s_id[cap + i], ic = s_id[2 * cap + i];
+                        // samples with |f + s d| < 1 on every axis, s = k - k*
+                        const float a0 = (-1.0f - fa) * ia, a1 = (1.0f - fa) * ia;
+                        const float b0_ = (-1.0f - fb) * ib, b1_ = (1.0f - fb) * ib;
+                        const float c0 = (-1.0f - fc) * ic, c1 = (1.0f - fc) * ic;
+                        const float slo = fmaxf(fmaxf(fminf(a0, a1), fminf(b0_, b1_)), fminf(c0, c1));
+                        const float shi = fminf(fminf(fmaxf(a0, a1), fmaxf(b0_, b1_)), fmaxf(c0, c1));
+                        if (!(shi > slo)) continue;
+                        const int kf = s_kf[i];
+                        const float kr = s_kr[i];
+                        // j = k - kf over the open interval, and 0 <= k < K
+                        const int j0 = max((int)ceilf(kr + slo), -kf);
+                        const int j1 = min((int)floorf(kr + shi), P.K - 1 - kf);
+                        if (j0 > j1) continue;
+                        float g[M];
+    #pragma unroll
+                        for (int m = 0; m < M; m++) g[m] = s_g[m * cap + i];
+                        float wsum = 0.0f;
+                        for (int j = j0; j <= j1; j++) {
+                            const float sj = (float)j - kr;
+                            const float ra = fmaf(sj, da, fa), rb = fmaf(sj, db, fb), rc = fmaf(sj, dc, fc);
+                            wsum += fmaxf(0.0f, 1.0f - fabsf(ra)) * fmaxf(0.0f, 1.0f - fabsf(rb)) *
+                                    fmaxf(0.0f, 1.0f - fabsf(rc));
+                        }
+    #pragma unroll
+                        for (int m = 0; m < M; m++) acc[m] = fmaf(wsum, g[m], acc[m]);
+                    }
+                }
+            }
+        }
+    }
+    if (inside) {
+        const int64_t nvox = (int64_t)nx * ny * nz;
+        const int64_t q = ((int64_t)z * ny + y) * nx + x;
+#pragma unroll
+        for (int m = 0; m < M; m++) A.grad[m * nvox + q] += acc[m];
+    }
+}
+
+template <int M, int ST>
+__global__ __launch_bounds__(kBlock) void adjoint_kernel(MarchArgs P, AdjointArgs G)
+{
+    extern __shared__ char lds_raw[];
+    const uint32_t logical = xcd_logical(P);
+    if (logical >= P.n_blocks) return;
+    const TileLane t = tile_lane(P, logical);
+    const RaySetup R = lane_ray(P, t);
+    float g[M];
+    ray_gradient<M, ST>(P, G, t, R, lds_raw, g);
     scatter_chunks<M>(P, R, g, G.grad, lds_raw + 256, (int *)lds_raw);
 }
 
@@ -1056,6 +1372,40 @@ template <int ST> MarchFn pick_march_st(int n_mat, bool spectrum, bool lds)
 }
 
 using AdjointFn = void (*)(MarchArgs, AdjointArgs);
+using RayGradFn = void (*)(MarchArgs, AdjointArgs, float *);
+using GatherFn = void (*)(GatherArgs);
+
+template <int ST> RayGradFn pick_ray_grad_st(int n_mat)
+{
+    switch (n_mat) {
+    case 1: return ray_grad_kernel<1, ST>;
+    case 2: return ray_grad_kernel<2, ST>;
+    case 3: return ray_grad_kernel<3, ST>;
+    case 4: return ray_grad_kernel<4, ST>;
+    default: return nullptr;
+    }
+}
+
+RayGradFn pick_ray_grad(int n_mat, int dtype)
+{
+    switch (dtype) {
+    case CTFP_F32: return pick_ray_grad_st<kStF32>(n_mat);
+    case CTFP_F16: return pick_ray_grad_st<kStF16>(n_mat);
+    case CTFP_BF16: return pick_ray_grad_st<kStBF16>(n_mat);
+    default: return nullptr;
+    }
+}
+
+GatherFn pick_gather(int n_mat)
+{
+    switch (n_mat) {
+    case 1: return gather_kernel<1>;
+    case 2: return gather_kernel<2>;
+    case 3: return gather_kernel<3>;
+    case 4: return gather_kernel<4>;
+    default: return nullptr;
+    }
+}
 
 template <int ST> AdjointFn pick_adjoint_st(int n_mat)
 {
@@ -1124,6 +1474,13 @@ int alloc_volume(ctfp_context *ctx, int n_mat)
     return CTFP_OK;
 }
 
+int ilog2(int v)
+{
+    int r = 0;
+    while ((1 << (r + 1)) <= v) r++;
+    return r;
+}
+
 int fill_march_args(ctfp_context *ctx, int64_t view_begin, int64_t view_count, MarchArgs &P)
 {
     const ctfp_scan_params &s = ctx->scan;
@@ -1163,8 +1520,12 @@ int fill_march_args(ctfp_context *ctx, int64_t view_begin, int64_t view_count, M
     P.view_cs = ctx->view_cs;
     P.view_begin = view_begin;
     P.n_views = (int32_t)view_count;
-    P.tiles_u = (int32_t)((s.d_num_u + kTileU - 1) / kTileU);
-    P.tiles_v = (int32_t)((s.d_num_v + kTileV - 1) / kTileV);
+    P.wu_log2 = ilog2(ctx->tune.wave_u);
+    P.bu_log2 = ilog2(ctx->tune.block_u);
+    const int64_t tile_pu = (int64_t)ctx->tune.wave_u * ctx->tune.block_u;
+    const int64_t tile_pv = (int64_t)(64 / ctx->tune.wave_u) * (4 / ctx->tune.block_u);
+    P.tiles_u = (int32_t)((s.d_num_u + tile_pu - 1) / tile_pu);
+    P.tiles_v = (int32_t)((s.d_num_v + tile_pv - 1) / tile_pv);
     uint64_t nb = (uint64_t)view_count * P.tiles_u * P.tiles_v;
     if (nb * 1ull >= (1ull << 31))
         return set_error(CTFP_ERR_INVALID, "too many tiles in one launch (%llu)", (unsigned long long)nb);
@@ -1224,6 +1585,7 @@ int ctfp_destroy(ctfp_context *ctx)
     DeviceGuard g(ctx->device);
     (void)hipDeviceSynchronize();
     dfree(ctx->vol);
+    dfree(ctx->adj_ws);
     dfree(ctx->A);
     dfree(ctx->W);
     dfree(ctx->I0);
@@ -1262,6 +1624,7 @@ static int write_layout(ctfp_context *ctx, const VolPtrs &vp, int n_mat, int dty
     HIP_TRY(hipGetLastError());
     if (!async) HIP_TRY(hipDeviceSynchronize());
     ctx->dtype = dtype;
+    ctx->generation++;
     return CTFP_OK;
 }
 
@@ -1340,6 +1703,7 @@ int ctfp_device_bytes(const ctfp_context *ctx, int64_t *bytes)
     if (ctx->I0) b += npix * (int64_t)sizeof(float);
     if (ctx->view_cs) b += ctx->scan.proj_num * (int64_t)sizeof(double2);
     if (ctx->dI) b += 2 * ctx->staged_views * npix * (int64_t)sizeof(float);
+    if (ctx->adj_ws) b += (int64_t)ctx->adj_ws_bytes;
     *bytes = b;
     return CTFP_OK;
 }
@@ -1423,6 +1787,7 @@ int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
     }
     const bool per_pixel = sp->det != nullptr || sp->scint_mu != nullptr;
     const int64_t rows = per_pixel ? nv : 1;
+    ctx->generation++; // the tables are replaced from here on
     dfree(ctx->A);
     dfree(ctx->W);
     if (!ctx->I0) HIP_TRY(hipMalloc(&ctx->I0, (size_t)npix * sizeof(float)));
@@ -1484,11 +1849,11 @@ int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
     return CTFP_OK;
 }
 
-static int launch_march(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *I,
-                        float *IA, float *L, void *stream, bool spectrum)
+// Launch arguments of a forward over views [view_begin, view_begin + count).
+static int forward_args(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *I,
+                        float *IA, float *L, bool spectrum, MarchArgs &P)
 {
     if (check_ctx(ctx)) return CTFP_ERR_INVALID;
-    MarchArgs P;
     int rc = fill_march_args(ctx, view_begin, view_count, P);
     if (rc) return rc;
     if (spectrum) {
@@ -1508,16 +1873,22 @@ static int launch_march(ctfp_context *ctx, int64_t view_begin, int64_t view_coun
     P.I = I;
     P.IA = IA;
     P.L = L;
+    const uint32_t lb = ctx->n_mat == 3 ? 16u : 4u * (uint32_t)ctx->n_mat;
+    P.chunk = ctx->tune.chunk;
+    P.lds_cells = (int32_t)((ctx->tune.lds_bytes - 256) / lb);
+    return CTFP_OK;
+}
+
+static int launch_march(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *I,
+                        float *IA, float *L, void *stream, bool spectrum)
+{
+    MarchArgs P;
+    int rc = forward_args(ctx, view_begin, view_count, I, IA, L, spectrum, P);
+    if (rc) return rc;
     const bool lds = ctx->tune.march == CTFP_MARCH_LDS;
     MarchFn fn = pick_march(ctx->n_mat, ctx->dtype, spectrum, lds);
     if (!fn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->n_mat);
-    size_t shmem = 0;
-    if (lds) {
-        const uint32_t lb = ctx->n_mat == 3 ? 16u : 4u * (uint32_t)ctx->n_mat;
-        P.chunk = ctx->tune.chunk;
-        P.lds_cells = (int32_t)((ctx->tune.lds_bytes - 256) / lb);
-        shmem = (size_t)ctx->tune.lds_bytes;
-    }
+    const size_t shmem = lds ? (size_t)ctx->tune.lds_bytes : 0;
     DeviceGuard g(ctx->device);
     dim3 grid(P.per_xcd * 8), block(kBlock);
     hipLaunchKernelGGL(fn, grid, block, shmem, (hipStream_t)stream, P);
@@ -1536,6 +1907,12 @@ int ctfp_set_tuning(ctfp_context *ctx, const ctfp_tuning *t)
                          t->chunk, kAnchor, kAnchor);
     if (t->lds_bytes < 1024 || t->lds_bytes > 64 * 1024)
         return set_error(CTFP_ERR_INVALID, "tuning: lds_bytes %d outside [1 KiB, 64 KiB]", t->lds_bytes);
+    if (t->wave_u != 4 && t->wave_u != 8 && t->wave_u != 16 && t->wave_u != 32)
+        return set_error(CTFP_ERR_INVALID, "tuning: wave_u %d is not 4, 8, 16 or 32", t->wave_u);
+    if (t->block_u != 1 && t->block_u != 2 && t->block_u != 4)
+        return set_error(CTFP_ERR_INVALID, "tuning: block_u %d is not 1, 2 or 4", t->block_u);
+    if (t->adjoint != CTFP_ADJOINT_GATHER && t->adjoint != CTFP_ADJOINT_SCATTER)
+        return set_error(CTFP_ERR_INVALID, "tuning: adjoint mode %d", t->adjoint);
     ctx->tune = *t;
     return CTFP_OK;
 }
@@ -1577,12 +1954,62 @@ int ctfp_backward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, con
     const size_t shmem = global ? 256 : (size_t)ctx->tune.lds_bytes;
     P.chunk = ctx->tune.chunk;
     P.lds_cells = global ? 0 : (int32_t)((ctx->tune.lds_bytes - 256) / lb);
-    AdjointArgs G{gI, gIAbsorb, L, grad};
-    AdjointFn fn = pick_adjoint(ctx->n_mat, ctx->dtype);
-    if (!fn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->n_mat);
     DeviceGuard g(ctx->device);
-    hipLaunchKernelGGL(fn, dim3(P.per_xcd * 8), dim3(kBlock), shmem, (hipStream_t)stream, P, G);
-    HIP_TRY(hipGetLastError());
+    if (ctx->tune.adjoint == CTFP_ADJOINT_SCATTER) {
+        AdjointArgs G{gI, gIAbsorb, L, grad};
+        AdjointFn fn = pick_adjoint(ctx->n_mat, ctx->dtype);
+        if (!fn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->n_mat);
+        hipLaunchKernelGGL(fn, dim3(P.per_xcd * 8), dim3(kBlock), shmem, (hipStream_t)stream, P, G);
+        HIP_TRY(hipGetLastError());
+        return CTFP_OK;
+    }
+    // gather: per group of views, the ray pass writes dl/dL * dx into the
+    // workspace and the voxel pass adds its backprojection to grad
+    RayGradFn rfn = pick_ray_grad(ctx->n_mat, ctx->dtype);
+    GatherFn gfn = pick_gather(ctx->n_mat);
+    if (!rfn || !gfn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->n_mat);
+    const ctfp_scan_params &s = ctx->scan;
+    const int64_t npix = s.d_num_u * s.d_num_v;
+    const int64_t per_view = (int64_t)ctx->n_mat * npix * (int64_t)sizeof(float);
+    const int64_t group = std::max<int64_t>(1, std::min<int64_t>(view_count, (512ll << 20) / per_view));
+    const size_t ws = (size_t)(group * per_view);
+    if (ctx->adj_ws_bytes < ws) {
+        dfree(ctx->adj_ws);
+        ctx->adj_ws_bytes = 0;
+        HIP_TRY(hipMalloc(&ctx->adj_ws, ws));
+        ctx->adj_ws_bytes = ws;
+    }
+    const int gather_lds = 40 * 1024;
+    GatherArgs A;
+    A.grad = grad;
+    A.gL = ctx->adj_ws;
+    A.bricks_x = (int32_t)((s.p_num_x + kBrickX - 1) / kBrickX);
+    A.bricks_y = (int32_t)((s.p_num_y + kBrickY - 1) / kBrickY);
+    const int64_t bricks_z = (s.p_num_z + kBrickZ - 1) / kBrickZ;
+    const int64_t bricks = (int64_t)A.bricks_x * A.bricks_y * bricks_z;
+    if (bricks >= (1ll << 31)) return set_error(CTFP_ERR_UNSUPPORTED, "volume too large for the gather");
+    A.cap = (gather_lds - 256) / ((11 + ctx->n_mat) * 4);
+    for (int64_t v0 = 0; v0 < view_count; v0 += group) {
+        const int64_t n = std::min(group, view_count - v0);
+        MarchArgs Q = P;
+        rc = fill_march_args(ctx, view_begin + v0, n, Q);
+        if (rc) return rc;
+        Q.n_act = P.n_act;
+        Q.A = P.A;
+        Q.W = P.W;
+        Q.w_stride_e = P.w_stride_e;
+        Q.w_stride_v = P.w_stride_v;
+        Q.chunk = P.chunk;
+        Q.lds_cells = P.lds_cells;
+        AdjointArgs G{gI ? gI + v0 * npix : nullptr, gIAbsorb ? gIAbsorb + v0 * npix : nullptr,
+                      L ? L + v0 * ctx->n_mat * npix : nullptr, grad};
+        hipLaunchKernelGGL(rfn, dim3(Q.per_xcd * 8), dim3(kBlock), shmem, (hipStream_t)stream, Q, G,
+                           ctx->adj_ws);
+        HIP_TRY(hipGetLastError());
+        A.P = Q;
+        hipLaunchKernelGGL(gfn, dim3((unsigned)bricks), dim3(kBlock), (size_t)gather_lds, (hipStream_t)stream, A);
+        HIP_TRY(hipGetLastError());
+    }
     return CTFP_OK;
 }
 
@@ -1652,6 +2079,248 @@ int ctfp_postprocess(int64_t n_views, int64_t npix, const float *I, const float 
 int ctfp_synchronize(void *stream)
 {
     HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return CTFP_OK;
+}
+
+// ---- autotuner ----------------------------------------------------------
+
+int ctfp_autotune(ctfp_context *ctx, int max_lds_bytes, ctfp_tuning *best)
+{
+    if (check_ctx(ctx)) return CTFP_ERR_INVALID;
+    if (!ctx->vol || !ctx->W) return set_error(CTFP_ERR_STATE, "autotune needs volumes and spectral tables");
+    const ctfp_scan_params &s = ctx->scan;
+    const int64_t npix = s.d_num_u * s.d_num_v;
+    // time enough views to fill the GPU several times (>= 8192 tiles of
+    // 256 rays), in two groups a quarter turn apart
+    const int64_t tiles_per_view = (npix + 255) / 256;
+    const int64_t want = std::max<int64_t>(1, (8192 + tiles_per_view - 1) / tiles_per_view);
+    const int64_t per_group = std::min<int64_t>(want, std::max<int64_t>(1, s.proj_num / 2));
+    const int64_t starts[2] = {0, std::min<int64_t>(s.proj_num / 4, s.proj_num - per_group)};
+    DeviceGuard g(ctx->device);
+    float *I = nullptr, *IA = nullptr;
+    HIP_TRY(hipMalloc(&I, (size_t)per_group * npix * sizeof(float)));
+    hipError_t e = hipMalloc(&IA, (size_t)per_group * npix * sizeof(float));
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ev1);
+    const ctfp_tuning saved = ctx->tune;
+    auto cleanup = [&]() {
+        ctx->tune = saved;
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        dfree(I);
+        dfree(IA);
+    };
+    if (e != hipSuccess) {
+        cleanup();
+        return set_error(CTFP_ERR_DEVICE, "autotune set-up: %s", hipGetErrorString(e));
+    }
+    // time one candidate (ms over both groups; +inf if it cannot launch)
+    auto time_it = [&](const ctfp_tuning &t, float &ms) -> int {
+        ctx->tune = t;
+        for (int rep = 0; rep < 2; rep++) { // rep 0 warms caches and code
+            hipError_t r = hipEventRecord(ev0, nullptr);
+            for (int gi = 0; gi < 2 && r == hipSuccess; gi++) {
+                int rc = launch_march(ctx, starts[gi], per_group, I, IA, nullptr, nullptr, true);
+                if (rc) return rc;
+            }
+            if (r == hipSuccess) r = hipEventRecord(ev1, nullptr);
+            if (r == hipSuccess) r = hipEventSynchronize(ev1);
+            if (r == hipSuccess) r = hipEventElapsedTime(&ms, ev0, ev1);
+            if (r != hipSuccess) return set_error(CTFP_ERR_DEVICE, "autotune: %s", hipGetErrorString(r));
+        }
+        return CTFP_OK;
+    };
+    ctfp_tuning cur = saved;
+    cur.march = CTFP_MARCH_LDS;
+    if (max_lds_bytes > 0 && cur.lds_bytes > max_lds_bytes) cur.lds_bytes = std::max(1024, max_lds_bytes);
+    float best_ms = 0.0f;
+    int rc = time_it(cur, best_ms);
+    // ray-tile shapes (wave_u, block_u), then staging step and LDS budget
+    const int shapes[][2] = {{8, 2}, {16, 1}, {4, 4}, {16, 2}, {8, 1}, {32, 1}, {8, 4}};
+    ctfp_tuning pick = cur;
+    for (const auto &sh : shapes) {
+        if (rc) break;
+        ctfp_tuning t = cur;
+        t.wave_u = sh[0];
+        t.block_u = sh[1];
+        float ms = 0.0f;
+        rc = time_it(t, ms);
+        if (!rc && ms < best_ms) { best_ms = ms; pick = t; }
+    }
+    const int chunks[] = {16, 32};
+    const int budgets[] = {24 * 1024, 32 * 1024, 40 * 1024, 48 * 1024, 64 * 1024};
+    const ctfp_tuning shape = pick;
+    for (int c : chunks) {
+        for (int b : budgets) {
+            if (rc) break;
+            if (max_lds_bytes > 0 && b > max_lds_bytes) continue;
+            ctfp_tuning t = shape;
+            t.chunk = c;
+            t.lds_bytes = b;
+            float ms = 0.0f;
+            rc = time_it(t, ms);
+            if (!rc && ms < best_ms) { best_ms = ms; pick = t; }
+        }
+    }
+    cleanup();
+    if (rc) return rc;
+    ctx->tune = pick;
+    if (best) *best = pick;
+    return CTFP_OK;
+}
+
+// ---- batches -------------------------------------------------------------
+
+} // extern "C"
+
+struct ctfp_batch {
+    int device = 0;
+    int dtype = CTFP_F32;
+    int n_items = 0;
+    std::vector<ctfp_batch_item> items;
+    std::vector<uint64_t> gens; // generation of each context at creation (checked per run)
+    BatchItem *d_items = nullptr;
+    uint32_t *d_taken = nullptr;
+    uint32_t *d_base = nullptr;
+    size_t shmem = 0;
+    unsigned grid = 0;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int64_t samples = 0;
+};
+
+namespace {
+void batch_free(ctfp_batch *b)
+{
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    dfree(b->d_items);
+    dfree(b->d_taken);
+    dfree(b->d_base);
+    delete b;
+}
+
+BatchFn pick_batch(int dtype)
+{
+    switch (dtype) {
+    case CTFP_F32: return batch_kernel<kStF32>;
+    case CTFP_F16: return batch_kernel<kStF16>;
+    case CTFP_BF16: return batch_kernel<kStBF16>;
+    default: return nullptr;
+    }
+}
+} // namespace
+
+extern "C" {
+
+int ctfp_batch_create(const ctfp_batch_item *items, int n_items, ctfp_batch **out)
+{
+    if (!out) return set_error(CTFP_ERR_INVALID, "null output");
+    *out = nullptr;
+    if (!items || n_items < 1 || n_items > 4096)
+        return set_error(CTFP_ERR_INVALID, "batch: %d items (1..4096)", n_items);
+    const ctfp_context *c0 = items[0].ctx;
+    if (check_ctx(c0)) return CTFP_ERR_INVALID;
+    std::vector<BatchItem> h(n_items);
+    std::vector<uint32_t> base((size_t)kXcds * (n_items + 1), 0u);
+    size_t shmem = 256;
+    int64_t samples = 0;
+    for (int i = 0; i < n_items; i++) {
+        ctfp_context *c = items[i].ctx;
+        if (check_ctx(c)) return CTFP_ERR_INVALID;
+        if (c->device != c0->device || c->dtype != c0->dtype)
+            return set_error(CTFP_ERR_INVALID, "batch: item %d is on another device or storage type", i);
+        if (c->tune.march != CTFP_MARCH_LDS)
+            return set_error(CTFP_ERR_INVALID, "batch: item %d is tuned for the global march", i);
+        if (c->spec_n_mat != c->n_mat || !c->W)
+            return set_error(CTFP_ERR_STATE, "batch: item %d has no spectral tables for its volumes", i);
+        int rc = forward_args(c, items[i].view_begin, items[i].view_count, items[i].I, items[i].IAbsorb,
+                              nullptr, true, h[i].P);
+        if (rc) return rc;
+        h[i].n_mat = c->n_mat;
+        shmem = std::max(shmem, (size_t)c->tune.lds_bytes);
+        samples += items[i].view_count * c->scan.d_num_u * c->scan.d_num_v * c->sys.int_num;
+        const uint64_t tiles = (uint64_t)h[i].P.tiles_u * h[i].P.tiles_v;
+        for (int x = 0; x < kXcds; x++) {
+            const int64_t nv = items[i].view_count > x ? (items[i].view_count - x + kXcds - 1) / kXcds : 0;
+            const uint64_t next = base[(size_t)x * (n_items + 1) + i] + (uint64_t)nv * tiles;
+            if (next >= (1ull << 31)) return set_error(CTFP_ERR_INVALID, "batch: too many tiles");
+            base[(size_t)x * (n_items + 1) + i + 1] = (uint32_t)next;
+        }
+    }
+    DeviceGuard g(c0->device);
+    BatchFn fn = pick_batch(c0->dtype);
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, shmem));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, c0->device));
+    if (per_cu < 1) return set_error(CTFP_ERR_UNSUPPORTED, "batch: kernel does not fit a CU");
+    ctfp_batch *b = new ctfp_batch();
+    b->device = c0->device;
+    b->dtype = c0->dtype;
+    b->n_items = n_items;
+    b->items.assign(items, items + n_items);
+    for (int i = 0; i < n_items; i++) b->gens.push_back(items[i].ctx->generation);
+    b->shmem = shmem;
+    b->grid = (unsigned)(per_cu * prop.multiProcessorCount);
+    b->grid = std::max(8u, b->grid / 8 * 8);
+    b->samples = samples;
+    hipError_t e = hipMalloc(&b->d_items, h.size() * sizeof(BatchItem));
+    if (e == hipSuccess) e = hipMemcpy(b->d_items, h.data(), h.size() * sizeof(BatchItem), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&b->d_taken, kXcds * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->d_base, base.size() * sizeof(uint32_t));
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_base, base.data(), base.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    // the launch sequence (zero the queue counters, run the persistent
+    // kernel) is captured once on a private stream and replayed as a graph
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        batch_reset<<<1, 64, 0, b->cap_stream>>>(b->d_taken);
+        BatchQueues Q{b->d_taken, b->d_base, n_items};
+        hipLaunchKernelGGL(fn, dim3(b->grid), dim3(kBlock), shmem, b->cap_stream, b->d_items, Q);
+        hipError_t launch = hipGetLastError();
+        e = hipStreamEndCapture(b->cap_stream, &b->graph);
+        if (e == hipSuccess) e = launch;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        batch_free(b);
+        return set_error(e == hipErrorOutOfMemory ? CTFP_ERR_NOMEM : CTFP_ERR_DEVICE, "batch: %s",
+                         hipGetErrorString(e));
+    }
+    *out = b;
+    return CTFP_OK;
+}
+
+int ctfp_batch_run(ctfp_batch *b, void *stream)
+{
+    if (!b) return set_error(CTFP_ERR_INVALID, "null batch");
+    for (int i = 0; i < b->n_items; i++)
+        if (b->items[i].ctx->generation != b->gens[i])
+            return set_error(CTFP_ERR_STATE,
+                             "batch: the volumes or spectral tables of item %d changed; rebuild the batch", i);
+    DeviceGuard g(b->device);
+    HIP_TRY(hipGraphLaunch(b->exec, (hipStream_t)stream));
+    return CTFP_OK;
+}
+
+int ctfp_batch_samples(const ctfp_batch *b, int64_t *samples)
+{
+    if (!b || !samples) return set_error(CTFP_ERR_INVALID, "null argument");
+    *samples = b->samples;
+    return CTFP_OK;
+}
+
+int ctfp_batch_destroy(ctfp_batch *b)
+{
+    if (!b) return CTFP_OK;
+    DeviceGuard g(b->device);
+    batch_free(b);
     return CTFP_OK;
 }
 

@@ -129,18 +129,30 @@ class Projector:
         self.close()
 
     def set_tuning(self, march: str | None = None, chunk: int | None = None,
-                   lds_bytes: int | None = None):
+                   lds_bytes: int | None = None, wave_u: int | None = None,
+                   block_u: int | None = None, adjoint: str | None = None):
         """Launch options of the march ("global" or "lds", samples per staged
-        chunk, LDS bytes per workgroup); results are identical for all."""
+        chunk, LDS bytes per workgroup, ray-tile shape; projections are
+        identical for all) and of the adjoint ("gather" or "scatter")."""
         t = L.Tuning()
         L.check(L.lib().ctfp_get_tuning(self._ctx, C.byref(t)))
         if march is not None:
             t.march = {"global": L.CTFP_MARCH_GLOBAL, "lds": L.CTFP_MARCH_LDS}[march]
-        if chunk is not None:
-            t.chunk = chunk
-        if lds_bytes is not None:
-            t.lds_bytes = lds_bytes
+        if adjoint is not None:
+            t.adjoint = {"gather": L.CTFP_ADJOINT_GATHER, "scatter": L.CTFP_ADJOINT_SCATTER}[adjoint]
+        for name, v in (("chunk", chunk), ("lds_bytes", lds_bytes), ("wave_u", wave_u),
+                        ("block_u", block_u)):
+            if v is not None:
+                setattr(t, name, v)
         L.check(L.lib().ctfp_set_tuning(self._ctx, C.byref(t)))
+
+    def autotune(self, max_lds_bytes: int = 0) -> dict:
+        """Time launch options on a sample of this scan's views, keep the
+        fastest (results do not depend on it) and return it.  max_lds_bytes
+        caps the LDS budget (for scans that share a batched launch)."""
+        t = L.Tuning()
+        L.check(L.lib().ctfp_autotune(self._ctx, int(max_lds_bytes), C.byref(t)))
+        return self.get_tuning()
 
     @property
     def handle(self) -> int:
@@ -160,7 +172,8 @@ class Projector:
         t = L.Tuning()
         L.check(L.lib().ctfp_get_tuning(self._ctx, C.byref(t)))
         return {"march": "lds" if t.march == L.CTFP_MARCH_LDS else "global", "chunk": t.chunk,
-                "lds_bytes": t.lds_bytes}
+                "lds_bytes": t.lds_bytes, "wave_u": t.wave_u, "block_u": t.block_u,
+                "adjoint": "scatter" if t.adjoint == L.CTFP_ADJOINT_SCATTER else "gather"}
 
     @property
     def samples_per_view(self) -> int:
@@ -315,6 +328,55 @@ class Projector:
         I0 = np.empty((self.nv, self.nu), dtype=np.float32)
         L.check(L.lib().ctfp_forward_host(self._ctx, I.ctypes.data, IA.ctypes.data, I0.ctypes.data))
         return I, IA, I0
+
+
+class Batch:
+    """Several scans in one persistent launch, captured once into a HIP graph.
+
+    `items` is a list of (projector, view_begin, view_count, I, IAbsorb) with
+    I / IAbsorb contiguous float32 CUDA tensors [view_count][nv][nu].  The
+    projectors (one device, one storage type, LDS march) must keep their
+    volumes and spectral tables while the batch exists."""
+
+    def __init__(self, items):
+        torch = _torch()
+        arr = (L.BatchItem * len(items))()
+        self._keep = []
+        for i, (proj, vb, vc, I, IA) in enumerate(items):
+            shape = (vc, proj.nv, proj.nu)
+            for t in (I, IA):
+                if tuple(t.shape) != shape or t.dtype != torch.float32 or not t.is_contiguous():
+                    raise ValueError(f"batch item {i}: outputs must be contiguous float32 {shape}")
+            arr[i] = L.BatchItem(proj._ctx.value, vb, vc, I.data_ptr(), IA.data_ptr())
+            self._keep.append((proj, I, IA))
+        self._b = C.c_void_p()
+        L.check(L.lib().ctfp_batch_create(arr, len(items), C.byref(self._b)))
+
+    def run(self, stream=None):
+        L.check(L.lib().ctfp_batch_run(self._b, _stream_handle(stream)))
+
+    @property
+    def samples(self) -> int:
+        n = C.c_int64()
+        L.check(L.lib().ctfp_batch_samples(self._b, C.byref(n)))
+        return int(n.value)
+
+    def close(self):
+        if self._b:
+            L.lib().ctfp_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def postprocess(I, IA, I0, stream=None):

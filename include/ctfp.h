@@ -113,7 +113,12 @@ typedef struct ctfp_tuning {
     int32_t march;     /* CTFP_MARCH_GLOBAL or CTFP_MARCH_LDS */
     int32_t chunk;     /* samples per staged chunk (LDS mode): a multiple of 16 in [16, 4096] */
     int32_t lds_bytes; /* LDS per workgroup (LDS mode), 1 KiB..64 KiB */
+    int32_t wave_u;    /* ray tile: pixels along u per wavefront, 4/8/16/32 (64 / wave_u along v) */
+    int32_t block_u;   /* ray tile: wavefronts along u per workgroup, 1/2/4 (4 / block_u along v) */
+    int32_t adjoint;   /* ctfp_backward: CTFP_ADJOINT_GATHER (default) or CTFP_ADJOINT_SCATTER */
 } ctfp_tuning;
+#define CTFP_ADJOINT_GATHER 0  /* ray pass to dl/dL, then a voxel-driven gather: no atomics, deterministic */
+#define CTFP_ADJOINT_SCATTER 1 /* ray-driven scatter through LDS boxes and float atomics */
 
 typedef struct ctfp_context ctfp_context;
 
@@ -192,8 +197,10 @@ int ctfp_line_integrals(ctfp_context *ctx, int64_t view_begin, int64_t view_coun
  * dl/d(density) to `grad` (device FP32 [n_mat][p_num_z][p_num_y][p_num_x];
  * the caller zeroes it).  L (device [view_count][n_mat][d_num_v][d_num_u],
  * from ctfp_forward) may be NULL, then the line integrals are recomputed per
- * ray.  Accumulation uses FP32 atomics: the summation order, hence the last
- * bits, can differ between runs. */
+ * ray.  With CTFP_ADJOINT_GATHER (the default) the result is deterministic;
+ * the call holds a workspace of dl/dL for up to 512 MiB of views, kept by the
+ * context.  With CTFP_ADJOINT_SCATTER accumulation uses FP32 atomics, so the
+ * last bits can differ between runs. */
 int ctfp_backward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, const float *gI,
                   const float *gIAbsorb, const float *L, float *grad, void *stream);
 
@@ -205,6 +212,36 @@ int ctfp_flat_field(ctfp_context *ctx, float *I0, void *stream);
 /* Whole scan into host buffers (convenience for the reference-shaped classes):
  * I, IAbsorb [proj_num][d_num_v][d_num_u], I0 [d_num_v][d_num_u] (NULL skips). */
 int ctfp_forward_host(ctfp_context *ctx, float *I, float *IAbsorb, float *I0);
+
+/* ---- launch tuning and batches (no reference counterpart) -------------- */
+
+/* Times candidate launch options (ray-tile shape, samples per staged chunk,
+ * LDS budget) on two groups of the scan's views a quarter turn apart, keeps
+ * the fastest in the context and returns it in `best` (may be NULL).
+ * max_lds_bytes > 0 caps the LDS budget: a batch runs all its items with the
+ * largest budget among them, so scans meant for one batch share a cap.
+ * Needs volumes and spectral tables; results are bit-identical for every
+ * candidate, so tuning only changes speed.  Synchronous; allocates scratch
+ * output for the timed views. */
+int ctfp_autotune(ctfp_context *ctx, int max_lds_bytes, ctfp_tuning *best);
+
+/* Several scans in one persistent launch.  Each item is a context (its own
+ * volume, detector, views and spectral tables, LDS march mode) and a view
+ * range with device outputs I, IAbsorb [view_count][d_num_v][d_num_u].  All
+ * items must share one device and one storage type.  The launch sequence is
+ * captured once into a HIP graph at creation and replayed by ctfp_batch_run.
+ * The contexts must outlive the batch, and their volumes and spectral tables
+ * must not be replaced while it exists (ctfp_batch_run checks the volumes). */
+typedef struct ctfp_batch ctfp_batch;
+typedef struct ctfp_batch_item {
+    ctfp_context *ctx;
+    int64_t view_begin, view_count;
+    float *I, *IAbsorb;
+} ctfp_batch_item;
+int ctfp_batch_create(const ctfp_batch_item *items, int n_items, ctfp_batch **out);
+int ctfp_batch_run(ctfp_batch *batch, void *stream);
+int ctfp_batch_samples(const ctfp_batch *batch, int64_t *samples);
+int ctfp_batch_destroy(ctfp_batch *batch);
 
 /* Floor to uint32 and p = log(I0 / floor(I)).  Replaces convertIandIAbsorbDt
  * and computeProj (CBCTForwardProj.cpp:79-90, 376-387).  Device pointers;

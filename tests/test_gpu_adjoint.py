@@ -32,8 +32,9 @@ def det_table(kw):
                      for m in kw["scint_mu"]])
 
 
+@pytest.mark.parametrize("mode", ["gather", "scatter"])
 @pytest.mark.parametrize("with_L", [False, True])
-def test_adjoint_matches_fp64_cpu_adjoint(require_gpu, with_L):
+def test_adjoint_matches_fp64_cpu_adjoint(require_gpu, with_L, mode):
     import torch
     s, vols = setup_case()
     kw = P.spectral_inputs(s, *s.det)
@@ -43,6 +44,7 @@ def test_adjoint_matches_fp64_cpu_adjoint(require_gpu, with_L):
     with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
         proj.set_volumes(vols)
         proj.set_spectral(**kw)
+        proj.set_tuning(adjoint=mode)
         _, _, Lg = proj.forward(want_L=True)
         dev = torch.device("cuda")
         grad = proj.backward(torch.from_numpy(gI).to(dev), torch.from_numpy(gIA).to(dev),
@@ -118,8 +120,45 @@ def test_adjoint_finite_differences(require_gpu):
     assert max(errs) < 1e-3, (max(errs), np.median(errs))
 
 
+def test_gather_adjoint_is_deterministic_and_matches_scatter(require_gpu):
+    """The gather adjoint sums every voxel in a fixed order (bit-identical on
+    repeat); the scatter adjoint reaches the same gradient through atomics.
+    The volume (45 x 37 x 30) does not divide into 8 x 8 x 4 bricks, and the
+    view range does not start at view 0."""
+    import torch
+    w, b, i = S.three_material_phantom(48, seed=6)
+    vols = [np.ascontiguousarray(v[:30, :37, :45]) for v in (w, b, i)]
+    s = S.config2()
+    s.n_vox, s.length_mm, s.det, s.d_size, s.views = (45, 37, 30), (22.5, 18.5, 15.0), (84, 60), 0.9, 9
+    kw = P.spectral_inputs(s, *s.det)
+    rng = np.random.default_rng(8)
+    dev = torch.device("cuda")
+    gI = torch.from_numpy(rng.normal(size=(6, s.det[1], s.det[0])).astype(np.float32)).to(dev)
+    gIA = torch.from_numpy(rng.normal(size=gI.shape).astype(np.float32)).to(dev)
+    with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
+        proj.set_volumes(vols)
+        proj.set_spectral(**kw)
+        a = proj.backward(gI, gIA, view_begin=2, view_count=6)
+        b2 = proj.backward(gI, gIA, view_begin=2, view_count=6)
+        torch.cuda.synchronize()
+        assert torch.equal(a, b2)
+        proj.set_tuning(adjoint="scatter")
+        c = proj.backward(gI, gIA, view_begin=2, view_count=6)
+        torch.cuda.synchronize()
+    g = O.geom_from_setup(s)
+    p = O.paras(g)
+    L64 = O.march_f64(g, p, vols, 2, 6)
+    gL = O.spectrum_grad_f64(L64, kw["atten"], kw["spectrum"], kw["i0_val"], gI.cpu().numpy(),
+                             gIA.cpu().numpy(), kw["grid"], det_table(kw))
+    ref = O.backproject_f64(g, p, gL, view0=2)
+    scale = O.backproject_f64(g, p, np.abs(gL), view0=2)
+    for got in (a, c):
+        err = np.abs(got.cpu().numpy() - ref) / np.maximum(scale, 1e-3 * scale.max())
+        assert err.max() < 1e-4, err.max()
+
+
 def test_adjoint_lds_equals_global_atomics(require_gpu):
-    """LDS-accumulated chunks against the all-global-atomics fallback (an LDS
+    """Scatter adjoint: LDS-accumulated chunks against the all-global-atomics fallback (an LDS
     budget too small for any chunk box).  Both sum in FP32 atomics in an
     order that varies from run to run, and random-sign upstream gradients
     make the sums cancel, so they agree to 1e-4 of the floor 1e-3 max|grad|,
@@ -133,7 +172,7 @@ def test_adjoint_lds_equals_global_atomics(require_gpu):
     with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
         proj.set_volumes(vols)
         proj.set_spectral(**kw)
-        proj.set_tuning(march="lds", chunk=64, lds_bytes=1024)
+        proj.set_tuning(march="lds", chunk=64, lds_bytes=1024, adjoint="scatter")
         ref = proj.backward(gI, None).cpu().numpy()
         for march, chunk, lds in [("lds", 16, 40 * 1024), ("lds", 48, 12 * 1024), ("lds", 32, 2048),
                                   ("global", 16, 40 * 1024)]:
