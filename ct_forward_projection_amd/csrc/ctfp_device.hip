@@ -84,6 +84,7 @@ struct ctfp_context {
     int n_act = 0;
     int spec_n_mat = 0;  // materials the spectral tables were built for
     float *A = nullptr;  // [n_act][n_mat]
+    _Float16 *Aq = nullptr; // [n_act rounded up to 32][16]: MFMA operand of 2^8 A
     float *W = nullptr;  // [n_act][nv or 1][nu]
     int64_t w_stride_e = 0;
     int32_t w_stride_v = 0;
@@ -285,6 +286,7 @@ struct MarchArgs {
     // spectrum
     int32_t n_act;
     const float *A; // [n_act][M]
+    const _Float16 *Aq; // [n_act rounded up to 32][16]: 2^8 A split into FP16 hi / lo (MFMA operand)
     const float *W;
     int64_t w_stride_e;
     int32_t w_stride_v;
@@ -691,8 +693,101 @@ __device__ __forceinline__ void attenuate(float tau, float &ex, float &ab)
     ab = fabsf(tau) < 0.35f ? tau * q : 1.0f - ex;
 }
 
-// Writes L and, with the spectrum, I and IAbsorb of one ray.
-template <int M, bool kSpectrum>
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float float16_t __attribute__((ext_vector_type(16)));
+
+// The spectrum of the wavefront's 64 rays with the attenuation exponents
+// from the matrix cores: tau[e][ray] = sum_k A'[e][k] L'[k][ray] with
+// v_mfma_f32_32x32x16_f16 (energies x rays, 32 x 32 per instruction).  Both
+// operands are split into FP16 hi + lo parts and K holds, per material,
+// (A_hi, L_hi), (A_hi, L_lo), (A_lo, L_hi): the products are exact in the FP32
+// accumulator and only A_lo L_lo (~2^-22 relative) is left out.  A is scaled
+// by 2^8 (P.Aq, [energy rows padded to 32][16] FP16) so A_lo stays normal.
+// Lane l (half h = l / 32, column c = l % 32) receives tau for rays c and
+// 32 + c and the energy rows 8b + 4h + q (b, q < 4) of each 32-row block, runs
+// the same exp / weighted FP64 sums as the per-lane loop for those, and one
+// exchange with lane l ^ 32 completes the sums of its own ray.  Every lane
+// of the wavefront must call it; `active` marks rays inside the detector.
+template <int M>
+__device__ __forceinline__ void spectrum_mfma(const MarchArgs &P, const TileLane &t, const float (&Lm)[M],
+                                              double &sI_out, double &sA_out)
+{
+    static_assert(3 * M <= 16, "K = 3M must fit one 32x32x16 MFMA");
+    const int lane = threadIdx.x & 63, h = lane >> 5;
+    // the other ray of this lane's column lives in lane ^ 32
+    float Lx[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) Lx[m] = __shfl_xor(Lm[m], 32);
+    const int pix_own = t.active ? t.v * (int)P.w_stride_v + t.u : -1;
+    const int pix_x = __shfl_xor(pix_own, 32);
+    double sI[2] = {0.0, 0.0}, sA[2] = {0.0, 0.0};
+    const float16_t zero = {};
+    const int n_act = P.n_act;
+#pragma unroll 1
+    for (int c = 0; c < 2; c++) {
+        // ray c * 32 + (lane % 32): this lane's own ray when c == h
+        const bool own = c == h;
+        _Float16 kv[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) kv[k] = (_Float16)0.0f;
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const float L = own ? Lm[m] : Lx[m];
+            const _Float16 hi = (_Float16)L;
+            const _Float16 lo = (_Float16)(L - (float)hi);
+            kv[3 * m] = hi;
+            kv[3 * m + 1] = lo;
+            kv[3 * m + 2] = hi;
+        }
+        // lanes 0-31 hold k = 0..7, lanes 32-63 k = 8..15 (a dword-wise select)
+        const uint32_t sel = 0u - (uint32_t)h;
+        uint32_t bw[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t lo = (uint32_t)__builtin_bit_cast(uint16_t, kv[2 * j]) |
+                                ((uint32_t)__builtin_bit_cast(uint16_t, kv[2 * j + 1]) << 16);
+            const uint32_t hi = (uint32_t)__builtin_bit_cast(uint16_t, kv[8 + 2 * j]) |
+                                ((uint32_t)__builtin_bit_cast(uint16_t, kv[8 + 2 * j + 1]) << 16);
+            bw[j] = (hi & sel) | (lo & ~sel);
+        }
+        const half8_t b = __builtin_bit_cast(half8_t, bw);
+        const int pix = own ? pix_own : pix_x;
+        const float *wp = P.W + (pix < 0 ? 0 : pix);
+        double si = 0.0, sa = 0.0;
+#pragma unroll 1
+        for (int rb = 0; rb * 32 < n_act; rb++) {
+            const half8_t a = *(const half8_t *)(P.Aq + ((rb * 32 + (lane & 31)) * 16 + 8 * h));
+            const float16_t d = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, zero, 0, 0, 0);
+#pragma unroll
+            for (int bb = 0; bb < 4; bb++) {
+                if (rb * 32 + 8 * bb >= n_act) break; // uniform: the rest of the block is past the bins
+                // four rows at a time: keeps the weight loads and exp temporaries few
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int e = rb * 32 + 8 * bb + 4 * h + q;
+                    if (e < n_act && pix >= 0) {
+                        float ex, ab;
+                        attenuate(d[4 * bb + q] * (1.0f / 256.0f), ex, ab);
+                        const double w = (double)wp[(int64_t)e * P.w_stride_e];
+                        si = fma(w, (double)ex, si);
+                        sa = fma(w, (double)ab, sa);
+                    }
+                }
+            }
+        }
+        if (c == 0) { sI[0] = si; sA[0] = sa; } else { sI[1] = si; sA[1] = sa; }
+    }
+    // partial sums of the partner's own ray go to the partner
+    const double rI = __shfl_xor(h ? sI[0] : sI[1], 32);
+    const double rA = __shfl_xor(h ? sA[0] : sA[1], 32);
+    sI_out = (h ? sI[1] : sI[0]) + rI;
+    sA_out = (h ? sA[1] : sA[0]) + rA;
+}
+
+// Writes L and, with the spectrum, I and IAbsorb of one ray.  Every lane of
+// the wavefront calls it; lanes past the detector edge write nothing.
+template <int M, int ST, bool kSpectrum>
 __device__ __forceinline__ void finish_ray(const MarchArgs &P, const TileLane &t, const float (&acc)[M])
 {
     const int64_t npix = (int64_t)P.nu * P.nv;
@@ -700,11 +795,20 @@ __device__ __forceinline__ void finish_ray(const MarchArgs &P, const TileLane &t
     float Lm[M];
 #pragma unroll
     for (int m = 0; m < M; m++) Lm[m] = acc[m] * P.dx_f;
-    if (P.L) {
+    if (P.L && t.active) {
 #pragma unroll
         for (int m = 0; m < M; m++) P.L[((int64_t)t.vw_local * M + m) * npix + pix] = Lm[m];
     }
-    if constexpr (kSpectrum) {
+    if constexpr (kSpectrum && ST != kStF32 && 3 * M <= 16) {
+        // the mixed-precision projector: exponents on the matrix cores
+        double sI, sA;
+        spectrum_mfma<M>(P, t, Lm, sI, sA);
+        if (t.active) {
+            P.I[(int64_t)t.vw_local * npix + pix] = (float)sI;
+            P.IA[(int64_t)t.vw_local * npix + pix] = (float)sA;
+        }
+    } else if constexpr (kSpectrum) {
+        if (!t.active) return;
         // Per-ray spectrum: tau_e = sum_m A[e][m] L_m (FP32), e^-tau and
         // 1 - e^-tau (attenuate: no cancellation at small tau) in FP32, and the
         // energy sums of W e^-tau and W (1 - e^-tau) in bin order in FP64 (an
@@ -737,7 +841,7 @@ __device__ __forceinline__ uint32_t xcd_logical(const MarchArgs &P)
 
 // Ray-march with direct global gathers: 8 (FP32) or 4 (16-bit) loads per sample.
 template <int M, int ST, bool kSpectrum>
-__global__ __launch_bounds__(kBlock) void march_kernel(MarchArgs P)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void march_kernel(MarchArgs P)
 {
     const uint32_t logical = xcd_logical(P);
     if (logical >= P.n_blocks) return;
@@ -765,7 +869,7 @@ __global__ __launch_bounds__(kBlock) void march_kernel(MarchArgs P)
         for (float fi = (float)i; i < i1; i++, fi += 1.0f)
             accumulate_global<M, ST>(rsrc, absolute_cell(A, offset_cell(A, fi), nx, ny, nz), oy, oz, acc);
     }
-    if (t.active) finish_ray<M, kSpectrum>(P, t, acc);
+    finish_ray<M, ST, kSpectrum>(P, t, acc);
 }
 
 // Block-wide min / max of six ints (cell box of a chunk).
@@ -928,11 +1032,11 @@ __device__ __forceinline__ void march_tile(const MarchArgs &P, uint32_t logical,
 #pragma unroll
     for (int m = 0; m < M; m++) acc[m] = 0.0f;
     march_chunks<M, ST>(P, R, rsrc, lds_raw + 256, (int *)lds_raw, acc);
-    if (t.active) finish_ray<M, kSpectrum>(P, t, acc);
+    finish_ray<M, ST, kSpectrum>(P, t, acc);
 }
 
 template <int M, int ST, bool kSpectrum>
-__global__ __launch_bounds__(kBlock) void march_lds_kernel(MarchArgs P)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void march_lds_kernel(MarchArgs P)
 {
     extern __shared__ char lds_raw[];
     const uint32_t logical = xcd_logical(P);
@@ -968,7 +1072,7 @@ __global__ void batch_reset(uint32_t *taken)
 }
 
 template <int ST>
-__global__ __launch_bounds__(kBlock) void batch_kernel(const BatchItem *__restrict__ items, BatchQueues Q)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void batch_kernel(const BatchItem *__restrict__ items, BatchQueues Q)
 {
     extern __shared__ char lds_raw[];
     uint32_t *slot = (uint32_t *)lds_raw + 60; // beside the block_box scratch (ints 0..31)
@@ -1586,6 +1690,7 @@ int ctfp_destroy(ctfp_context *ctx)
     (void)hipDeviceSynchronize();
     dfree(ctx->vol);
     dfree(ctx->adj_ws);
+    dfree(ctx->Aq);
     dfree(ctx->A);
     dfree(ctx->W);
     dfree(ctx->I0);
@@ -1789,10 +1894,29 @@ int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
     const int64_t rows = per_pixel ? nv : 1;
     ctx->generation++; // the tables are replaced from here on
     dfree(ctx->A);
+    dfree(ctx->Aq);
     dfree(ctx->W);
     if (!ctx->I0) HIP_TRY(hipMalloc(&ctx->I0, (size_t)npix * sizeof(float)));
     HIP_TRY(hipMalloc(&ctx->A, A.size() * sizeof(float)));
     HIP_TRY(hipMemcpy(ctx->A, A.data(), A.size() * sizeof(float), hipMemcpyHostToDevice));
+    {
+        // MFMA operand: row e, k = 3m + (0, 1, 2) = (hi, hi, lo) of 2^8 A[e][m]
+        const int rows = std::max(32, (na + 31) / 32 * 32);
+        std::vector<_Float16> Aq((size_t)rows * 16, (_Float16)0.0f);
+        if (3 * sp->n_mat <= 16) {
+            for (int i = 0; i < na; i++) {
+                for (int m = 0; m < sp->n_mat; m++) {
+                    const float a = A[(size_t)i * sp->n_mat + m] * 256.0f;
+                    const _Float16 hi = (_Float16)a;
+                    Aq[(size_t)i * 16 + 3 * m] = hi;
+                    Aq[(size_t)i * 16 + 3 * m + 1] = hi;
+                    Aq[(size_t)i * 16 + 3 * m + 2] = (_Float16)(a - (float)hi);
+                }
+            }
+        }
+        HIP_TRY(hipMalloc(&ctx->Aq, Aq.size() * sizeof(_Float16)));
+        HIP_TRY(hipMemcpy(ctx->Aq, Aq.data(), Aq.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMalloc(&ctx->W, (size_t)std::max(na, 1) * rows * nu * sizeof(float)));
     // stage the per-bin tables of the active bins
     std::vector<float> T, eta, mu;
@@ -1864,6 +1988,7 @@ static int forward_args(ctfp_context *ctx, int64_t view_begin, int64_t view_coun
         if (!I || !IA) return set_error(CTFP_ERR_INVALID, "null output");
         P.n_act = ctx->n_act;
         P.A = ctx->A;
+        P.Aq = ctx->Aq;
         P.W = ctx->W;
         P.w_stride_e = ctx->w_stride_e;
         P.w_stride_v = ctx->w_stride_v;
