@@ -909,9 +909,79 @@ __device__ __forceinline__ void block_box(int (&lo)[3], int (&hi)[3], int *scrat
 __device__ __forceinline__ int small_div(int n, int d, float inv_d)
 {
     int q = (int)((float)n * inv_d);
-    q -= (q * d > n);
-    q += ((q + 1) * d <= n);
+    q -= (__mul24(q, d) > n);
+    q += (__mul24(q + 1, d) <= n);
     return q;
+}
+
+// 1 / d to within an ulp (v_rcp_f32); small_div corrects the quotient.
+__device__ __forceinline__ float small_rcp(int d) { return __builtin_amdgcn_rcpf((float)d); }
+
+// Cooperative, row-coalesced load of the cell box [lo, lo + (sx, sy, sz)) into
+// LDS (cell i of the box at byte i * LB, x fastest).  Cells outside the
+// padded volume are zeros.  A thread keeps one x column and walks rows
+// (y, z) with a fixed stride, so the x test and offset are hoisted and the
+// loop holds no division.
+template <int M>
+__device__ __forceinline__ void fill_box(__amdgpu_buffer_rsrc_t rsrc, char *lds, const int (&lo)[3], int sx,
+                                         int sy, int sz, int nx, int ny, int nz, uint32_t oy, uint32_t oz)
+{
+    constexpr uint32_t LB = lds_cell_bytes<M>();
+    const int rows = sy * sz;
+    if (sx <= kBlock) {
+        const int per = kBlock / sx; // rows per pass
+        const int tid = threadIdx.x;
+        if (tid >= per * sx) return;
+        const float inv_sx = small_rcp(sx), inv_sy = small_rcp(sy);
+        const int r0 = small_div(tid, sx, inv_sx);
+        const int x = lo[0] + (tid - r0 * sx);
+        int lz = small_div(r0, sy, inv_sy);
+        int ly = r0 - lz * sy;
+        const int dz = small_div(per, sy, inv_sy), dy = per - dz * sy;
+        const bool xin = (uint32_t)(x + kVoxPadLo) < (uint32_t)(nx + kVoxPadLo + kVoxPadHi);
+        const uint32_t xoff = ((uint32_t)(x + kVoxPadLo) & 0xFFFFFu) * (4u * M);
+        const uint32_t ymax = (uint32_t)(ny + kVoxPadLo + kVoxPadHi), zmax = (uint32_t)(nz + kVoxPadLo + kVoxPadHi);
+        uint32_t loff = (uint32_t)tid * LB;
+        const uint32_t lstep = (uint32_t)(per * sx) * LB;
+        for (int r = r0; r < rows; r += per) {
+            const uint32_t yi = (uint32_t)(lo[1] + ly + kVoxPadLo), zi = (uint32_t)(lo[2] + lz + kVoxPadLo);
+            uint32_t d[M];
+            if (xin && yi < ymax && zi < zmax) {
+                load_raw<M>(rsrc, __umul24(zi, oz) + (__umul24(yi, oy) + xoff), 0, d);
+            } else {
+#pragma unroll
+                for (int m = 0; m < M; m++) d[m] = 0u;
+            }
+            lds_write<M>(lds, loff, d);
+            loff += lstep;
+            ly += dy;
+            lz += dz;
+            if (ly >= sy) {
+                ly -= sy;
+                lz++;
+            }
+        }
+    } else {
+        const int cells = rows * sx;
+        const float inv_sx = small_rcp(sx), inv_sy = small_rcp(sy);
+        for (int i = threadIdx.x; i < cells; i += kBlock) {
+            const int row = small_div(i, sx, inv_sx), lx = i - row * sx;
+            const int lz = small_div(row, sy, inv_sy), ly = row - lz * sy;
+            const int x = lo[0] + lx, y = lo[1] + ly, z = lo[2] + lz;
+            uint32_t d[M];
+            if (x >= -kVoxPadLo && x < nx + kVoxPadHi && y >= -kVoxPadLo && y < ny + kVoxPadHi &&
+                z >= -kVoxPadLo && z < nz + kVoxPadHi) {
+                const uint32_t xi = (uint32_t)(x + kVoxPadLo) & 0xFFFFFu;
+                const uint32_t yi = (uint32_t)(y + kVoxPadLo) & 0xFFFFFu;
+                const uint32_t zi = (uint32_t)(z + kVoxPadLo) & 0xFFFFFu;
+                load_raw<M>(rsrc, zi * oz + (yi * oy + xi * (4u * M)), 0, d);
+            } else {
+#pragma unroll
+                for (int m = 0; m < M; m++) d[m] = 0u;
+            }
+            lds_write<M>(lds, (uint32_t)i * LB, d);
+        }
+    }
 }
 
 // Ray-march of one block's rays with the volume staged in LDS per chunk of
@@ -945,27 +1015,7 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
         const int sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
         const bool staged = (int64_t)sx * sy * sz <= (int64_t)P.lds_cells;
         if (staged) {
-            // cooperative, row-coalesced fill of the box [lo, hi]; cells
-            // outside the padded volume are zeros
-            const int cells = sx * sy * sz;
-            const float inv_sx = 1.0f / (float)sx, inv_sy = 1.0f / (float)sy;
-            for (int i = threadIdx.x; i < cells; i += kBlock) {
-                const int row = small_div(i, sx, inv_sx), lx = i - row * sx;
-                const int lz = small_div(row, sy, inv_sy), ly = row - lz * sy;
-                const int x = lo[0] + lx, y = lo[1] + ly, z = lo[2] + lz;
-                uint32_t d[M];
-                if (x >= -kVoxPadLo && x < nx + kVoxPadHi && y >= -kVoxPadLo && y < ny + kVoxPadHi &&
-                    z >= -kVoxPadLo && z < nz + kVoxPadHi) {
-                    const uint32_t xi = (uint32_t)(x + kVoxPadLo) & 0xFFFFFu;
-                    const uint32_t yi = (uint32_t)(y + kVoxPadLo) & 0xFFFFFu;
-                    const uint32_t zi = (uint32_t)(z + kVoxPadLo) & 0xFFFFFu;
-                    load_raw<M>(rsrc, zi * oz + (yi * oy + xi * (4u * M)), 0, d);
-                } else {
-#pragma unroll
-                    for (int m = 0; m < M; m++) d[m] = 0u;
-                }
-                lds_write<M>(lds, (uint32_t)i * LB, d);
-            }
+            fill_box<M>(rsrc, lds, lo, sx, sy, sz, nx, ny, nz, oy, oz);
             __syncthreads();
             const int ry = sx * (int)LB, rz = sx * sy * (int)LB;
             const uint32_t roff[4] = {0u, (uint32_t)ry, (uint32_t)rz, (uint32_t)(ry + rz)};
@@ -1177,7 +1227,7 @@ __device__ __forceinline__ void scatter_chunks(const MarchArgs &P, const RaySetu
                 }
             }
             __syncthreads();
-            const float inv_sx = 1.0f / (float)sx, inv_sy = 1.0f / (float)sy;
+            const float inv_sx = small_rcp(sx), inv_sy = small_rcp(sy);
             for (int i = threadIdx.x; i < cells; i += kBlock) {
                 const int row = small_div(i, sx, inv_sx), lx = i - row * sx;
                 const int lz = small_div(row, sy, inv_sy), ly = row - lz * sy;
