@@ -265,7 +265,7 @@ def run_batch(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--views", type=int, default=0, help="views per step (0 = the full 360)")
     ap.add_argument("--dtype", choices=["f32", "f16", "bf16"], default="f16",
@@ -360,6 +360,7 @@ def main():
         log(f"{label}: timed {args.steps} steps, wall {wall:.3f} s, per-launch ms {kern}")
         return wall, kern
 
+    marched = proj.marched_samples(0, views)
     wall, kern_ms = timed(proj, args.dtype)
     ms_per_step = wall * 1e3 / args.steps
     gsps = samples / (ms_per_step / 1e3) / 1e9
@@ -385,6 +386,8 @@ def main():
                    "views_per_step": views, "samples_per_step": samples,
                    "materials": n_mat, "volume_dtype": args.dtype, "tuning": proj.get_tuning()},
         "views_per_s": round(views / (ms_per_step / 1e3), 2),
+        "median_launch_ms": round(sorted(kern_ms)[len(kern_ms) // 2], 3),
+        "evaluated_fraction": round(marched / samples, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},

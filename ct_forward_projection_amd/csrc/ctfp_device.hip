@@ -1094,6 +1094,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     march_tile<M, ST, kSpectrum>(P, logical, lds_raw);
 }
 
+// Samples the march evaluates (each ray's clipped window [k_lo, k_hi)),
+// summed per workgroup into *count.
+__global__ __launch_bounds__(kBlock) void count_kernel(MarchArgs P, unsigned long long *count)
+{
+    __shared__ unsigned long long part[kBlock / 64];
+    const uint32_t logical = blockIdx.x;
+    if (logical >= P.n_blocks) return;
+    const TileLane t = tile_lane(P, logical);
+    const RaySetup R = lane_ray(P, t);
+    unsigned long long n = (unsigned long long)R.n;
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) n += __shfl_xor(n, s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int w = 0; w < kBlock / 64; w++) tot += part[w];
+        atomicAdd(count, tot);
+    }
+}
+
 // ------------------------------------------------------------ batches ---
 //
 // Several scans (contexts, each with its own volume, detector, views and
@@ -2254,6 +2275,28 @@ int ctfp_postprocess(int64_t n_views, int64_t npix, const float *I, const float 
 int ctfp_synchronize(void *stream)
 {
     HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return CTFP_OK;
+}
+
+int ctfp_count_samples(ctfp_context *ctx, int64_t view_begin, int64_t view_count, int64_t *marched)
+{
+    if (check_ctx(ctx) || !marched) return set_error(CTFP_ERR_INVALID, "null argument");
+    MarchArgs P;
+    int rc = fill_march_args(ctx, view_begin, view_count, P);
+    if (rc) return rc;
+    DeviceGuard g(ctx->device);
+    unsigned long long *d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(unsigned long long)));
+    unsigned long long h = 0;
+    hipError_t e = hipMemcpy(d, &h, sizeof h, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        count_kernel<<<P.n_blocks, kBlock>>>(P, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return set_error(CTFP_ERR_DEVICE, "count: %s", hipGetErrorString(e));
+    *marched = (int64_t)h;
     return CTFP_OK;
 }
 

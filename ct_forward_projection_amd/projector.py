@@ -175,6 +175,13 @@ class Projector:
                 "lds_bytes": t.lds_bytes, "wave_u": t.wave_u, "block_u": t.block_u,
                 "adjoint": "scatter" if t.adjoint == L.CTFP_ADJOINT_SCATTER else "gather"}
 
+    def marched_samples(self, view_begin: int = 0, view_count: int | None = None) -> int:
+        """Samples the march evaluates (rays' windows clipped to the volume)."""
+        vb, vc = self._views(view_begin, view_count)
+        n = C.c_int64()
+        L.check(L.lib().ctfp_count_samples(self._ctx, vb, vc, C.byref(n)))
+        return int(n.value)
+
     @property
     def samples_per_view(self) -> int:
         return int(self.nu * self.nv * self.info["int_num"])

@@ -215,6 +215,13 @@ int ctfp_forward_host(ctfp_context *ctx, float *I, float *IAbsorb, float *I0);
 
 /* ---- launch tuning and batches (no reference counterpart) -------------- */
 
+/* Number of samples the march evaluates for views [view_begin, +view_count):
+ * each ray's window clipped to the volume's support, against the fixed
+ * K per ray of the reference's lattice (the rest are exact zeros).
+ * Synchronous. */
+int ctfp_count_samples(ctfp_context *ctx, int64_t view_begin, int64_t view_count, int64_t *marched);
+
+
 /* Times candidate launch options (ray-tile shape, samples per staged chunk,
  * LDS budget) on two groups of the scan's views a quarter turn apart, keeps
  * the fastest in the context and returns it in `best` (may be NULL).

@@ -415,3 +415,55 @@ def test_lds_staging_equals_global_gather(require_gpu, dtype):
         for bad in (0, 1, 7, 24, 4112):
             with pytest.raises(P.CTFPError):
                 proj.set_tuning(chunk=bad)
+
+
+def _windows_numpy(s, view):
+    """Per-ray sample windows [k_lo, k_hi) of a scan, restated in numpy: the
+    samples whose trilinear support can reach the volume, plus one each side."""
+    import math
+    info = P.compute_paras(P.scan_params(**s.scan_kwargs()))
+    K, dx = int(info["int_num"]), info["dx"]
+    nu, nv = s.det
+    uu, vv = np.meshgrid(np.arange(nu, dtype=np.float64), np.arange(nv, dtype=np.float64))
+    hu, hv = nu * s.d_size / 2, nv * s.d_size / 2
+    detU = hu - s.d_size / 2 - uu * s.d_size
+    detV = hv - s.d_size / 2 - vv * s.d_size
+    r = np.sqrt(s.sdd ** 2 + detU ** 2 + detV ** 2)
+    dX = np.sqrt(s.sdd ** 2 + detU ** 2) / r * np.sqrt(s.sdd ** 2 + detV ** 2) / r
+    dY = np.sqrt(s.sdd ** 2 + detU ** 2) / r * detU / r
+    dZ = detV / r
+    th = s.rot * view * np.float32(360.0 / s.views) * math.pi / 180.0
+    c, sn = math.cos(th), math.sin(th)
+    psz = [s.length_mm[a] / s.n_vox[a] for a in range(3)]
+    half = [s.length_mm[a] / 2 for a in range(3)]
+    t0 = s.sod - info["fov_r"] + dx / 2
+    dirs = [(dX * c - dY * sn) / psz[0], -(dX * sn + dY * c) / psz[1], -dZ / psz[2]]
+    offs = [(half[0] - s.sod * c) / psz[0], (half[1] + s.sod * sn) / psz[1], half[2] / psz[2]]
+    lo = np.zeros_like(uu)
+    hi = np.full_like(uu, K - 1.0)
+    for a in range(3):
+        st = offs[a] + dirs[a] * t0
+        d = dirs[a] * dx
+        with np.errstate(divide="ignore", invalid="ignore"):
+            k1 = (-1.0 - st) / d
+            k2 = (s.n_vox[a] - st) / d
+        lo = np.maximum(lo, np.minimum(k1, k2))
+        hi = np.minimum(hi, np.maximum(k1, k2))
+    ok = hi >= lo
+    klo = np.where(ok, np.maximum(0, np.floor(lo) - 1), 0)
+    khi = np.where(ok, np.minimum(K, np.ceil(hi) + 2), 0)
+    return int((khi - klo).sum())
+
+
+def test_count_samples_matches_window_restatement(require_gpu):
+    s, vols = small_three_material(n=40, views=5, det=(96, 64), d_size=1.0)
+    with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
+        proj.set_volumes(vols)
+        total = proj.marched_samples()
+        per_view = [proj.marched_samples(v, 1) for v in range(s.views)]
+    expect = [_windows_numpy(s, v) for v in range(s.views)]
+    assert sum(per_view) == total
+    for got, ref in zip(per_view, expect):
+        # FP64 on both sides; a window end exactly on an integer could round
+        # differently, one sample per such ray at most
+        assert abs(got - ref) <= max(2, ref // 100000), (got, ref)
