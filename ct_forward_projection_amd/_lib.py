@@ -35,7 +35,7 @@ EXPORTED = (
     "ctfp_set_tuning", "ctfp_get_tuning", "ctfp_backward", "ctfp_set_volumes_async",
     "ctfp_get_scan", "ctfp_get_device", "ctfp_device_bytes", "ctfp_autotune",
     "ctfp_batch_create", "ctfp_batch_run", "ctfp_batch_samples", "ctfp_batch_destroy",
-    "ctfp_count_samples",
+    "ctfp_count_samples", "ctfp_set_focal_spot", "ctfp_focal_spot_offsets",
 )
 
 CTFP_MARCH_GLOBAL = 0
@@ -150,6 +150,8 @@ def _declare(lib):
         "ctfp_device_bytes": (i32, [vp, P(i64)]),
         "ctfp_autotune": (i32, [vp, i32, P(Tuning)]),
         "ctfp_count_samples": (i32, [vp, i64, i64, P(i64)]),
+        "ctfp_set_focal_spot": (i32, [vp, _FP, _FP]),
+        "ctfp_focal_spot_offsets": (i32, [i64, C.c_float, C.c_uint32, _FP, _FP]),
         "ctfp_batch_create": (i32, [P(BatchItem), i32, P(vp)]),
         "ctfp_batch_run": (i32, [vp, vp]),
         "ctfp_batch_samples": (i32, [vp, P(i64)]),

@@ -91,6 +91,8 @@ struct ctfp_context {
     float *I0 = nullptr; // [nv][nu]
     // per-view rotation (cos, sin) of the reference's FP32 angle
     double2 *view_cs = nullptr;
+    // per-view focal-spot source offsets (mm), or null
+    double2 *focal = nullptr;
     // host-mode output staging
     float *dI = nullptr, *dIA = nullptr;
     int64_t staged_views = 0;
@@ -277,6 +279,7 @@ struct MarchArgs {
     double half_x, half_y, half_z, inv_px, inv_py, inv_pz;
     int32_t nu, nv, K;
     const double2 *view_cs;
+    const double2 *focal; // per-view source offsets (mm, detector u / v directions), or null
     int64_t view_begin;
     int32_t n_views;
     int32_t tiles_u, tiles_v;
@@ -351,8 +354,9 @@ struct RayLine {
     double sa, sb, sc; // source (t = 0)
     double da, db, dc; // d(index)/dt
 };
-__device__ __forceinline__ RayLine ray_line(const MarchArgs &P, int u, int v, double2 cs)
+__device__ __forceinline__ RayLine ray_line(const MarchArgs &P, int u, int v, int vw)
 {
+    const double2 cs = P.view_cs[vw];
     // computeIntPointCoordinatesKernel (CBCTPolyProjGPU.cu:1813-1829), FP64.
     double detU = P.half_u - P.dsize * 0.5 - (double)u * P.dsize;
     double detV = P.half_v - P.dsize * 0.5 - (double)v * P.dsize;
@@ -366,25 +370,52 @@ __device__ __forceinline__ RayLine ray_line(const MarchArgs &P, int u, int v, do
     double dX = cg * cb, dY = cg * sb, dZ = sg;
     // transformKernel (CBCTPolyProjGPU.cu:1915-1917) is affine in t.
     RayLine L;
-    L.da = (dX * cs.x - dY * cs.y) * P.inv_px;
-    L.db = -(dX * cs.y + dY * cs.x) * P.inv_py;
-    L.dc = -dZ * P.inv_pz;
-    L.sa = (P.half_x - P.sod * cs.x) * P.inv_px;
-    L.sb = (P.half_y + P.sod * cs.y) * P.inv_py;
-    L.sc = P.half_z * P.inv_pz;
+    if (!P.focal) {
+        L.da = (dX * cs.x - dY * cs.y) * P.inv_px;
+        L.db = -(dX * cs.y + dY * cs.x) * P.inv_py;
+        L.dc = -dZ * P.inv_pz;
+        L.sa = (P.half_x - P.sod * cs.x) * P.inv_px;
+        L.sb = (P.half_y + P.sod * cs.y) * P.inv_py;
+        L.sc = P.half_z * P.inv_pz;
+        return L;
+    }
+    // Focal spot: the source moves to (-SOD, su, sv) in its frame and the ray
+    // still reaches the point where the nominal ray meets the detector plane
+    // x = SDD - SOD; samples are measured from the displaced source.
+    const double2 f = P.focal[vw];
+    const double qx = P.sdd, qy = P.sdd * dY / dX - f.x, qz = P.sdd * dZ / dX - f.y;
+    const double inv_n = 1.0 / sqrt(qx * qx + qy * qy + qz * qz);
+    const double ex = qx * inv_n, ey = qy * inv_n, ez = qz * inv_n;
+    L.da = (ex * cs.x - ey * cs.y) * P.inv_px;
+    L.db = -(ex * cs.y + ey * cs.x) * P.inv_py;
+    L.dc = -ez * P.inv_pz;
+    // source (-SOD, su, sv) rotated like every sample
+    L.sa = (P.half_x - P.sod * cs.x - f.x * cs.y) * P.inv_px;
+    L.sb = (P.half_y + P.sod * cs.y - f.x * cs.x) * P.inv_py;
+    L.sc = (P.half_z - f.y) * P.inv_pz;
     return L;
 }
 
 // Continuous detector coordinates (u, v) of the ray through the index-space
 // point (a, b, c) in the view cs: the inverse of ray_line's direction model
 // (azimuth tan = detU / sqrt(SDD^2 + detV^2), elevation sin = detV / r).
-__device__ __forceinline__ void project_point(const MarchArgs &P, double2 cs, double a, double b, double c,
+__device__ __forceinline__ void project_point(const MarchArgs &P, int vw, double a, double b, double c,
                                               double &u, double &v)
 {
+    const double2 cs = P.view_cs[vw];
     const double qx = a / P.inv_px - P.half_x + P.sod * cs.x; // world, relative to the source
     const double qy = b / P.inv_py - P.half_y - P.sod * cs.y;
     const double qz = c / P.inv_pz - P.half_z;
-    const double X = qx * cs.x - qy * cs.y, Y = -qx * cs.y - qy * cs.x, Z = -qz;
+    double X = qx * cs.x - qy * cs.y, Y = -qx * cs.y - qy * cs.x, Z = -qz;
+    if (P.focal) {
+        // through the displaced source to the detector plane, then the nominal
+        // direction from the nominal source to that point
+        const double2 f = P.focal[vw];
+        const double lam = P.sdd / X;
+        Y = f.x + lam * (Y - f.x);
+        Z = f.y + lam * (Z - f.y);
+        X = P.sdd;
+    }
     const double al = Z / sqrt(X * X + Y * Y), be = Y / X;
     const double den = 1.0 - al * al * be * be;
     const double detV = al * P.sdd * sqrt((1.0 + be * be) / den);
@@ -395,7 +426,7 @@ __device__ __forceinline__ void project_point(const MarchArgs &P, double2 cs, do
 
 __device__ __forceinline__ RaySetup setup_ray(const MarchArgs &P, int u, int v, int vw)
 {
-    const RayLine Ln = ray_line(P, u, v, P.view_cs[vw]);
+    const RayLine Ln = ray_line(P, u, v, vw);
     double sa = Ln.sa + Ln.da * P.t0, sb2 = Ln.sb + Ln.db * P.t0, sc = Ln.sc + Ln.dc * P.t0; // k = 0
     double da = Ln.da * P.dx, db = Ln.db * P.dx, dc = Ln.dc * P.dx;
     double lo = 0.0, hi = (double)P.K - 1.0;
@@ -1407,13 +1438,13 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
 #pragma unroll
     for (int m = 0; m < M; m++) acc[m] = 0.0f;
     for (int vw = 0; vw < P.n_views; vw++) {
-        const double2 cs = P.view_cs[P.view_begin + vw];
+        const int view = (int)(P.view_begin + vw);
         // shadow of the brick: centre projection and the Jacobian per voxel
         double u0, v0, ua, va, ub, vb, uc, vc;
-        project_point(P, cs, ca, cb, cc, u0, v0);
-        project_point(P, cs, ca + 1.0, cb, cc, ua, va);
-        project_point(P, cs, ca, cb + 1.0, cc, ub, vb);
-        project_point(P, cs, ca, cb, cc + 1.0, uc, vc);
+        project_point(P, view, ca, cb, cc, u0, v0);
+        project_point(P, view, ca + 1.0, cb, cc, ua, va);
+        project_point(P, view, ca, cb + 1.0, cc, ub, vb);
+        project_point(P, view, ca, cb, cc + 1.0, uc, vc);
         const float Jua = (float)(ua - u0), Jub = (float)(ub - u0), Juc = (float)(uc - u0);
         const float Jva = (float)(va - v0), Jvb = (float)(vb - v0), Jvc = (float)(vc - v0);
         // a voxel's support cube spans +-1 voxel: +-h pixels around its centre
@@ -1443,7 +1474,7 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
                 __syncthreads(); // the previous band's entries are consumed
                 for (int i = threadIdx.x; i < count; i += kBlock) {
                     const int pv = b0 + i / rw, pu = c0 + i % rw;
-                    const RayLine Ln = ray_line(P, pu, pv, cs);
+                    const RayLine Ln = ray_line(P, pu, pv, (int)(P.view_begin + vw));
                     // position at sample k: q0 + k * d, relative to the brick centre
                     const double dxa = Ln.da * P.dx, dxb = Ln.db * P.dx, dxc = Ln.dc * P.dx;
                     const double qa = Ln.sa + Ln.da * P.t0 - ca, qb = Ln.sb + Ln.db * P.t0 - cb;
@@ -1693,6 +1724,7 @@ int fill_march_args(ctfp_context *ctx, int64_t view_begin, int64_t view_count, M
     P.nv = (int32_t)s.d_num_v;
     P.K = (int32_t)y.int_num;
     P.view_cs = ctx->view_cs;
+    P.focal = ctx->focal;
     P.view_begin = view_begin;
     P.n_views = (int32_t)view_count;
     P.wu_log2 = ilog2(ctx->tune.wave_u);
@@ -1761,6 +1793,7 @@ int ctfp_destroy(ctfp_context *ctx)
     (void)hipDeviceSynchronize();
     dfree(ctx->vol);
     dfree(ctx->adj_ws);
+    dfree(ctx->focal);
     dfree(ctx->Aq);
     dfree(ctx->A);
     dfree(ctx->W);
@@ -1860,6 +1893,29 @@ int ctfp_set_volumes_async(ctfp_context *ctx, int n_mat, const float *const *vol
     }
     DeviceGuard g(ctx->device);
     return write_layout(ctx, vp, n_mat, storage_dtype, (hipStream_t)stream, true);
+}
+
+int ctfp_set_focal_spot(ctfp_context *ctx, const float *offset_u, const float *offset_v)
+{
+    if (check_ctx(ctx)) return CTFP_ERR_INVALID;
+    if ((offset_u == nullptr) != (offset_v == nullptr))
+        return set_error(CTFP_ERR_INVALID, "focal spot: give both offset arrays or neither");
+    DeviceGuard g(ctx->device);
+    ctx->generation++;
+    if (!offset_u) {
+        dfree(ctx->focal);
+        return CTFP_OK;
+    }
+    const int64_t n = ctx->scan.proj_num;
+    std::vector<double2> f((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        if (!std::isfinite(offset_u[i]) || !std::isfinite(offset_v[i]))
+            return set_error(CTFP_ERR_INVALID, "focal spot: offset of view %lld is not finite", (long long)i);
+        f[(size_t)i] = make_double2((double)offset_u[i], (double)offset_v[i]);
+    }
+    if (!ctx->focal) HIP_TRY(hipMalloc(&ctx->focal, (size_t)n * sizeof(double2)));
+    HIP_TRY(hipMemcpy(ctx->focal, f.data(), (size_t)n * sizeof(double2), hipMemcpyHostToDevice));
+    return CTFP_OK;
 }
 
 int ctfp_get_scan(const ctfp_context *ctx, ctfp_scan_params *out)

@@ -228,4 +228,33 @@ int ctfp_grid_period_index(int64_t d_num, const float *T, int64_t *out, int64_t 
     return CTFP_OK;
 }
 
+/* computeFoSpOffset (CBCTForwardProj.cpp:122-148) with the disc test fixed:
+ * MSVC rand() after srand(seed) (the reference uses 10), offsets
+ * -r + (rand() % (size * 10000)) / 10000 mm, drawn again until
+ * u^2 + v^2 <= r^2 (the reference compares against pi r^2). */
+int ctfp_focal_spot_offsets(int64_t n_views, float spot_mm, uint32_t seed, float *offset_u, float *offset_v)
+{
+    if (n_views < 0 || !(spot_mm > 0.0f) || spot_mm > 100.0f || !offset_u || !offset_v)
+        return set_error(CTFP_ERR_INVALID, "focal spot offsets: %lld views, spot %g mm",
+                         (long long)n_views, (double)spot_mm);
+    uint32_t x = seed;
+    auto rnd = [&x]() {
+        x = x * 214013u + 2531011u;
+        return (int)((x >> 16) & 0x7FFF);
+    };
+    const float r = spot_mm / 2.0f;
+    const int span = (int)(spot_mm * 10000);
+    if (span < 1) return set_error(CTFP_ERR_INVALID, "focal spot below 0.1 um");
+    for (int64_t i = 0; i < n_views; i++) {
+        float u, v;
+        do {
+            u = -r + (float)(rnd() % span) / 10000.0f;
+            v = -r + (float)(rnd() % span) / 10000.0f;
+        } while (u * u + v * v > r * r);
+        offset_u[i] = u;
+        offset_v[i] = v;
+    }
+    return CTFP_OK;
+}
+
 } // extern "C"

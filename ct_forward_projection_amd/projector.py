@@ -66,6 +66,15 @@ def grid_transmission(d_num: int, d_size_mm: float, interspace_width: int, strip
     return T
 
 
+def focal_spot_offsets(n_views: int, spot_mm: float, seed: int = 10) -> np.ndarray:
+    """Per-view focal-spot offsets [n_views][2] in mm (CBCTForwardProj.cpp:122-148,
+    with the disc test u^2 + v^2 <= r^2)."""
+    ou = np.empty(n_views, dtype=np.float32)
+    ov = np.empty(n_views, dtype=np.float32)
+    L.check(L.lib().ctfp_focal_spot_offsets(n_views, spot_mm, seed, _fptr(ou), _fptr(ov)))
+    return np.stack([ou, ov], axis=1)
+
+
 def grid_period_index(T: np.ndarray) -> np.ndarray:
     """Grid.cpp:605-620: 0, strict local minima, d_num-1."""
     t = np.ascontiguousarray(T, dtype=np.float32)
@@ -174,6 +183,18 @@ class Projector:
         return {"march": "lds" if t.march == L.CTFP_MARCH_LDS else "global", "chunk": t.chunk,
                 "lds_bytes": t.lds_bytes, "wave_u": t.wave_u, "block_u": t.block_u,
                 "adjoint": "scatter" if t.adjoint == L.CTFP_ADJOINT_SCATTER else "gather"}
+
+    def set_focal_spot(self, offsets=None):
+        """Per-view focal-spot source offsets [proj_num][2] (mm, along the
+        detector's u and v), or None for a point source."""
+        if offsets is None:
+            L.check(L.lib().ctfp_set_focal_spot(self._ctx, None, None))
+            return
+        o = np.ascontiguousarray(offsets, dtype=np.float32)
+        if o.shape != (self.views, 2):
+            raise ValueError(f"focal offsets must be [{self.views}][2]")
+        ou, ov = np.ascontiguousarray(o[:, 0]), np.ascontiguousarray(o[:, 1])
+        L.check(L.lib().ctfp_set_focal_spot(self._ctx, _fptr(ou), _fptr(ov)))
 
     def marched_samples(self, view_begin: int = 0, view_count: int | None = None) -> int:
         """Samples the march evaluates (rays' windows clipped to the volume)."""

@@ -145,6 +145,23 @@ int ctfp_get_system_info(const ctfp_context *ctx, ctfp_system_info *out);
 int ctfp_set_volumes(ctfp_context *ctx, int n_mat, const float *const *volumes,
                      int volumes_on_device, int storage_dtype);
 
+/* Focal spot (the reference's FoSp path, computeFoSpOffset /
+ * transformFocalSpotKernel, CBCTForwardProj.cpp:122-148,
+ * CBCTPolyProjGPU.cu:1927-1954, made physical): per view, the source moves
+ * by (offset_u[view], offset_v[view]) mm along the detector's u and v
+ * directions while the detector stays, so every ray still reaches the point
+ * where its nominal ray meets the detector, from the displaced source, and
+ * the sample lattice is measured from there.  (The reference instead shifts
+ * every sample by the offset, moving the whole ray bundle.)  Host arrays of
+ * proj_num floats, copied; NULL, NULL restores the point source. */
+int ctfp_set_focal_spot(ctfp_context *ctx, const float *offset_u, const float *offset_v);
+
+/* Per-view offsets for a focal spot of diameter spot_mm: the reference's
+ * generator (MSVC rand() after srand(seed), the reference seeds 10; 0.1 um
+ * steps) with the disc test u^2 + v^2 <= r^2 in place of its pi r^2. */
+int ctfp_focal_spot_offsets(int64_t n_views, float spot_mm, uint32_t seed, float *offset_u,
+                            float *offset_v);
+
 /* Stream-ordered form of ctfp_set_volumes for FP32 device volumes (no
  * reference counterpart: the reference copies host volumes into its texture
  * arrays synchronously, initTexture3D / updateTex, CBCTPolyProjGPU.cu:2175-2221).  The layout pass is queued on `stream` and

@@ -272,9 +272,39 @@ void orc_march_f32(const orc_geom *g, const orc_paras *s, int n_mat, const float
     }
 }
 
-/* FP64 evaluation of the same lattice and filter. */
-void orc_march_f64(const orc_geom *g, const orc_paras *s, int n_mat, const float *vols,
-                   int64_t view0, int64_t n_views, double *L, int nthreads)
+/* A ray in the source frame (x along the central ray, y with detector u,
+ * z with detector v): position(t) = (x0, y0, z0) + t * (dx, dy, dz). */
+typedef struct { double x0, y0, z0, dx, dy, dz; } line_f64;
+
+/* The nominal ray of pixel (u, v) leaves the source at (-SOD, 0, 0).  With a
+ * focal-spot offset (su, sv) (mm, along y and z) the source moves to
+ * (-SOD, su, sv) while the detector stays: the ray still reaches the point
+ * where the nominal ray meets the detector plane x = SDD - SOD, and the
+ * sample lattice t_k is measured from the displaced source.  (The reference's
+ * focal-spot path instead shifts every sample by the offset,
+ * CBCTPolyProjGPU.cu:1944-1945, which moves the whole ray bundle.) */
+static line_f64 ray_line_f64(const orc_geom *g, const orc_paras *s, int64_t u, int64_t v,
+                             const double *focal)
+{
+    ray_f64 d = ray_dir_f64(g, s, u, v);
+    line_f64 r;
+    if (!focal) {
+        r.x0 = -(double)g->sod; r.y0 = 0.0; r.z0 = 0.0;
+        r.dx = d.cgcb; r.dy = d.cgsb; r.dz = d.sg;
+        return r;
+    }
+    const double sdd = g->sdd, su = focal[0], sv = focal[1];
+    const double qx = sdd, qy = sdd * d.cgsb / d.cgcb - su, qz = sdd * d.sg / d.cgcb - sv;
+    const double n = sqrt(qx * qx + qy * qy + qz * qz);
+    r.x0 = -(double)g->sod; r.y0 = su; r.z0 = sv;
+    r.dx = qx / n; r.dy = qy / n; r.dz = qz / n;
+    return r;
+}
+
+/* FP64 evaluation of the same lattice and filter.  focal: per-view source
+ * offsets [n_views][2] (mm) indexed by absolute view, or NULL. */
+void orc_march_focal_f64(const orc_geom *g, const orc_paras *s, int n_mat, const float *vols,
+                         int64_t view0, int64_t n_views, const double *focal, double *L, int nthreads)
 {
     set_threads(nthreads);
     const int64_t nvox = g->nx * g->ny * g->nz;
@@ -285,14 +315,15 @@ void orc_march_f64(const orc_geom *g, const orc_paras *s, int n_mat, const float
             double th = (double)orc_view_angle(g, s, view0 + iv);
             double cth = cos(th), sth = sin(th);
             double acc[16];
+            const double *f = focal ? focal + 2 * (view0 + iv) : NULL;
             for (int64_t u = 0; u < g->nu; u++) {
-                ray_f64 d = ray_dir_f64(g, s, u, v);
+                line_f64 d = ray_line_f64(g, s, u, v, f);
                 for (int m = 0; m < n_mat; m++) acc[m] = 0.0;
                 for (int64_t k = 0; k < s->n_steps; k++) {
                     double t = t0 + (double)k * (double)s->dx;
-                    double X = d.cgcb * t - (double)g->sod;
-                    double Y = d.cgsb * t;
-                    double Z = d.sg * t;
+                    double X = d.dx * t + d.x0;
+                    double Y = d.dy * t + d.y0;
+                    double Z = d.dz * t + d.z0;
                     double a = (X * cth - Y * sth + s->half_x) / s->px;
                     double b = (-(X * sth + Y * cth) + s->half_y) / s->py;
                     double c = (-Z + s->half_z) / s->pz;
@@ -303,6 +334,35 @@ void orc_march_f64(const orc_geom *g, const orc_paras *s, int n_mat, const float
                     L[((iv * n_mat + m) * g->nv + v) * g->nu + u] = acc[m] * (double)s->dx;
             }
         }
+    }
+}
+
+void orc_march_f64(const orc_geom *g, const orc_paras *s, int n_mat, const float *vols,
+                   int64_t view0, int64_t n_views, double *L, int nthreads)
+{
+    orc_march_focal_f64(g, s, n_mat, vols, view0, n_views, NULL, L, nthreads);
+}
+
+/* Focal-spot offsets of the reference's computeFoSpOffset
+ * (CBCTForwardProj.cpp:122-148): MSVC rand() after srand(10), offsets
+ * -r + (rand() % (size * 10000)) / 10000 mm, drawn again until inside the
+ * spot.  The reference's test compares against pi r^2 (and so keeps points
+ * outside the disc); here the test is the disc, u^2 + v^2 <= r^2. */
+void orc_focal_offsets(int64_t n_views, float spot_mm, uint32_t seed, float *ou, float *ov)
+{
+    uint32_t x = seed;
+    const float r = spot_mm / 2.0f;
+    const int span = (int)(spot_mm * 10000);
+    for (int64_t i = 0; i < n_views; i++) {
+        float u, v;
+        do {
+            x = x * 214013u + 2531011u;
+            u = -r + (float)((int)((x >> 16) & 0x7FFF) % span) / 10000.0f;
+            x = x * 214013u + 2531011u;
+            v = -r + (float)((int)((x >> 16) & 0x7FFF) % span) / 10000.0f;
+        } while (u * u + v * v > r * r);
+        ou[i] = u;
+        ov[i] = v;
     }
 }
 
@@ -655,8 +715,8 @@ int orc_num_threads(void)
  * (zero-padded border: corners outside the volume are dropped).
  * gL: [n_views][n_mat][nv][nu]; grad: [n_mat][nz][ny][nx], accumulated.
  * Serial (the order of additions is fixed). */
-void orc_backproject_f64(const orc_geom *g, const orc_paras *s, int n_mat, int64_t view0,
-                         int64_t n_views, const double *gL, double *grad)
+void orc_backproject_focal_f64(const orc_geom *g, const orc_paras *s, int n_mat, int64_t view0,
+                               int64_t n_views, const double *focal, const double *gL, double *grad)
 {
     const int64_t nvox = g->nx * g->ny * g->nz;
     const int64_t npix = g->nu * g->nv;
@@ -666,7 +726,7 @@ void orc_backproject_f64(const orc_geom *g, const orc_paras *s, int n_mat, int64
         double cth = cos(th), sth = sin(th);
         for (int64_t v = 0; v < g->nv; v++) {
             for (int64_t u = 0; u < g->nu; u++) {
-                ray_f64 d = ray_dir_f64(g, s, u, v);
+                line_f64 d = ray_line_f64(g, s, u, v, focal ? focal + 2 * (view0 + iv) : NULL);
                 double gm[16];
                 int any = 0;
                 for (int m = 0; m < n_mat; m++) {
@@ -676,7 +736,7 @@ void orc_backproject_f64(const orc_geom *g, const orc_paras *s, int n_mat, int64
                 if (!any) continue;
                 for (int64_t k = 0; k < s->n_steps; k++) {
                     double t = t0 + (double)k * (double)s->dx;
-                    double X = d.cgcb * t - (double)g->sod, Y = d.cgsb * t, Z = d.sg * t;
+                    double X = d.dx * t + d.x0, Y = d.dy * t + d.y0, Z = d.dz * t + d.z0;
                     double a = (X * cth - Y * sth + s->half_x) / s->px;
                     double b = (-(X * sth + Y * cth) + s->half_y) / s->py;
                     double c = (-Z + s->half_z) / s->pz;
@@ -701,6 +761,12 @@ void orc_backproject_f64(const orc_geom *g, const orc_paras *s, int n_mat, int64
             }
         }
     }
+}
+
+void orc_backproject_f64(const orc_geom *g, const orc_paras *s, int n_mat, int64_t view0,
+                         int64_t n_views, const double *gL, double *grad)
+{
+    orc_backproject_focal_f64(g, s, n_mat, view0, n_views, NULL, gL, grad);
 }
 
 /* Chain rule of the spectrum, per ray: with tau_e = sum_m atten[e][m] L_m,

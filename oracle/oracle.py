@@ -68,6 +68,9 @@ def lib():
             "orc_postprocess": (None, [i64, i64, _F, _F, _F, vp, vp, _F]),
             "orc_num_threads": (i32, []),
             "orc_backproject_f64": (None, [G, P, i32, i64, i64, _D, _D]),
+            "orc_march_focal_f64": (None, [G, P, i32, _F, i64, i64, _D, _D, i32]),
+            "orc_backproject_focal_f64": (None, [G, P, i32, i64, i64, _D, _D, _D]),
+            "orc_focal_offsets": (None, [i64, C.c_float, C.c_uint32, _F, _F]),
             "orc_spectrum_grad_f64": (None, [i64, i64, i64, i32, i32, _D, _F, _F, C.c_double, _F, _F,
                                              _D, _D, _D]),
         }
@@ -119,13 +122,29 @@ def march_f32(g, p, vols, view0=0, n_views=None, threads=0):
     return out
 
 
-def march_f64(g, p, vols, view0=0, n_views=None, threads=0):
+def march_f64(g, p, vols, view0=0, n_views=None, threads=0, focal=None):
+    """FP64 line integrals [views][M][nv][nu]; focal: per-view source offsets
+    [all views][2] in mm (detector u and v directions) or None."""
     vols = f32(np.stack([np.asarray(v) for v in vols]))
     n_mat = vols.shape[0]
     n_views = g.n_views - view0 if n_views is None else n_views
     out = np.empty((n_views, n_mat, g.nv, g.nu), dtype=np.float64)
-    lib().orc_march_f64(C.byref(g), C.byref(p), n_mat, _f(vols), view0, n_views, _d(out), threads)
+    if focal is None:
+        lib().orc_march_f64(C.byref(g), C.byref(p), n_mat, _f(vols), view0, n_views, _d(out), threads)
+    else:
+        fo = np.ascontiguousarray(focal, dtype=np.float64)
+        lib().orc_march_focal_f64(C.byref(g), C.byref(p), n_mat, _f(vols), view0, n_views, _d(fo),
+                                  _d(out), threads)
     return out
+
+
+def focal_offsets(n_views, spot_mm, seed=10):
+    """Per-view focal-spot offsets (u, v) in mm: the reference's generator
+    (CBCTForwardProj.cpp:122-148) with the disc test u^2 + v^2 <= r^2."""
+    ou = np.empty(n_views, np.float32)
+    ov = np.empty(n_views, np.float32)
+    lib().orc_focal_offsets(n_views, spot_mm, seed, _f(ou), _f(ov))
+    return np.stack([ou, ov], axis=1)
 
 
 def spectrum_f32(L, atten, spectrum, i0val, grid=None, det=None, threads=0):
@@ -260,12 +279,17 @@ def num_threads():
     return lib().orc_num_threads()
 
 
-def backproject_f64(g, p, gL, view0=0):
+def backproject_f64(g, p, gL, view0=0, focal=None):
     """Adjoint of march_f64: gL [views][M][nv][nu] -> grad [M][nz][ny][nx]."""
     gL = np.ascontiguousarray(gL, dtype=np.float64)
     n_views, n_mat = gL.shape[:2]
     grad = np.zeros((n_mat, g.nz, g.ny, g.nx), dtype=np.float64)
-    lib().orc_backproject_f64(C.byref(g), C.byref(p), n_mat, view0, n_views, _d(gL), _d(grad))
+    if focal is None:
+        lib().orc_backproject_f64(C.byref(g), C.byref(p), n_mat, view0, n_views, _d(gL), _d(grad))
+    else:
+        fo = np.ascontiguousarray(focal, dtype=np.float64)
+        lib().orc_backproject_focal_f64(C.byref(g), C.byref(p), n_mat, view0, n_views, _d(fo), _d(gL),
+                                        _d(grad))
     return grad
 
 

@@ -179,3 +179,30 @@ def test_adjoint_lds_equals_global_atomics(require_gpu):
             proj.set_tuning(march=march, chunk=chunk, lds_bytes=lds)
             got = proj.backward(gI, None).cpu().numpy()
             assert rel_err(got, ref) < 1e-4, (march, chunk, lds)
+
+
+@pytest.mark.parametrize("mode", ["gather", "scatter"])
+def test_adjoint_with_focal_spot(require_gpu, mode):
+    import torch
+    s, vols = setup_case(n=36, views=5, det=(64, 48))
+    kw = P.spectral_inputs(s, *s.det)
+    focal = P.focal_spot_offsets(s.views, 1.5, seed=10).astype(np.float64)
+    rng = np.random.default_rng(12)
+    gI = rng.normal(size=(s.views, s.det[1], s.det[0])).astype(np.float32)
+    with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
+        proj.set_volumes(vols)
+        proj.set_spectral(**kw)
+        proj.set_focal_spot(focal)
+        proj.set_tuning(adjoint=mode)
+        grad = proj.backward(torch.from_numpy(gI).cuda(), None)
+        torch.cuda.synchronize()
+        grad = grad.cpu().numpy()
+    g = O.geom_from_setup(s)
+    p = O.paras(g)
+    L64 = O.march_f64(g, p, vols, focal=focal)
+    gL = O.spectrum_grad_f64(L64, kw["atten"], kw["spectrum"], kw["i0_val"], gI, None, kw["grid"],
+                             det_table(kw))
+    ref = O.backproject_f64(g, p, gL, focal=focal)
+    scale = O.backproject_f64(g, p, np.abs(gL), focal=focal)
+    err = np.abs(grad - ref) / np.maximum(scale, 1e-3 * scale.max())
+    assert err.max() < 1e-4, err.max()

@@ -467,3 +467,38 @@ def test_count_samples_matches_window_restatement(require_gpu):
         # FP64 on both sides; a window end exactly on an integer could round
         # differently, one sample per such ray at most
         assert abs(got - ref) <= max(2, ref // 100000), (got, ref)
+
+
+def test_focal_spot_parity(require_gpu):
+    """Per-view source offsets (a 1.2 mm spot): line integrals and the full
+    spectrum against the FP64 oracle, and LDS staging against the global
+    gather bit for bit."""
+    import torch
+    s, vols = small_three_material(n=48, views=6, det=(80, 60), d_size=0.9)
+    kw = P.spectral_inputs(s, *s.det)
+    focal = P.focal_spot_offsets(s.views, 1.2, seed=10)
+    with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
+        proj.set_volumes(vols)
+        proj.set_spectral(**kw)
+        proj.set_focal_spot(focal)
+        I, IA, L = proj.forward(want_L=True)
+        I0 = proj.flat_field()
+        proj.set_tuning(march="global")
+        Ig, IAg, Lg = proj.forward(want_L=True)
+        torch.cuda.synchronize()
+        assert torch.equal(I, Ig) and torch.equal(L, Lg)
+        proj.set_focal_spot(None)
+        _, _, Lnom = proj.forward(want_L=True)
+        torch.cuda.synchronize()
+    g = O.geom_from_setup(s)
+    p = O.paras(g)
+    L64 = O.march_f64(g, p, vols, focal=focal.astype(np.float64))
+    det = np.stack([O.det_response(kw["det_factor"], float(m), kw["scint_thickness"])
+                    for m in kw["scint_mu"]])
+    I64, IA64, I064 = O.spectrum_f64(L64, kw["atten"], kw["spectrum"], kw["i0_val"], kw["grid"], det)
+    L, I, I0 = to_np(L), to_np(I), to_np(I0)
+    assert rel_err(L, L64) < TOL
+    assert rel_err(I, I64) < TOL
+    assert rel_err(np.log(I0[None] / I), np.log(I064[None] / I64), 1e-2) < TOL
+    # the offsets matter: the point-source projection is far from this one
+    assert rel_err(to_np(Lnom), L64) > 100 * TOL

@@ -399,3 +399,93 @@ def test_spectrum_grad_matches_central_differences():
         Lm[idx] -= h
         fd = (loss(Lp) - loss(Lm)) / (2 * h)
         assert gL[idx] == pytest.approx(fd, rel=1e-6)
+
+
+# ------------------------------------------------------------- focal spot
+
+def _focal_line(g, p, th, u, v, su, sv):
+    """Object-frame line of ray (u, v) from a source displaced by (su, sv) mm:
+    it passes through the point where the nominal ray meets the detector
+    plane x = SDD - SOD (source frame)."""
+    d = np.asarray(O.ray_direction_f64(g, p, u, v))
+    S = np.array([-g.sod, 0.0, 0.0])
+    Q = S + (g.sdd / d[0]) * d
+    Sp = np.array([-g.sod, su, sv])
+    e = (Q - Sp) / np.linalg.norm(Q - Sp)
+    c, s = math.cos(th), math.sin(th)
+    to_obj = lambda X: np.array([X[0] * c - X[1] * s, -(X[0] * s + X[1] * c), -X[2]])
+    return to_obj(Sp), to_obj(e)
+
+
+def test_focal_spot_chords():
+    """A uniform sphere on the rotation axis seen from displaced sources:
+    each line integral is the sphere's chord along the line from the
+    displaced source through the detector point of the nominal ray.  The
+    offsets move rays ~3 voxels at the sphere, so the nominal chords would
+    fail the same check."""
+    n, r = 64, 20.0
+    g = O.geom(n, n, n, 64.0, 64.0, 64.0, 2, 35, 25, 300.0, 450.0, 4.0)
+    p = O.paras(g)
+    vol = _sphere(n, r, (32.0, 32.0, 32.0), rho=1.5)
+    focal = np.array([[9.0, -6.0], [-7.5, 4.5]])
+    L = O.march_f64(g, p, [vol], focal=focal)[:, 0]
+    L0 = O.march_f64(g, p, [vol])[:, 0]
+    discriminating = 0
+    for view in range(2):
+        th = O.view_angle(g, p, view)
+        for v in range(g.nv):
+            for u in range(g.nu):
+                p0, e = _focal_line(g, p, th, u, v, *focal[view])
+                b = np.linalg.norm(np.cross(-p0, e))
+                chord = 2 * math.sqrt(max(r * r - b * b, 0.0)) * 1.5
+                if b < r - 2.0:
+                    assert abs(L[view, v, u] - chord) < 1.6, (view, u, v)
+                    discriminating += abs(L0[view, v, u] - chord) > 4.0
+                elif b > r + 1.8:
+                    assert L[view, v, u] == 0.0
+    assert discriminating > 50
+
+
+def test_focal_spot_zero_offset_and_symmetry():
+    n = 40
+    g = O.geom(n, n, n, 40.0, 40.0, 40.0, 2, 30, 22, 300.0, 450.0, 2.0)
+    p = O.paras(g)
+    rng = np.random.default_rng(3)
+    vol = rng.uniform(0.5, 1.5, (n, n, n)).astype(np.float32)
+    L0 = O.march_f64(g, p, [vol])
+    Lz = O.march_f64(g, p, [vol], focal=np.zeros((2, 2)))
+    assert np.abs(Lz - L0).max() <= 1e-11 * np.abs(L0).max()
+    # a centro-symmetric volume: pixel (u, v) from offset f equals pixel
+    # (NU-1-u, NV-1-v) from offset -f
+    # voxel i sits at i * p - len / 2: index 20 is the rotation centre
+    sym = _sphere(n, 12.0, (20.0, 20.0, 20.0), rho=1.0) + _sphere(n, 5.0, (20.0, 20.0, 20.0), rho=0.5)
+    f = np.array([[1.3, -0.7], [0.4, 0.9]])
+    Lp = O.march_f64(g, p, [sym], focal=f)[:, 0]
+    Lm = O.march_f64(g, p, [sym], focal=-f)[:, 0]
+    # (the FP32 geometry scalars of the reference leave ~1e-7 of asymmetry)
+    assert np.abs(Lp - Lm[:, ::-1, ::-1]).max() <= 1e-6 * np.abs(Lp).max()
+
+
+def test_focal_backprojection_is_the_adjoint():
+    g = small_geom(n=20, nu=14, nv=10, views=2)
+    p = O.paras(g)
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=(2, 20, 20, 20)).astype(np.float32)
+    y = rng.normal(size=(2, 2, 10, 14))
+    focal = np.array([[0.8, -0.3], [-0.5, 0.6]])
+    Ax = O.march_f64(g, p, list(x), focal=focal)
+    ATy = O.backproject_f64(g, p, y, focal=focal)
+    lhs = float((Ax * y).sum())
+    rhs = float((x.astype(np.float64) * ATy).sum())
+    assert lhs == pytest.approx(rhs, rel=1e-10)
+
+
+def test_focal_offsets_in_disc_and_quantised():
+    f = O.focal_offsets(2000, 0.4, seed=10)
+    r = 0.2
+    assert (f[:, 0] ** 2 + f[:, 1] ** 2 <= r * r + 1e-7).all()
+    steps = (f.astype(np.float64) + r) * 1e4
+    assert np.abs(steps - np.round(steps)).max() < 2e-3
+    # roughly uniform over the disc: mean near 0, mean radius^2 near r^2 / 2
+    assert abs(f.mean()) < 0.01
+    assert (f ** 2).sum(axis=1).mean() == pytest.approx(r * r / 2, rel=0.1)
