@@ -344,7 +344,20 @@ std::vector<float> CBCTForwardProj::gridTable()
     return T;
 }
 
-void CBCTForwardProj::runScan(bool singleMaterial, bool withGrid, bool withResponse)
+// computeFoSpOffset (CBCTForwardProj.cpp:122-148): per-view focal-spot
+// offsets within a disc of diameter focalSpotSize (mm), from MSVC rand()
+// after srand(10) as in the reference, but kept inside the disc.
+void CBCTForwardProj::computeFoSpOffset()
+{
+    const size_t n = mCTScanParas.projNum;
+    vFoSpU.assign(n, 0.0f);
+    vFoSpV.assign(n, 0.0f);
+    check(ctfp_focal_spot_offsets((int64_t)n, mCTScanParas.focalSpotSize, 10u, vFoSpU.data(), vFoSpV.data()));
+    h_mForwardProj.foSpOffsetU = vFoSpU.data();
+    h_mForwardProj.foSpOffsetV = vFoSpV.data();
+}
+
+void CBCTForwardProj::runScan(bool singleMaterial, bool withGrid, bool withResponse, bool focalSpot)
 {
     hasGrid_ = withGrid;
     hasResponse_ = withResponse;
@@ -403,6 +416,10 @@ void CBCTForwardProj::runScan(bool singleMaterial, bool withGrid, bool withRespo
         sp.det_factor = mCTScanParas.mScintilltorInfo.detResponseFactor;
     }
     check(ctfp_set_spectral(ctx.c, &sp));
+    if (focalSpot) {
+        computeFoSpOffset();
+        check(ctfp_set_focal_spot(ctx.c, h_mForwardProj.foSpOffsetU, h_mForwardProj.foSpOffsetV));
+    }
     if (verbose) std::cout << "projecting " << mCTScanParas.projNum << " views, " << E << " energy bins\n";
     check(ctfp_forward_host(ctx.c, I, IAbsorb, h_mForwardProj.I0));
 

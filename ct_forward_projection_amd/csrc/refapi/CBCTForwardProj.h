@@ -35,6 +35,7 @@ class CBCTForwardProj {
 
     virtual void computePerScinThinckness();
     virtual void updateDetResponse();
+    virtual void computeFoSpOffset();
 
     virtual void readSpecrtumNorm();
     virtual void readGridMassAttu();
@@ -79,7 +80,7 @@ class CBCTForwardProj {
 
   protected:
     // Runs one scan through the C ABI and writes the reference's output files.
-    void runScan(bool singleMaterial, bool withGrid, bool withResponse);
+    void runScan(bool singleMaterial, bool withGrid, bool withResponse, bool focalSpot = false);
     void runTestI0(bool withGrid);
     void allocHost(bool withGrid);
     std::vector<float> gridTable() ; // T[e][u] for every bin
@@ -89,6 +90,6 @@ class CBCTForwardProj {
 
     // storage behind the public raw pointers
     std::vector<float> vI, vIA, vI0, vProj, vSpec, vGridT, vGridMu, vInterMu, vScinMu, vThick,
-        vDet, vGridRow, vADet;
+        vDet, vGridRow, vADet, vFoSpU, vFoSpV;
     std::vector<uint32> vIu, vIAu;
 };

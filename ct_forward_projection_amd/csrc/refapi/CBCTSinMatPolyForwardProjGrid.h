@@ -13,6 +13,9 @@ class CBCTSinMatPolyForwardProjGrid : private CBCTPolyForwardProjGrid {
 
     virtual void computePolyForwProj();
     void computePolyForwProjNoResponse();
+    // The reference's focal-spot variant (unreachable from its run() and
+    // broken there): a point source replaced by per-view source offsets.
+    void computePolyForwProjFoSp();
 
     using CBCTForwardProj::I;
     using CBCTForwardProj::IAbsorb;

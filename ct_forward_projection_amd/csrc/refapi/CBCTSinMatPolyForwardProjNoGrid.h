@@ -12,6 +12,9 @@ class CBCTSinMatPolyForwardProjNoGrid : private CBCTPolyForwardProjNoGrid {
 
     virtual void computePolyForwProj();
     void computePolyForwProjNoResponse();
+    // The reference's focal-spot variant (unreachable from its run() and
+    // broken there): a point source replaced by per-view source offsets.
+    void computePolyForwProjFoSp();
 
     using CBCTForwardProj::I;
     using CBCTForwardProj::IAbsorb;

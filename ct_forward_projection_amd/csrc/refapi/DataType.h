@@ -32,7 +32,7 @@ struct CTScanParas {
     int rotatedDirection = 1;
     float dSize = 0; // mm
     size_t I0Val = 0;
-    float focalSpotSize = 0; // mm
+    float focalSpotSize = 0.4f; // mm; the reference fixes 0.4 (ForwardProjection.cpp:444)
     int spectrumStep = 1;
     int specEnergyNum = 0;
     ScintilltorInfo mScintilltorInfo;

@@ -60,6 +60,14 @@ void ForwardProjection::_getCTScanConfig()
     F.GetEntryValue(sec, "SOD", -100.0f, mCTScanParas.sod);
     bool okV = F.GetEntryValue(sec, "ScanAngleNum", (size_t)0, mCTScanParas.projNum);
     F.GetEntryValue(sec, "RotatedDirection", -100, mCTScanParas.rotatedDirection);
+    // optional, not read by the reference (which fixes 0.4 mm): used only by
+    // the focal-spot projections (computePolyForwProjFoSp)
+    float spot = -100.0f;
+    F.GetEntryValue(sec, "FocalSpotSize", -100.0f, spot);
+    if (spot != -100.0f) {
+        if (!(spot > 0.0f) || spot > 100.0f) bad("[ScanPara] FocalSpotSize must be in (0, 100] mm");
+        mCTScanParas.focalSpotSize = spot;
+    }
     for (int flag : {_isTestI0, _isDetResponse, _isGrid, _isSingleMaterial})
         if (flag != 0 && flag != 1) bad("[ScanPara] IsTestI0/IsDetResponse/IsGrid/IsSingleMaterial must be 0 or 1");
     if (!(mCTScanParas.sdd > 0) || !(mCTScanParas.sod > 0) || !(mCTScanParas.sdd > mCTScanParas.sod))
