@@ -332,6 +332,9 @@ __device__ __forceinline__ void load_vox(__amdgpu_buffer_rsrc_t r, uint32_t voff
 // Support interval of index s + k*d inside (-1, n): intersect into [lo, hi].
 __device__ __forceinline__ void clip_axis(double s, double d, double n, double &lo, double &hi)
 {
+    // no fused multiply-adds beyond the explicit ones: every kernel that
+    // inlines this evaluates the same operations
+#pragma clang fp contract(off)
     if (fabs(d) < 1e-300) {
         if (!(s > -1.0 && s < n)) { lo = 1.0; hi = 0.0; }
         return;
@@ -356,6 +359,7 @@ struct RayLine {
 };
 __device__ __forceinline__ RayLine ray_line(const MarchArgs &P, int u, int v, int vw)
 {
+#pragma clang fp contract(off)
     const double2 cs = P.view_cs[vw];
     // computeIntPointCoordinatesKernel (CBCTPolyProjGPU.cu:1813-1829), FP64.
     double detU = P.half_u - P.dsize * 0.5 - (double)u * P.dsize;
@@ -402,6 +406,7 @@ __device__ __forceinline__ RayLine ray_line(const MarchArgs &P, int u, int v, in
 __device__ __forceinline__ void project_point(const MarchArgs &P, int vw, double a, double b, double c,
                                               double &u, double &v)
 {
+#pragma clang fp contract(off)
     const double2 cs = P.view_cs[vw];
     const double qx = a / P.inv_px - P.half_x + P.sod * cs.x; // world, relative to the source
     const double qy = b / P.inv_py - P.half_y - P.sod * cs.y;
@@ -426,6 +431,7 @@ __device__ __forceinline__ void project_point(const MarchArgs &P, int vw, double
 
 __device__ __forceinline__ RaySetup setup_ray(const MarchArgs &P, int u, int v, int vw)
 {
+#pragma clang fp contract(off)
     const RayLine Ln = ray_line(P, u, v, vw);
     double sa = Ln.sa + Ln.da * P.t0, sb2 = Ln.sb + Ln.db * P.t0, sc = Ln.sc + Ln.dc * P.t0; // k = 0
     double da = Ln.da * P.dx, db = Ln.db * P.dx, dc = Ln.dc * P.dx;
@@ -520,6 +526,7 @@ struct Anchor {
 };
 __device__ __forceinline__ Anchor anchor_at(const RaySetup &R, int ka)
 {
+#pragma clang fp contract(off)
     const double j = (double)(ka - R.k_lo);
     const double pa = fma(j, R.da, R.a0), pb = fma(j, R.db, R.b0), pc = fma(j, R.dc, R.c0);
     const double ga = floor(pa), gb = floor(pb), gc = floor(pc);
@@ -1168,9 +1175,9 @@ struct BatchQueues {
     int32_t n_items;
 };
 
-__global__ void batch_reset(uint32_t *taken)
+__global__ void batch_reset(uint32_t *taken, int n)
 {
-    if (threadIdx.x < kXcds) taken[threadIdx.x] = 0u;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) taken[i] = 0u;
 }
 
 template <int ST>
@@ -2449,16 +2456,21 @@ int ctfp_autotune(ctfp_context *ctx, int max_lds_bytes, ctfp_tuning *best)
 } // extern "C"
 
 struct ctfp_batch {
+    // items of one LDS budget share a persistent launch (and its occupancy)
+    struct Group {
+        BatchItem *d_items = nullptr;
+        uint32_t *d_base = nullptr; // [kXcds][n + 1]
+        int n = 0;
+        size_t shmem = 0;
+        unsigned grid = 0;
+    };
     int device = 0;
     int dtype = CTFP_F32;
     int n_items = 0;
     std::vector<ctfp_batch_item> items;
     std::vector<uint64_t> gens; // generation of each context at creation (checked per run)
-    BatchItem *d_items = nullptr;
-    uint32_t *d_taken = nullptr;
-    uint32_t *d_base = nullptr;
-    size_t shmem = 0;
-    unsigned grid = 0;
+    std::vector<Group> groups;
+    uint32_t *d_taken = nullptr; // [groups][kXcds]
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -2472,9 +2484,11 @@ void batch_free(ctfp_batch *b)
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
-    dfree(b->d_items);
+    for (auto &g : b->groups) {
+        dfree(g.d_items);
+        dfree(g.d_base);
+    }
     dfree(b->d_taken);
-    dfree(b->d_base);
     delete b;
 }
 
@@ -2500,8 +2514,6 @@ int ctfp_batch_create(const ctfp_batch_item *items, int n_items, ctfp_batch **ou
     const ctfp_context *c0 = items[0].ctx;
     if (check_ctx(c0)) return CTFP_ERR_INVALID;
     std::vector<BatchItem> h(n_items);
-    std::vector<uint32_t> base((size_t)kXcds * (n_items + 1), 0u);
-    size_t shmem = 256;
     int64_t samples = 0;
     for (int i = 0; i < n_items; i++) {
         ctfp_context *c = items[i].ctx;
@@ -2516,57 +2528,83 @@ int ctfp_batch_create(const ctfp_batch_item *items, int n_items, ctfp_batch **ou
                               nullptr, true, h[i].P);
         if (rc) return rc;
         h[i].n_mat = c->n_mat;
-        shmem = std::max(shmem, (size_t)c->tune.lds_bytes);
         samples += items[i].view_count * c->scan.d_num_u * c->scan.d_num_v * c->sys.int_num;
-        const uint64_t tiles = (uint64_t)h[i].P.tiles_u * h[i].P.tiles_v;
-        for (int x = 0; x < kXcds; x++) {
-            const int64_t nv = items[i].view_count > x ? (items[i].view_count - x + kXcds - 1) / kXcds : 0;
-            const uint64_t next = base[(size_t)x * (n_items + 1) + i] + (uint64_t)nv * tiles;
-            if (next >= (1ull << 31)) return set_error(CTFP_ERR_INVALID, "batch: too many tiles");
-            base[(size_t)x * (n_items + 1) + i + 1] = (uint32_t)next;
-        }
     }
+    // groups by LDS budget, in order of first appearance
+    std::vector<int> budgets;
+    for (int i = 0; i < n_items; i++)
+        if (std::find(budgets.begin(), budgets.end(), items[i].ctx->tune.lds_bytes) == budgets.end())
+            budgets.push_back(items[i].ctx->tune.lds_bytes);
     DeviceGuard g(c0->device);
     BatchFn fn = pick_batch(c0->dtype);
-    int per_cu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, shmem));
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, c0->device));
-    if (per_cu < 1) return set_error(CTFP_ERR_UNSUPPORTED, "batch: kernel does not fit a CU");
     ctfp_batch *b = new ctfp_batch();
     b->device = c0->device;
     b->dtype = c0->dtype;
     b->n_items = n_items;
     b->items.assign(items, items + n_items);
     for (int i = 0; i < n_items; i++) b->gens.push_back(items[i].ctx->generation);
-    b->shmem = shmem;
-    b->grid = (unsigned)(per_cu * prop.multiProcessorCount);
-    b->grid = std::max(8u, b->grid / 8 * 8);
     b->samples = samples;
-    hipError_t e = hipMalloc(&b->d_items, h.size() * sizeof(BatchItem));
-    if (e == hipSuccess) e = hipMemcpy(b->d_items, h.data(), h.size() * sizeof(BatchItem), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc(&b->d_taken, kXcds * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&b->d_base, base.size() * sizeof(uint32_t));
-    if (e == hipSuccess)
-        e = hipMemcpy(b->d_base, base.data(), base.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
-    // the launch sequence (zero the queue counters, run the persistent
-    // kernel) is captured once on a private stream and replayed as a graph
+    hipError_t e = hipSuccess;
+    auto fail = [&](int code, const char *what) {
+        batch_free(b);
+        return set_error(code, "batch: %s", what);
+    };
+    for (int budget : budgets) {
+        std::vector<BatchItem> gi;
+        for (int i = 0; i < n_items; i++)
+            if (items[i].ctx->tune.lds_bytes == budget) gi.push_back(h[i]);
+        const int n = (int)gi.size();
+        std::vector<uint32_t> base((size_t)kXcds * (n + 1), 0u);
+        for (int i = 0; i < n; i++) {
+            const uint64_t tiles = (uint64_t)gi[i].P.tiles_u * gi[i].P.tiles_v;
+            const int64_t vc = gi[i].P.n_views;
+            for (int x = 0; x < kXcds; x++) {
+                const int64_t nv = vc > x ? (vc - x + kXcds - 1) / kXcds : 0;
+                const uint64_t next = base[(size_t)x * (n + 1) + i] + (uint64_t)nv * tiles;
+                if (next >= (1ull << 31)) return fail(CTFP_ERR_INVALID, "too many tiles");
+                base[(size_t)x * (n + 1) + i + 1] = (uint32_t)next;
+            }
+        }
+        ctfp_batch::Group grp;
+        grp.n = n;
+        grp.shmem = (size_t)std::max(budget, 256);
+        int per_cu = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, grp.shmem);
+        if (e != hipSuccess) return fail(CTFP_ERR_DEVICE, hipGetErrorString(e));
+        if (per_cu < 1) return fail(CTFP_ERR_UNSUPPORTED, "kernel does not fit a CU");
+        grp.grid = std::max(8u, (unsigned)(per_cu * prop.multiProcessorCount) / 8 * 8);
+        e = hipMalloc(&grp.d_items, gi.size() * sizeof(BatchItem));
+        if (e == hipSuccess) e = hipMemcpy(grp.d_items, gi.data(), gi.size() * sizeof(BatchItem), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc(&grp.d_base, base.size() * sizeof(uint32_t));
+        if (e == hipSuccess)
+            e = hipMemcpy(grp.d_base, base.data(), base.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+        b->groups.push_back(grp);
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? CTFP_ERR_NOMEM : CTFP_ERR_DEVICE,
+                                         hipGetErrorString(e));
+    }
+    const int ng = (int)b->groups.size();
+    e = hipMalloc(&b->d_taken, (size_t)ng * kXcds * sizeof(uint32_t));
+    // the launch sequence (zero every queue counter, then one persistent
+    // kernel per group) is captured once on a private stream and replayed
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        batch_reset<<<1, 64, 0, b->cap_stream>>>(b->d_taken);
-        BatchQueues Q{b->d_taken, b->d_base, n_items};
-        hipLaunchKernelGGL(fn, dim3(b->grid), dim3(kBlock), shmem, b->cap_stream, b->d_items, Q);
+        batch_reset<<<1, 256, 0, b->cap_stream>>>(b->d_taken, ng * kXcds);
         hipError_t launch = hipGetLastError();
+        for (int k = 0; k < ng && launch == hipSuccess; k++) {
+            const ctfp_batch::Group &grp = b->groups[k];
+            BatchQueues Q{b->d_taken + k * kXcds, grp.d_base, grp.n};
+            hipLaunchKernelGGL(fn, dim3(grp.grid), dim3(kBlock), grp.shmem, b->cap_stream, grp.d_items, Q);
+            launch = hipGetLastError();
+        }
         e = hipStreamEndCapture(b->cap_stream, &b->graph);
         if (e == hipSuccess) e = launch;
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-        batch_free(b);
-        return set_error(e == hipErrorOutOfMemory ? CTFP_ERR_NOMEM : CTFP_ERR_DEVICE, "batch: %s",
-                         hipGetErrorString(e));
-    }
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? CTFP_ERR_NOMEM : CTFP_ERR_DEVICE, hipGetErrorString(e));
     *out = b;
     return CTFP_OK;
 }

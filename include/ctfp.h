@@ -242,8 +242,7 @@ int ctfp_count_samples(ctfp_context *ctx, int64_t view_begin, int64_t view_count
 /* Times candidate launch options (ray-tile shape, samples per staged chunk,
  * LDS budget) on two groups of the scan's views a quarter turn apart, keeps
  * the fastest in the context and returns it in `best` (may be NULL).
- * max_lds_bytes > 0 caps the LDS budget: a batch runs all its items with the
- * largest budget among them, so scans meant for one batch share a cap.
+ * max_lds_bytes > 0 caps the LDS budget (0: no cap).
  * Needs volumes and spectral tables; results are bit-identical for every
  * candidate, so tuning only changes speed.  Synchronous; allocates scratch
  * output for the timed views. */
@@ -252,7 +251,8 @@ int ctfp_autotune(ctfp_context *ctx, int max_lds_bytes, ctfp_tuning *best);
 /* Several scans in one persistent launch.  Each item is a context (its own
  * volume, detector, views and spectral tables, LDS march mode) and a view
  * range with device outputs I, IAbsorb [view_count][d_num_v][d_num_u].  All
- * items must share one device and one storage type.  The launch sequence is
+ * items must share one device and one storage type.  Items with the same LDS
+ * budget share one persistent launch; the launches (one per budget) are
  * captured once into a HIP graph at creation and replayed by ctfp_batch_run.
  * The contexts must outlive the batch, and their volumes and spectral tables
  * must not be replaced while it exists (ctfp_batch_run checks the volumes). */
