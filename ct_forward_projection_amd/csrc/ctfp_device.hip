@@ -97,7 +97,7 @@ struct ctfp_context {
     float *dI = nullptr, *dIA = nullptr;
     int64_t staged_views = 0;
     // launch tuning
-    ctfp_tuning tune{CTFP_MARCH_LDS, 16, 40 * 1024, 8, 2, CTFP_ADJOINT_GATHER};
+    ctfp_tuning tune{CTFP_MARCH_LDS, 16, 40 * 1024, 16, 1, CTFP_ADJOINT_GATHER};
     // dl/dL workspace of the gather adjoint
     float *adj_ws = nullptr;
     size_t adj_ws_bytes = 0;
@@ -597,6 +597,23 @@ __device__ __forceinline__ void sample_box(const RaySetup &R, int j0, int j1, in
     }
 }
 
+// Cells that samples i0 .. i1-1 after anchor A use, [lo, hi] with the +1
+// corner included, merged into lo / hi.  A sample's position is one rounding
+// of an affine function of its index (offset_cell), so positions, and their
+// cells, are monotone in the index: the two end samples bound the rest
+// exactly.
+__device__ __forceinline__ void segment_box(const Anchor &A, int i0, int i1, int (&lo)[3], int (&hi)[3])
+{
+    const float f0 = (float)i0, f1 = (float)(i1 - 1);
+    const Cell a = offset_cell(A, f0), b = offset_cell(A, f1);
+    lo[0] = min(lo[0], A.ia + min(a.x, b.x));
+    lo[1] = min(lo[1], A.ib + min(a.y, b.y));
+    lo[2] = min(lo[2], A.ic + min(a.z, b.z));
+    hi[0] = max(hi[0], A.ia + max(a.x, b.x) + 1);
+    hi[1] = max(hi[1], A.ib + max(a.y, b.y) + 1);
+    hi[2] = max(hi[2], A.ic + max(a.z, b.z) + 1);
+}
+
 // Trilinear accumulation of one sample into the M per-material sums from
 // 8 corners (FP32 voxels) or 4 rows of x-pairs (16-bit).  `fetch(row, out)`
 // returns the M dwords at corner row `row` (0: y0z0, 1: y1z0, 2: y0z1,
@@ -605,12 +622,11 @@ template <int M, int ST, typename Fetch>
 __device__ __forceinline__ void accumulate(const Cell &c, Fetch fetch, float (&acc)[M])
 {
     const float bx = 1.0f - c.ax, by = 1.0f - c.ay, bz = 1.0f - c.az;
-    const float w00 = by * bz, w10 = c.ay * bz, w01 = by * c.az, w11 = c.ay * c.az;
     if constexpr (ST == kStF32) {
         uint32_t d[8][M];
 #pragma unroll
         for (int r = 0; r < 8; r++) fetch(r, d[r]);
-        const float wr[4] = {w00, w10, w01, w11};
+        const float wr[4] = {by * bz, c.ay * bz, by * c.az, c.ay * c.az};
 #pragma unroll
         for (int m = 0; m < M; m++) {
             float a = acc[m];
@@ -625,6 +641,7 @@ __device__ __forceinline__ void accumulate(const Cell &c, Fetch fetch, float (&a
         uint32_t d[4][M];
 #pragma unroll
         for (int r = 0; r < 4; r++) fetch(r, d[r]);
+        const float w00 = by * bz, w10 = c.ay * bz, w01 = by * c.az, w11 = c.ay * c.az;
         const uint32_t p[4] = {pack_w<ST>(bx * w00, c.ax * w00), pack_w<ST>(bx * w10, c.ax * w10),
                                pack_w<ST>(bx * w01, c.ax * w01), pack_w<ST>(bx * w11, c.ax * w11)};
 #pragma unroll
@@ -943,6 +960,45 @@ __device__ __forceinline__ void block_box(int (&lo)[3], int (&hi)[3], int *scrat
     }
 }
 
+// Block-wide min of lo[] and max of hi[] with one barrier.  The scratch holds
+// two buffers (ints [24 p, 24 p + 24), p = parity) that calls use in turn: a
+// buffer is written again only two calls later, when every wave has passed
+// the barrier of the call in between and so has read it.  The caller must
+// not touch LDS that another wave may still read from before this barrier.
+template <int D>
+__device__ __forceinline__ void block_range(int (&lo)[D], int (&hi)[D], int *scratch, int parity)
+{
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+#pragma unroll
+        for (int s = 32; s >= 1; s >>= 1) {
+            lo[d] = min(lo[d], __shfl_xor(lo[d], s));
+            hi[d] = max(hi[d], __shfl_xor(hi[d], s));
+        }
+    }
+    int *buf = scratch + parity * 24;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            buf[wave * 6 + d] = lo[d];
+            buf[wave * 6 + 3 + d] = hi[d];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        int a = buf[d], b = buf[3 + d];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; w++) {
+            a = min(a, buf[w * 6 + d]);
+            b = max(b, buf[w * 6 + 3 + d]);
+        }
+        lo[d] = a;
+        hi[d] = b;
+    }
+}
+
 // Exact q = n / d for 0 <= n < 2^22, 1 <= d < 2^10, from a float reciprocal.
 __device__ __forceinline__ int small_div(int n, int d, float inv_d)
 {
@@ -954,6 +1010,9 @@ __device__ __forceinline__ int small_div(int n, int d, float inv_d)
 
 // 1 / d to within an ulp (v_rcp_f32); small_div corrects the quotient.
 __device__ __forceinline__ float small_rcp(int d) { return __builtin_amdgcn_rcpf((float)d); }
+
+// Rows a thread loads at once in fill_box.
+constexpr int kFillBatch = 4;
 
 // Cooperative, row-coalesced load of the cell box [lo, lo + (sx, sy, sz)) into
 // LDS (cell i of the box at byte i * LB, x fastest).  Cells outside the
@@ -967,10 +1026,10 @@ __device__ __forceinline__ void fill_box(__amdgpu_buffer_rsrc_t rsrc, char *lds,
     constexpr uint32_t LB = lds_cell_bytes<M>();
     const int rows = sy * sz;
     if (sx <= kBlock) {
-        const int per = kBlock / sx; // rows per pass
+        const float inv_sx = small_rcp(sx), inv_sy = small_rcp(sy);
+        const int per = small_div(kBlock, sx, inv_sx); // rows per pass
         const int tid = threadIdx.x;
         if (tid >= per * sx) return;
-        const float inv_sx = small_rcp(sx), inv_sy = small_rcp(sy);
         const int r0 = small_div(tid, sx, inv_sx);
         const int x = lo[0] + (tid - r0 * sx);
         int lz = small_div(r0, sy, inv_sy);
@@ -981,22 +1040,27 @@ __device__ __forceinline__ void fill_box(__amdgpu_buffer_rsrc_t rsrc, char *lds,
         const uint32_t ymax = (uint32_t)(ny + kVoxPadLo + kVoxPadHi), zmax = (uint32_t)(nz + kVoxPadLo + kVoxPadHi);
         uint32_t loff = (uint32_t)tid * LB;
         const uint32_t lstep = (uint32_t)(per * sx) * LB;
-        for (int r = r0; r < rows; r += per) {
-            const uint32_t yi = (uint32_t)(lo[1] + ly + kVoxPadLo), zi = (uint32_t)(lo[2] + lz + kVoxPadLo);
-            uint32_t d[M];
-            if (xin && yi < ymax && zi < zmax) {
-                load_raw<M>(rsrc, __umul24(zi, oz) + (__umul24(yi, oy) + xoff), 0, d);
-            } else {
+        // kFillBatch rows per pass of the loop: their loads are all in flight
+        // before the first LDS write.  A row outside the padded volume (or
+        // past the box) loads the padded cell at offset 0, which is a zero.
+        for (int r = r0; r < rows; r += kFillBatch * per) {
+            uint32_t d[kFillBatch][M];
 #pragma unroll
-                for (int m = 0; m < M; m++) d[m] = 0u;
+            for (int b = 0; b < kFillBatch; b++) {
+                const uint32_t yi = (uint32_t)(lo[1] + ly + kVoxPadLo), zi = (uint32_t)(lo[2] + lz + kVoxPadLo);
+                const bool in = xin && yi < ymax && zi < zmax;
+                load_raw<M>(rsrc, in ? __umul24(zi, oz) + (__umul24(yi, oy) + xoff) : 0u, 0, d[b]);
+                ly += dy;
+                lz += dz;
+                if (ly >= sy) {
+                    ly -= sy;
+                    lz++;
+                }
             }
-            lds_write<M>(lds, loff, d);
-            loff += lstep;
-            ly += dy;
-            lz += dz;
-            if (ly >= sy) {
-                ly -= sy;
-                lz++;
+#pragma unroll
+            for (int b = 0; b < kFillBatch; b++) {
+                if (r + b * per < rows) lds_write<M>(lds, loff, d[b]);
+                loff += lstep;
             }
         }
     } else {
@@ -1038,17 +1102,28 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
     const uint32_t oy = P.row_y & 0xFFFFFFu, oz = P.row_z & 0xFFFFFFu;
     const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
     const int k_hi = R.k_lo + R.n;
-    int kr_lo[3] = {R.n > 0 ? R.k_lo : INT_MAX, 0, 0}, kr_hi[3] = {R.n > 0 ? k_hi : INT_MIN, 0, 0};
-    __syncthreads();
-    block_box(kr_lo, kr_hi, scratch);
+    int kr_lo[1] = {R.n > 0 ? R.k_lo : INT_MAX}, kr_hi[1] = {R.n > 0 ? k_hi : INT_MIN};
+    __syncthreads(); // the block's previous user of the LDS has finished
+    block_range<1>(kr_lo, kr_hi, scratch, 0);
     const int kb_hi = kr_hi[0];
+    int parity = 1;
     // chunks start on the anchor grid (chunk is a multiple of kAnchor)
-    for (int k0 = kr_lo[0] & ~(kAnchor - 1); k0 < kb_hi; k0 += P.chunk) {
+    for (int k0 = kr_lo[0] & ~(kAnchor - 1); k0 < kb_hi; k0 += P.chunk, parity ^= 1) {
         const int c_lo = max(k0, R.k_lo), c_hi = min(k0 + P.chunk, k_hi);
+        // the chunk's cell box from the lane's FP32 sample positions; the
+        // first anchor is kept for the march
         int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-        if (c_hi > c_lo) sample_box(R, c_lo - R.k_lo, c_hi - R.k_lo, lo, hi);
-        __syncthreads(); // scratch and staged cells of the previous chunk are free
-        block_box(lo, hi, scratch);
+        const Anchor A0 = anchor_at(R, k0);
+        {
+            const int i0 = max(k0, c_lo) - k0, i1 = min(k0 + kAnchor, c_hi) - k0;
+            if (i0 < i1) segment_box(A0, i0, i1, lo, hi);
+        }
+        for (int ka = k0 + kAnchor; ka < k0 + P.chunk; ka += kAnchor) {
+            const int i0 = max(ka, c_lo) - ka, i1 = min(ka + kAnchor, c_hi) - ka;
+            if (i0 < i1) segment_box(anchor_at(R, ka), i0, i1, lo, hi);
+        }
+        // every wave has finished the previous chunk past this barrier
+        block_range<3>(lo, hi, scratch, parity);
         if (hi[0] < lo[0]) continue; // no lane of the block samples this chunk (block-uniform)
         const int sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
         const bool staged = (int64_t)sx * sy * sz <= (int64_t)P.lds_cells;
@@ -1061,7 +1136,7 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
                 const int i1 = min(ka + kAnchor, c_hi) - ka;
                 int i = max(ka, c_lo) - ka;
                 if (i >= i1) continue;
-                const Anchor A = anchor_at(R, ka);
+                const Anchor A = ka == k0 ? A0 : anchor_at(R, ka);
                 // byte offset of the anchor's cell in the box; samples add
                 // their (small, signed) cell offsets with 24-bit multiplies
                 const char *pa =
@@ -1085,7 +1160,7 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
                 const int i1 = min(ka + kAnchor, c_hi) - ka;
                 int i = max(ka, c_lo) - ka;
                 if (i >= i1) continue;
-                const Anchor A = anchor_at(R, ka);
+                const Anchor A = ka == k0 ? A0 : anchor_at(R, ka);
                 for (float fi = (float)i; i < i1; i++, fi += 1.0f)
                     accumulate_global<M, ST>(rsrc, absolute_cell(A, offset_cell(A, fi), nx, ny, nz), oy,
                                              oz, acc);
@@ -1184,7 +1259,7 @@ template <int ST>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void batch_kernel(const BatchItem *__restrict__ items, BatchQueues Q)
 {
     extern __shared__ char lds_raw[];
-    uint32_t *slot = (uint32_t *)lds_raw + 60; // beside the block_box scratch (ints 0..31)
+    uint32_t *slot = (uint32_t *)lds_raw + 60; // beside the block_range scratch (ints 0..47)
     const uint32_t home = blockIdx.x & (kXcds - 1);
     const int stride = Q.n_items + 1;
     for (uint32_t qi = 0; qi < (uint32_t)kXcds; qi++) {
