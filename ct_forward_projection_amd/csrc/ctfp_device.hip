@@ -499,14 +499,17 @@ __device__ __forceinline__ float dot2_acc(uint32_t w, uint32_t d, float acc)
                                                __builtin_bit_cast(bf16x2_t, d), acc, false);
 }
 
-// Round-to-nearest pack of two FP32 weights into the storage's 16-bit pair.
+typedef __bf16 bf16x2_v __attribute__((ext_vector_type(2)));
+typedef float f32x2_v __attribute__((ext_vector_type(2)));
+
+// Round-to-nearest-even pack of two FP32 weights into the storage's 16-bit
+// pair (v_cvt_pk_f16_f32 / v_cvt_pk_bf16_f32).
 template <int ST> __device__ __forceinline__ uint32_t pack_w(float lo, float hi)
 {
     if constexpr (ST == kStF16) {
         return __builtin_bit_cast(uint32_t, __floats2half2_rn(lo, hi));
     } else {
-        return (uint32_t)__bfloat16_as_ushort(__float2bfloat16(lo)) |
-               ((uint32_t)__bfloat16_as_ushort(__float2bfloat16(hi)) << 16);
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_v){lo, hi}, bf16x2_v));
     }
 }
 
@@ -1043,6 +1046,32 @@ __device__ __forceinline__ void fill_box(__amdgpu_buffer_rsrc_t rsrc, char *lds,
         // kFillBatch rows per pass of the loop: their loads are all in flight
         // before the first LDS write.  A row outside the padded volume (or
         // past the box) loads the padded cell at offset 0, which is a zero.
+        const bool inside = lo[0] >= -kVoxPadLo && lo[0] + sx <= nx + kVoxPadHi && lo[1] >= -kVoxPadLo &&
+                            lo[1] + sy <= ny + kVoxPadHi && lo[2] >= -kVoxPadLo && lo[2] + sz <= nz + kVoxPadHi;
+        if (inside) { // block-uniform: no row needs a bounds test
+            uint32_t off = __umul24((uint32_t)(lo[2] + lz + kVoxPadLo), oz) +
+                           (__umul24((uint32_t)(lo[1] + ly + kVoxPadLo), oy) + xoff);
+            const uint32_t step = (uint32_t)dy * oy + (uint32_t)dz * oz, wrap = oz - (uint32_t)sy * oy;
+            for (int r = r0; r < rows; r += kFillBatch * per) {
+                uint32_t d[kFillBatch][M];
+#pragma unroll
+                for (int b = 0; b < kFillBatch; b++) {
+                    load_raw<M>(rsrc, r + b * per < rows ? off : 0u, 0, d[b]);
+                    ly += dy;
+                    off += step;
+                    if (ly >= sy) {
+                        ly -= sy;
+                        off += wrap;
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < kFillBatch; b++) {
+                    if (r + b * per < rows) lds_write<M>(lds, loff, d[b]);
+                    loff += lstep;
+                }
+            }
+            return;
+        }
         for (int r = r0; r < rows; r += kFillBatch * per) {
             uint32_t d[kFillBatch][M];
 #pragma unroll
@@ -1125,7 +1154,8 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
         // every wave has finished the previous chunk past this barrier
         block_range<3>(lo, hi, scratch, parity);
         if (hi[0] < lo[0]) continue; // no lane of the block samples this chunk (block-uniform)
-        const int sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
+        // 16-bit cells are x-pairs that already hold the +1 x corner
+        const int sx = hi[0] - lo[0] + (ST == kStF32 ? 1 : 0), sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
         const bool staged = (int64_t)sx * sy * sz <= (int64_t)P.lds_cells;
         if (staged) {
             fill_box<M>(rsrc, lds, lo, sx, sy, sz, nx, ny, nz, oy, oz);

@@ -393,7 +393,7 @@ def test_mixed_layout_edges(require_gpu):
 
 # -------------------------------------------------------- launch tuning
 
-@pytest.mark.parametrize("dtype", [P.CTFP_F32, P.CTFP_F16])
+@pytest.mark.parametrize("dtype", [P.CTFP_F32, P.CTFP_F16, P.CTFP_BF16])
 def test_lds_staging_equals_global_gather(require_gpu, dtype):
     """The LDS-staged march reads the same corners in the same order as the
     global-gather march: outputs are bit-identical for every chunk length and
@@ -406,7 +406,8 @@ def test_lds_staging_equals_global_gather(require_gpu, dtype):
         proj.set_spectral(**kw)
         proj.set_tuning(march="global")
         ref = proj.forward(want_L=True)
-        for chunk, lds in [(16, 40 * 1024), (48, 40 * 1024), (16, 8 * 1024), (64, 64 * 1024), (32, 2048)]:
+        for chunk, lds in [(16, 40 * 1024), (48, 40 * 1024), (16, 8 * 1024), (64, 64 * 1024), (32, 2048),
+                           (32, 4 * 1024), (48, 12 * 1024)]:
             proj.set_tuning(march="lds", chunk=chunk, lds_bytes=lds)
             got = proj.forward(want_L=True)
             torch.cuda.synchronize()
