@@ -304,6 +304,8 @@ struct MarchArgs {
 };
 
 constexpr int kBlock = 256; // 4 wavefronts per workgroup
+// Zero rows after the last active bin of the spectral weight table.
+constexpr int kWeightPadBins = 8;
 
 template <int M>
 __device__ __forceinline__ void load_vox(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
@@ -816,22 +818,27 @@ __device__ __forceinline__ void spectrum_mfma(const MarchArgs &P, const TileLane
         for (int rb = 0; rb * 32 < n_act; rb++) {
             const half8_t a = *(const half8_t *)(P.Aq + ((rb * 32 + (lane & 31)) * 16 + 8 * h));
             const float16_t d = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, zero, 0, 0, 0);
+            // the weight of bin e sits at wp + e * stride: one pointer steps
+            // through the bins (no 64-bit multiply per bin)
+            const float *we = wp + (int64_t)(rb * 32 + 4 * h) * P.w_stride_e;
 #pragma unroll
             for (int bb = 0; bb < 4; bb++) {
                 if (rb * 32 + 8 * bb >= n_act) break; // uniform: the rest of the block is past the bins
                 // four rows at a time: keeps the weight loads and exp temporaries few
                 __builtin_amdgcn_sched_barrier(0);
+                // the four weights first, so their loads overlap; bins past
+                // the last active one read the table's zero rows and add 0
+                float wv[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) wv[q] = we[q * P.w_stride_e];
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    const int e = rb * 32 + 8 * bb + 4 * h + q;
-                    if (e < n_act && pix >= 0) {
-                        float ex, ab;
-                        attenuate(d[4 * bb + q] * (1.0f / 256.0f), ex, ab);
-                        const double w = (double)wp[(int64_t)e * P.w_stride_e];
-                        si = fma(w, (double)ex, si);
-                        sa = fma(w, (double)ab, sa);
-                    }
+                    float ex, ab;
+                    attenuate(d[4 * bb + q] * (1.0f / 256.0f), ex, ab);
+                    si = fma((double)wv[q], (double)ex, si);
+                    sa = fma((double)wv[q], (double)ab, sa);
                 }
+                we += 8 * P.w_stride_e;
             }
         }
         if (c == 0) { sI[0] = si; sA[0] = sa; } else { sI[1] = si; sA[1] = sa; }
@@ -997,8 +1004,10 @@ __device__ __forceinline__ void block_range(int (&lo)[D], int (&hi)[D], int *scr
             a = min(a, buf[w * 6 + d]);
             b = max(b, buf[w * 6 + 3 + d]);
         }
-        lo[d] = a;
-        hi[d] = b;
+        // equal in every lane: scalar registers, so what follows from the
+        // box runs on the scalar unit
+        lo[d] = __builtin_amdgcn_readfirstlane(a);
+        hi[d] = __builtin_amdgcn_readfirstlane(b);
     }
 }
 
@@ -1030,14 +1039,15 @@ __device__ __forceinline__ void fill_box(__amdgpu_buffer_rsrc_t rsrc, char *lds,
     const int rows = sy * sz;
     if (sx <= kBlock) {
         const float inv_sx = small_rcp(sx), inv_sy = small_rcp(sy);
-        const int per = small_div(kBlock, sx, inv_sx); // rows per pass
+        // rows per pass and its (y, z) step: block-uniform
+        const int per = __builtin_amdgcn_readfirstlane(small_div(kBlock, sx, inv_sx));
+        const int dz = __builtin_amdgcn_readfirstlane(small_div(per, sy, inv_sy)), dy = per - dz * sy;
         const int tid = threadIdx.x;
         if (tid >= per * sx) return;
         const int r0 = small_div(tid, sx, inv_sx);
-        const int x = lo[0] + (tid - r0 * sx);
+        const int x = lo[0] + (tid - __mul24(r0, sx));
         int lz = small_div(r0, sy, inv_sy);
-        int ly = r0 - lz * sy;
-        const int dz = small_div(per, sy, inv_sy), dy = per - dz * sy;
+        int ly = r0 - __mul24(lz, sy);
         const bool xin = (uint32_t)(x + kVoxPadLo) < (uint32_t)(nx + kVoxPadLo + kVoxPadHi);
         const uint32_t xoff = ((uint32_t)(x + kVoxPadLo) & 0xFFFFFu) * (4u * M);
         const uint32_t ymax = (uint32_t)(ny + kVoxPadLo + kVoxPadHi), zmax = (uint32_t)(nz + kVoxPadLo + kVoxPadHi);
@@ -2043,7 +2053,7 @@ int ctfp_device_bytes(const ctfp_context *ctx, int64_t *bytes)
     const int64_t npix = ctx->scan.d_num_u * ctx->scan.d_num_v;
     int64_t b = ctx->vol ? (int64_t)ctx->vol_bytes : 0;
     if (ctx->A) b += (int64_t)ctx->n_act * ctx->spec_n_mat * (int64_t)sizeof(float);
-    if (ctx->W) b += (int64_t)std::max(ctx->n_act, 1) * ctx->w_stride_e * (int64_t)sizeof(float);
+    if (ctx->W) b += (int64_t)(std::max(ctx->n_act, 1) + kWeightPadBins) * ctx->w_stride_e * (int64_t)sizeof(float);
     if (ctx->I0) b += npix * (int64_t)sizeof(float);
     if (ctx->view_cs) b += ctx->scan.proj_num * (int64_t)sizeof(double2);
     if (ctx->dI) b += 2 * ctx->staged_views * npix * (int64_t)sizeof(float);
@@ -2156,7 +2166,11 @@ int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
         HIP_TRY(hipMalloc(&ctx->Aq, Aq.size() * sizeof(_Float16)));
         HIP_TRY(hipMemcpy(ctx->Aq, Aq.data(), Aq.size() * sizeof(_Float16), hipMemcpyHostToDevice));
     }
-    HIP_TRY(hipMalloc(&ctx->W, (size_t)std::max(na, 1) * rows * nu * sizeof(float)));
+    // kWeightPadBins zero rows follow the last active bin (the matrix-core
+    // spectrum reads bins in groups that may run past it)
+    HIP_TRY(hipMalloc(&ctx->W, (size_t)(std::max(na, 1) + kWeightPadBins) * rows * nu * sizeof(float)));
+    HIP_TRY(hipMemset(ctx->W + (size_t)std::max(na, 1) * rows * nu, 0,
+                      (size_t)kWeightPadBins * rows * nu * sizeof(float)));
     // stage the per-bin tables of the active bins
     std::vector<float> T, eta, mu;
     float *dS = nullptr, *dT = nullptr, *dEta = nullptr, *dMu = nullptr, *dThick = nullptr;
