@@ -1011,6 +1011,68 @@ __device__ __forceinline__ void block_range(int (&lo)[D], int (&hi)[D], int *scr
     }
 }
 
+// Min / max over each 16-lane row, in every lane of the row (DPP quad
+// permutes and row mirrors fused into v_min / v_max; all lanes active).
+__device__ __forceinline__ int row_min(int v)
+{
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));  // quad_perm [1, 0, 3, 2]
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));  // quad_perm [2, 3, 0, 1]
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true)); // row_half_mirror
+    return min(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true)); // row_mirror
+}
+__device__ __forceinline__ int row_max(int v)
+{
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));
+    return max(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true));
+}
+
+// Ints of the march's box scratch: three buffers of 8 (lo x, y, z at 0..2,
+// hi x, y, z at 4..6).
+constexpr int kBoxBufs = 3;
+
+// Every box buffer back to (INT_MAX, INT_MIN); thread 0 only.
+__device__ __forceinline__ void box_reset(int *buf)
+{
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        buf[d] = INT_MAX;
+        buf[4 + d] = INT_MIN;
+    }
+}
+
+// Block-wide min of lo[] and max of hi[] with one barrier, as scalars.  Each
+// 16-lane row reduces its lanes with DPP, one lane per row folds the row
+// into buffer `call` with LDS atomic min / max, and after the barrier every
+// lane reads it.  The buffer must be reset before the call; the call then
+// resets buffer call - 1 (mod 3), whose last reads all came before this
+// barrier, for use two calls later.
+template <int D>
+__device__ __forceinline__ void block_box_atomic(int (&lo)[D], int (&hi)[D], int *scratch, int call)
+{
+    int *buf = scratch + call * 8;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        lo[d] = row_min(lo[d]);
+        hi[d] = row_max(hi[d]);
+    }
+    if ((threadIdx.x & 15) == 0) {
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            atomicMin(buf + d, lo[d]);
+            atomicMax(buf + 4 + d, hi[d]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        lo[d] = __builtin_amdgcn_readfirstlane(buf[d]);
+        hi[d] = __builtin_amdgcn_readfirstlane(buf[4 + d]);
+    }
+    if (threadIdx.x == 0) box_reset(scratch + (call == 0 ? kBoxBufs - 1 : call - 1) * 8);
+}
+
 // Exact q = n / d for 0 <= n < 2^22, 1 <= d < 2^10, from a float reciprocal.
 __device__ __forceinline__ int small_div(int n, int d, float inv_d)
 {
@@ -1142,12 +1204,18 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
     const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
     const int k_hi = R.k_lo + R.n;
     int kr_lo[1] = {R.n > 0 ? R.k_lo : INT_MAX}, kr_hi[1] = {R.n > 0 ? k_hi : INT_MIN};
-    __syncthreads(); // the block's previous user of the LDS has finished
-    block_range<1>(kr_lo, kr_hi, scratch, 0);
+    // the block's previous users of the scratch have finished two barriers
+    // ago; the barrier below publishes the reset buffers
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int b = 0; b < kBoxBufs; b++) box_reset(scratch + 8 * b);
+    }
+    __syncthreads();
+    int call = 0;
+    block_box_atomic<1>(kr_lo, kr_hi, scratch, call);
     const int kb_hi = kr_hi[0];
-    int parity = 1;
     // chunks start on the anchor grid (chunk is a multiple of kAnchor)
-    for (int k0 = kr_lo[0] & ~(kAnchor - 1); k0 < kb_hi; k0 += P.chunk, parity ^= 1) {
+    for (int k0 = kr_lo[0] & ~(kAnchor - 1); k0 < kb_hi; k0 += P.chunk) {
         const int c_lo = max(k0, R.k_lo), c_hi = min(k0 + P.chunk, k_hi);
         // the chunk's cell box from the lane's FP32 sample positions; the
         // first anchor is kept for the march
@@ -1162,7 +1230,8 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
             if (i0 < i1) segment_box(anchor_at(R, ka), i0, i1, lo, hi);
         }
         // every wave has finished the previous chunk past this barrier
-        block_range<3>(lo, hi, scratch, parity);
+        call = call == kBoxBufs - 1 ? 0 : call + 1;
+        block_box_atomic<3>(lo, hi, scratch, call);
         if (hi[0] < lo[0]) continue; // no lane of the block samples this chunk (block-uniform)
         // 16-bit cells are x-pairs that already hold the +1 x corner
         const int sx = hi[0] - lo[0] + (ST == kStF32 ? 1 : 0), sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
@@ -1299,7 +1368,7 @@ template <int ST>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void batch_kernel(const BatchItem *__restrict__ items, BatchQueues Q)
 {
     extern __shared__ char lds_raw[];
-    uint32_t *slot = (uint32_t *)lds_raw + 60; // beside the block_range scratch (ints 0..47)
+    uint32_t *slot = (uint32_t *)lds_raw + 60; // beside the march's box scratch (ints 0..23)
     const uint32_t home = blockIdx.x & (kXcds - 1);
     const int stride = Q.n_items + 1;
     for (uint32_t qi = 0; qi < (uint32_t)kXcds; qi++) {
