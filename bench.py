@@ -369,6 +369,21 @@ def main():
     marched = proj.marched_samples(0, views)
     wall, kern_ms = timed(proj, args.dtype)
     ms_per_step = wall * 1e3 / args.steps
+
+    # stage split: the same launch without the spectrum (line integrals of
+    # the M materials written instead of I and IAbsorb), outside the timed run
+    Lbuf = torch.empty((views, len(vols), nv, nu), dtype=torch.float32, device=dev)
+    proj.line_integrals(0, views, out=Lbuf, stream=stream)
+    march_ms = []
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        proj.line_integrals(0, views, out=Lbuf, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        march_ms.append(e0.elapsed_time(e1))
+    del Lbuf
+    log(f"march only (line integrals): per-launch ms {march_ms}")
     gsps = samples / (ms_per_step / 1e3) / 1e9
     avg_launch_s = sum(kern_ms) / len(kern_ms) / 1e3
     achieved = samples * bytes_per_sample / avg_launch_s / 1e9
@@ -394,6 +409,8 @@ def main():
         "views_per_s": round(views / (ms_per_step / 1e3), 2),
         "median_launch_ms": round(sorted(kern_ms)[len(kern_ms) // 2], 3),
         "evaluated_fraction": round(marched / samples, 4),
+        "stages_ms": {"march_only": round(sorted(march_ms)[1], 3),
+                      "spectrum": round(sorted(kern_ms)[len(kern_ms) // 2] - sorted(march_ms)[1], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
