@@ -230,6 +230,23 @@ __global__ void build_weights(WeightArgs a, float *__restrict__ W)
 }
 
 // I0[v][u] = sum_e W[e][v][u] in bin order.
+// Compensated (Kahan) FP32 sum: c keeps the rounding error of each addition,
+// so a sum of ~100 positive terms is accurate to a few FP32 ulps of the total
+// (a plain FP32 sum can lose ~n ulps, visible in ln(I0/I) of thin rays).
+struct KSum {
+    float s, c;
+};
+__device__ __forceinline__ KSum ksum_zero() { return KSum{0.0f, 0.0f}; }
+__device__ __forceinline__ void ksum_add(KSum &k, float x)
+{
+#pragma clang fp contract(off)
+    const float y = x - k.c;
+    const float t = k.s + y;
+    k.c = (t - k.s) - y;
+    k.s = t;
+}
+__device__ __forceinline__ float ksum_value(const KSum &k) { return k.s - k.c; }
+
 __global__ void flat_field_kernel(const float *__restrict__ W, int n_act, int64_t stride_e,
                                   int stride_v, int64_t nu, int64_t nv, float *__restrict__ I0)
 {
@@ -237,9 +254,9 @@ __global__ void flat_field_kernel(const float *__restrict__ W, int n_act, int64_
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix;
          p += (int64_t)gridDim.x * blockDim.x) {
         int64_t u = p % nu, v = p / nu;
-        double acc = 0.0; // bin order, FP64 like the per-ray sums
-        for (int e = 0; e < n_act; e++) acc += (double)W[e * stride_e + v * stride_v + u];
-        I0[p] = (float)acc;
+        KSum acc = ksum_zero(); // bin order, compensated like the per-ray sums
+        for (int e = 0; e < n_act; e++) ksum_add(acc, W[e * stride_e + v * stride_v + u]);
+        I0[p] = ksum_value(acc);
     }
 }
 
@@ -348,10 +365,21 @@ __device__ __forceinline__ void clip_axis(double s, double d, double n, double &
 
 // Ray (u, v) of view `vw`: FP64 set-up of the index-space start (at k_lo) and
 // step, and the sample range [k_lo, k_hi).
+// A ray's samples in index space.  Positions are 32.32 fixed point (units of
+// 2^-32 voxel) so that anchors are exact integer steps: every path that
+// reaches an anchor, by whatever steps, gets the same value, and the march
+// needs no FP64 after the set-up.
 struct RaySetup {
-    double a0, b0, c0, da, db, dc;
+    int64_t p0[3]; // position of sample k_lo
+    int64_t d[3];  // step per sample
+    float df[3];   // step per sample in FP32 (samples after an anchor)
     int k_lo, n;
 };
+
+constexpr double kFix = 4294967296.0; // 2^32
+
+// FP64 index-space coordinate to 32.32 fixed point (|x| < 2^31).
+__device__ __forceinline__ int64_t to_fix(double x) { return (int64_t)rint(x * kFix); }
 
 // Index-space line of ray (u, v) in the view with rotation cs = (cos, sin):
 // position(t) = src + t * dir, t in mm from the source (FP64).
@@ -450,12 +478,30 @@ __device__ __forceinline__ RaySetup setup_ray(const MarchArgs &P, int u, int v, 
     }
     R.k_lo = k_lo;
     R.n = max(0, k_hi - k_lo);
-    R.a0 = sa + da * k_lo;
-    R.b0 = sb2 + db * k_lo;
-    R.c0 = sc + dc * k_lo;
-    R.da = da;
-    R.db = db;
-    R.dc = dc;
+    R.p0[0] = to_fix(sa + da * k_lo);
+    R.p0[1] = to_fix(sb2 + db * k_lo);
+    R.p0[2] = to_fix(sc + dc * k_lo);
+    R.d[0] = to_fix(da);
+    R.d[1] = to_fix(db);
+    R.d[2] = to_fix(dc);
+    R.df[0] = (float)da;
+    R.df[1] = (float)db;
+    R.df[2] = (float)dc;
+    return R;
+}
+
+// A lane without a ray (past the detector edge): no samples.
+__device__ __forceinline__ RaySetup empty_ray()
+{
+    RaySetup R;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        R.p0[a] = 0;
+        R.d[a] = 0;
+        R.df[a] = 0.0f;
+    }
+    R.k_lo = 0;
+    R.n = 0;
     return R;
 }
 
@@ -516,8 +562,8 @@ template <int ST> __device__ __forceinline__ uint32_t pack_w(float lo, float hi)
 }
 
 // Sample positions.  Every kAnchor-th sample of the ray (global sample index
-// k a multiple of kAnchor) is an anchor whose index-space position is
-// evaluated in FP64 and split into an integer cell and an FP32 offset; the
+// k a multiple of kAnchor) is an anchor whose position is exact in 32.32
+// fixed point and split into an integer cell and an FP32 offset; the
 // kAnchor samples from it on are offset + i * step in FP32 (|error| <
 // ~3e-6 * max(1, |step|) voxel, against ~1e-4 for the reference's FP32
 // t * d - SOD).  The anchor grid is fixed, so every launch option sees the
@@ -529,22 +575,51 @@ struct Anchor {
     float fa, fb, fc;     // offset of the anchor inside its cell, [0, 1)
     float da, db, dc;     // per-sample step (FP32)
 };
-__device__ __forceinline__ Anchor anchor_at(const RaySetup &R, int ka)
+
+// A fixed-point position (three axes).
+struct FixPos {
+    int64_t p[3];
+};
+
+// Position of global sample k: one 64-bit product, for loop starts.
+__device__ __forceinline__ FixPos fix_at(const RaySetup &R, int k)
 {
-#pragma clang fp contract(off)
-    const double j = (double)(ka - R.k_lo);
-    const double pa = fma(j, R.da, R.a0), pb = fma(j, R.db, R.b0), pc = fma(j, R.dc, R.c0);
-    const double ga = floor(pa), gb = floor(pb), gc = floor(pc);
+    const int64_t j = (int64_t)(k - R.k_lo);
+    FixPos q;
+#pragma unroll
+    for (int a = 0; a < 3; a++) q.p[a] = R.p0[a] + j * R.d[a];
+    return q;
+}
+
+// n samples' worth of steps.
+__device__ __forceinline__ FixPos fix_steps(const RaySetup &R, int n)
+{
+    FixPos q;
+#pragma unroll
+    for (int a = 0; a < 3; a++) q.p[a] = (int64_t)n * R.d[a];
+    return q;
+}
+
+__device__ __forceinline__ void fix_add(FixPos &q, const FixPos &s)
+{
+#pragma unroll
+    for (int a = 0; a < 3; a++) q.p[a] += s.p[a];
+}
+
+// The anchor at fixed-point position q: the cell is the high word (a floor),
+// the offset the top 24 bits of the low word (exact in FP32, < 1).
+__device__ __forceinline__ Anchor anchor_from(const FixPos &q, const RaySetup &R)
+{
     Anchor A;
-    A.ia = (int)ga;
-    A.ib = (int)gb;
-    A.ic = (int)gc;
-    A.fa = (float)(pa - ga);
-    A.fb = (float)(pb - gb);
-    A.fc = (float)(pc - gc);
-    A.da = (float)R.da;
-    A.db = (float)R.db;
-    A.dc = (float)R.dc;
+    A.ia = (int)(q.p[0] >> 32);
+    A.ib = (int)(q.p[1] >> 32);
+    A.ic = (int)(q.p[2] >> 32);
+    A.fa = (float)((uint32_t)q.p[0] >> 8) * 0x1p-24f;
+    A.fb = (float)((uint32_t)q.p[1] >> 8) * 0x1p-24f;
+    A.fc = (float)((uint32_t)q.p[2] >> 8) * 0x1p-24f;
+    A.da = R.df[0];
+    A.db = R.df[1];
+    A.dc = R.df[2];
     return A;
 }
 
@@ -585,21 +660,6 @@ __device__ __forceinline__ Cell absolute_cell(const Anchor &A, const Cell &o, in
     c.y = min(max(A.ib + o.y, -2), ny);
     c.z = min(max(A.ic + o.z, -2), nz);
     return c;
-}
-
-// Cells that samples [j0, j1) of a ray can touch, from the FP64 positions of
-// the two ends widened by 1e-4 voxel (far above the FP32 offset error), as
-// [lo, hi] with the +1 corner included.
-__device__ __forceinline__ void sample_box(const RaySetup &R, int j0, int j1, int (&lo)[3], int (&hi)[3])
-{
-    const double ja = (double)j0, jb = (double)(j1 - 1);
-    const double a[3] = {fma(ja, R.da, R.a0), fma(ja, R.db, R.b0), fma(ja, R.dc, R.c0)};
-    const double b[3] = {fma(jb, R.da, R.a0), fma(jb, R.db, R.b0), fma(jb, R.dc, R.c0)};
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        lo[d] = (int)floor(fmin(a[d], b[d]) - 1e-4);
-        hi[d] = (int)floor(fmax(a[d], b[d]) + 1e-4) + 1;
-    }
 }
 
 // Cells that samples i0 .. i1-1 after anchor A use, [lo, hi] with the +1
@@ -770,7 +830,7 @@ typedef float float16_t __attribute__((ext_vector_type(16)));
 // of the wavefront must call it; `active` marks rays inside the detector.
 template <int M>
 __device__ __forceinline__ void spectrum_mfma(const MarchArgs &P, const TileLane &t, const float (&Lm)[M],
-                                              double &sI_out, double &sA_out)
+                                              float &sI_out, float &sA_out)
 {
     static_assert(3 * M <= 16, "K = 3M must fit one 32x32x16 MFMA");
     const int lane = threadIdx.x & 63, h = lane >> 5;
@@ -780,7 +840,7 @@ __device__ __forceinline__ void spectrum_mfma(const MarchArgs &P, const TileLane
     for (int m = 0; m < M; m++) Lx[m] = __shfl_xor(Lm[m], 32);
     const int pix_own = t.active ? t.v * (int)P.w_stride_v + t.u : -1;
     const int pix_x = __shfl_xor(pix_own, 32);
-    double sI[2] = {0.0, 0.0}, sA[2] = {0.0, 0.0};
+    float sI[2] = {0.0f, 0.0f}, sA[2] = {0.0f, 0.0f};
     const float16_t zero = {};
     const int n_act = P.n_act;
 #pragma unroll 1
@@ -813,7 +873,7 @@ __device__ __forceinline__ void spectrum_mfma(const MarchArgs &P, const TileLane
         const half8_t b = __builtin_bit_cast(half8_t, bw);
         const int pix = own ? pix_own : pix_x;
         const float *wp = P.W + (pix < 0 ? 0 : pix);
-        double si = 0.0, sa = 0.0;
+        KSum si = ksum_zero(), sa = ksum_zero();
 #pragma unroll 1
         for (int rb = 0; rb * 32 < n_act; rb++) {
             const half8_t a = *(const half8_t *)(P.Aq + ((rb * 32 + (lane & 31)) * 16 + 8 * h));
@@ -835,17 +895,23 @@ __device__ __forceinline__ void spectrum_mfma(const MarchArgs &P, const TileLane
                 for (int q = 0; q < 4; q++) {
                     float ex, ab;
                     attenuate(d[4 * bb + q] * (1.0f / 256.0f), ex, ab);
-                    si = fma((double)wv[q], (double)ex, si);
-                    sa = fma((double)wv[q], (double)ab, sa);
+                    ksum_add(si, wv[q] * ex);
+                    ksum_add(sa, wv[q] * ab);
                 }
                 we += 8 * P.w_stride_e;
             }
         }
-        if (c == 0) { sI[0] = si; sA[0] = sa; } else { sI[1] = si; sA[1] = sa; }
+        if (c == 0) {
+            sI[0] = ksum_value(si);
+            sA[0] = ksum_value(sa);
+        } else {
+            sI[1] = ksum_value(si);
+            sA[1] = ksum_value(sa);
+        }
     }
     // partial sums of the partner's own ray go to the partner
-    const double rI = __shfl_xor(h ? sI[0] : sI[1], 32);
-    const double rA = __shfl_xor(h ? sA[0] : sA[1], 32);
+    const float rI = __shfl_xor(h ? sI[0] : sI[1], 32);
+    const float rA = __shfl_xor(h ? sA[0] : sA[1], 32);
     sI_out = (h ? sI[1] : sI[0]) + rI;
     sA_out = (h ? sA[1] : sA[0]) + rA;
 }
@@ -866,33 +932,33 @@ __device__ __forceinline__ void finish_ray(const MarchArgs &P, const TileLane &t
     }
     if constexpr (kSpectrum && ST != kStF32 && 3 * M <= 16) {
         // the mixed-precision projector: exponents on the matrix cores
-        double sI, sA;
+        float sI, sA;
         spectrum_mfma<M>(P, t, Lm, sI, sA);
         if (t.active) {
-            P.I[(int64_t)t.vw_local * npix + pix] = (float)sI;
-            P.IA[(int64_t)t.vw_local * npix + pix] = (float)sA;
+            P.I[(int64_t)t.vw_local * npix + pix] = sI;
+            P.IA[(int64_t)t.vw_local * npix + pix] = sA;
         }
     } else if constexpr (kSpectrum) {
         if (!t.active) return;
         // Per-ray spectrum: tau_e = sum_m A[e][m] L_m (FP32), e^-tau and
         // 1 - e^-tau (attenuate: no cancellation at small tau) in FP32, and the
-        // energy sums of W e^-tau and W (1 - e^-tau) in bin order in FP64 (an
-        // FP32 sum of ~100 terms would cost ~1e-6 of I, visible in ln(I0/I)
-        // of thin rays; FP64 FMA runs at the FP32 rate).
+        // energy sums of W e^-tau and W (1 - e^-tau) in bin order, compensated
+        // (a plain FP32 sum of ~100 terms would cost ~1e-6 of I, visible in
+        // ln(I0/I) of thin rays).
         const float *wp = P.W + (int64_t)t.v * P.w_stride_v + t.u;
-        double sI = 0.0, sA = 0.0;
+        KSum sI = ksum_zero(), sA = ksum_zero();
         for (int e = 0; e < P.n_act; e++) {
             float tau = 0.0f;
 #pragma unroll
             for (int m = 0; m < M; m++) tau = fmaf(P.A[e * M + m], Lm[m], tau);
             float ex, ab;
             attenuate(tau, ex, ab);
-            double w = (double)wp[(int64_t)e * P.w_stride_e];
-            sI = fma(w, (double)ex, sI);
-            sA = fma(w, (double)ab, sA);
+            const float w = wp[(int64_t)e * P.w_stride_e];
+            ksum_add(sI, w * ex);
+            ksum_add(sA, w * ab);
         }
-        P.I[(int64_t)t.vw_local * npix + pix] = (float)sI;
-        P.IA[(int64_t)t.vw_local * npix + pix] = (float)sA;
+        P.I[(int64_t)t.vw_local * npix + pix] = ksum_value(sI);
+        P.IA[(int64_t)t.vw_local * npix + pix] = ksum_value(sA);
     }
 }
 
@@ -911,14 +977,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     const uint32_t logical = xcd_logical(P);
     if (logical >= P.n_blocks) return;
     const TileLane t = tile_lane(P, logical);
-    RaySetup R;
-    if (t.active) {
-        R = setup_ray(P, t.u, t.v, (int)(P.view_begin + t.vw_local));
-    } else {
-        R.a0 = R.b0 = R.c0 = R.da = R.db = R.dc = 0.0;
-        R.k_lo = 0;
-        R.n = 0;
-    }
+    const RaySetup R = t.active ? setup_ray(P, t.u, t.v, (int)(P.view_begin + t.vw_local)) : empty_ray();
     __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(P.vol), 0, (int)P.vol_bytes, 0x00020000);
     const uint32_t oy = P.row_y & 0xFFFFFFu, oz = P.row_z & 0xFFFFFFu;
@@ -927,8 +986,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
     for (int m = 0; m < M; m++) acc[m] = 0.0f;
     const int k_hi = R.k_lo + R.n;
-    for (int ka = R.k_lo & ~(kAnchor - 1); ka < k_hi; ka += kAnchor) {
-        const Anchor A = anchor_at(R, ka);
+    const FixPos d16 = fix_steps(R, kAnchor);
+    FixPos q = fix_at(R, R.k_lo & ~(kAnchor - 1));
+    for (int ka = R.k_lo & ~(kAnchor - 1); ka < k_hi; ka += kAnchor, fix_add(q, d16)) {
+        const Anchor A = anchor_from(q, R);
         const int i1 = min(ka + kAnchor, k_hi) - ka;
         int i = max(ka, R.k_lo) - ka;
         for (float fi = (float)i; i < i1; i++, fi += 1.0f)
@@ -1214,20 +1275,28 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
     int call = 0;
     block_box_atomic<1>(kr_lo, kr_hi, scratch, call);
     const int kb_hi = kr_hi[0];
+    // the chunk's first anchor walks in exact fixed-point steps
+    const int kstart = kr_lo[0] & ~(kAnchor - 1);
+    const FixPos d16 = fix_steps(R, kAnchor), dchunk = fix_steps(R, P.chunk);
+    FixPos pk = fix_at(R, kstart);
     // chunks start on the anchor grid (chunk is a multiple of kAnchor)
-    for (int k0 = kr_lo[0] & ~(kAnchor - 1); k0 < kb_hi; k0 += P.chunk) {
+    for (int k0 = kstart; k0 < kb_hi; k0 += P.chunk, fix_add(pk, dchunk)) {
         const int c_lo = max(k0, R.k_lo), c_hi = min(k0 + P.chunk, k_hi);
         // the chunk's cell box from the lane's FP32 sample positions; the
         // first anchor is kept for the march
         int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-        const Anchor A0 = anchor_at(R, k0);
+        const Anchor A0 = anchor_from(pk, R);
         {
             const int i0 = max(k0, c_lo) - k0, i1 = min(k0 + kAnchor, c_hi) - k0;
             if (i0 < i1) segment_box(A0, i0, i1, lo, hi);
         }
-        for (int ka = k0 + kAnchor; ka < k0 + P.chunk; ka += kAnchor) {
-            const int i0 = max(ka, c_lo) - ka, i1 = min(ka + kAnchor, c_hi) - ka;
-            if (i0 < i1) segment_box(anchor_at(R, ka), i0, i1, lo, hi);
+        {
+            FixPos q = pk;
+            for (int ka = k0 + kAnchor; ka < k0 + P.chunk; ka += kAnchor) {
+                fix_add(q, d16);
+                const int i0 = max(ka, c_lo) - ka, i1 = min(ka + kAnchor, c_hi) - ka;
+                if (i0 < i1) segment_box(anchor_from(q, R), i0, i1, lo, hi);
+            }
         }
         // every wave has finished the previous chunk past this barrier
         call = call == kBoxBufs - 1 ? 0 : call + 1;
@@ -1241,11 +1310,12 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
             __syncthreads();
             const int ry = sx * (int)LB, rz = sx * sy * (int)LB;
             const uint32_t roff[4] = {0u, (uint32_t)ry, (uint32_t)rz, (uint32_t)(ry + rz)};
-            for (int ka = k0; ka < k0 + P.chunk; ka += kAnchor) {
+            FixPos q = pk;
+            for (int ka = k0; ka < k0 + P.chunk; ka += kAnchor, fix_add(q, d16)) {
                 const int i1 = min(ka + kAnchor, c_hi) - ka;
                 int i = max(ka, c_lo) - ka;
                 if (i >= i1) continue;
-                const Anchor A = ka == k0 ? A0 : anchor_at(R, ka);
+                const Anchor A = ka == k0 ? A0 : anchor_from(q, R);
                 // byte offset of the anchor's cell in the box; samples add
                 // their (small, signed) cell offsets with 24-bit multiplies
                 const char *pa =
@@ -1265,11 +1335,12 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
                 }
             }
         } else {
-            for (int ka = k0; ka < k0 + P.chunk; ka += kAnchor) {
+            FixPos q = pk;
+            for (int ka = k0; ka < k0 + P.chunk; ka += kAnchor, fix_add(q, d16)) {
                 const int i1 = min(ka + kAnchor, c_hi) - ka;
                 int i = max(ka, c_lo) - ka;
                 if (i >= i1) continue;
-                const Anchor A = ka == k0 ? A0 : anchor_at(R, ka);
+                const Anchor A = ka == k0 ? A0 : anchor_from(q, R);
                 for (float fi = (float)i; i < i1; i++, fi += 1.0f)
                     accumulate_global<M, ST>(rsrc, absolute_cell(A, offset_cell(A, fi), nx, ny, nz), oy,
                                              oz, acc);
@@ -1280,15 +1351,7 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
 
 __device__ __forceinline__ RaySetup lane_ray(const MarchArgs &P, const TileLane &t)
 {
-    RaySetup R;
-    if (t.active) {
-        R = setup_ray(P, t.u, t.v, (int)(P.view_begin + t.vw_local));
-    } else {
-        R.a0 = R.b0 = R.c0 = R.da = R.db = R.dc = 0.0;
-        R.k_lo = 0;
-        R.n = 0;
-    }
-    return R;
+    return t.active ? setup_ray(P, t.u, t.v, (int)(P.view_begin + t.vw_local)) : empty_ray();
 }
 
 // One ray tile (logical block `logical` of P) through the LDS-staged march.
@@ -1432,10 +1495,19 @@ __device__ __forceinline__ void scatter_chunks(const MarchArgs &P, const RaySetu
         for (int m = 0; m < M; m++)
             if (g[m] != 0.0f) unsafeAtomicAdd(grad + m * nvox + q, g[m] * w);
     };
-    for (int k0 = kr_lo[0] & ~(kAnchor - 1); k0 < kb_hi; k0 += P.chunk) {
+    const int kstart = kr_lo[0] & ~(kAnchor - 1);
+    const FixPos d16 = fix_steps(R, kAnchor), dchunk = fix_steps(R, P.chunk);
+    FixPos pk = fix_at(R, kstart);
+    for (int k0 = kstart; k0 < kb_hi; k0 += P.chunk, fix_add(pk, dchunk)) {
         const int c_lo = max(k0, R.k_lo), c_hi = min(k0 + P.chunk, k_hi);
         int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-        if (c_hi > c_lo) sample_box(R, c_lo - R.k_lo, c_hi - R.k_lo, lo, hi);
+        {
+            FixPos q = pk;
+            for (int ka = k0; ka < k0 + P.chunk; ka += kAnchor, fix_add(q, d16)) {
+                const int i0 = max(ka, c_lo) - ka, i1 = min(ka + kAnchor, c_hi) - ka;
+                if (i0 < i1) segment_box(anchor_from(q, R), i0, i1, lo, hi);
+            }
+        }
         __syncthreads();
         block_box(lo, hi, scratch);
         if (hi[0] < lo[0]) continue; // no lane of the block samples this chunk (block-uniform)
@@ -1447,11 +1519,12 @@ __device__ __forceinline__ void scatter_chunks(const MarchArgs &P, const RaySetu
             __syncthreads();
             const int ry = sx * (int)LW, rz = sx * sy * (int)LW;
             const int roff[4] = {0, ry, rz, ry + rz};
-            for (int ka = k0; ka < k0 + P.chunk; ka += kAnchor) {
+            FixPos q = pk;
+            for (int ka = k0; ka < k0 + P.chunk; ka += kAnchor, fix_add(q, d16)) {
                 const int i1 = min(ka + kAnchor, c_hi) - ka;
                 int i = max(ka, c_lo) - ka;
                 if (i >= i1) continue;
-                const Anchor A = anchor_at(R, ka);
+                const Anchor A = anchor_from(q, R);
                 const int base = (((A.ic - lo[2]) * sy + (A.ib - lo[1])) * sx + (A.ia - lo[0])) * (int)LW;
                 for (float fi = (float)i; i < i1; i++, fi += 1.0f) {
                     const Cell c = offset_cell(A, fi);
@@ -1484,11 +1557,12 @@ __device__ __forceinline__ void scatter_chunks(const MarchArgs &P, const RaySetu
                 }
             }
         } else {
-            for (int ka = k0; ka < k0 + P.chunk; ka += kAnchor) {
+            FixPos q = pk;
+            for (int ka = k0; ka < k0 + P.chunk; ka += kAnchor, fix_add(q, d16)) {
                 const int i1 = min(ka + kAnchor, c_hi) - ka;
                 int i = max(ka, c_lo) - ka;
                 if (i >= i1) continue;
-                const Anchor A = anchor_at(R, ka);
+                const Anchor A = anchor_from(q, R);
                 for (float fi = (float)i; i < i1; i++, fi += 1.0f) {
                     const Cell o = offset_cell(A, fi);
                     const int x = A.ia + o.x, y = A.ib + o.y, z = A.ic + o.z;
@@ -1540,21 +1614,21 @@ __device__ __forceinline__ void ray_gradient(const MarchArgs &P, const AdjointAr
         const float c = (G.gIA ? G.gIA[q] : 0.0f) - (G.gI ? G.gI[q] : 0.0f);
         if (c != 0.0f) {
             const float *wp = P.W + (int64_t)t.v * P.w_stride_v + t.u;
-            double sm[M];
+            KSum sm[M];
 #pragma unroll
-            for (int m = 0; m < M; m++) sm[m] = 0.0;
+            for (int m = 0; m < M; m++) sm[m] = ksum_zero();
             for (int e = 0; e < P.n_act; e++) {
                 float tau = 0.0f;
 #pragma unroll
                 for (int m = 0; m < M; m++) tau = fmaf(P.A[e * M + m], Lm[m], tau);
                 float ex, ab;
                 attenuate(tau, ex, ab);
-                const double we = (double)wp[(int64_t)e * P.w_stride_e] * (double)ex;
+                const float we = wp[(int64_t)e * P.w_stride_e] * ex;
 #pragma unroll
-                for (int m = 0; m < M; m++) sm[m] = fma((double)P.A[e * M + m], we, sm[m]);
+                for (int m = 0; m < M; m++) ksum_add(sm[m], P.A[e * M + m] * we);
             }
 #pragma unroll
-            for (int m = 0; m < M; m++) g[m] = (float)((double)c * sm[m] * (double)P.dx_f);
+            for (int m = 0; m < M; m++) g[m] = c * ksum_value(sm[m]) * P.dx_f;
         }
     }
 }

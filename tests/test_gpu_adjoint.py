@@ -160,9 +160,11 @@ def test_gather_adjoint_is_deterministic_and_matches_scatter(require_gpu):
 def test_adjoint_lds_equals_global_atomics(require_gpu):
     """Scatter adjoint: LDS-accumulated chunks against the all-global-atomics fallback (an LDS
     budget too small for any chunk box).  Both sum in FP32 atomics in an
-    order that varies from run to run, and random-sign upstream gradients
-    make the sums cancel, so they agree to 1e-4 of the floor 1e-3 max|grad|,
-    not bitwise."""
+    order that varies from run to run, so they agree to rounding, not
+    bitwise.  With random-sign upstream gradients the sums cancel, and that
+    rounding scales with the sum of the absolute contributions per voxel,
+    which is the backprojection of |gI| (every ray factor is non-negative):
+    the difference is measured against it, as in the FP64 adjoint test."""
     import torch
     s, vols = setup_case(n=48, views=5, det=(90, 70), d_size=0.9)
     kw = P.spectral_inputs(s, *s.det)
@@ -174,11 +176,14 @@ def test_adjoint_lds_equals_global_atomics(require_gpu):
         proj.set_spectral(**kw)
         proj.set_tuning(march="lds", chunk=64, lds_bytes=1024, adjoint="scatter")
         ref = proj.backward(gI, None).cpu().numpy()
+        scale = proj.backward(gI.abs(), None).cpu().numpy().astype(np.float64)
+        floor = 1e-3 * scale.max()
         for march, chunk, lds in [("lds", 16, 40 * 1024), ("lds", 48, 12 * 1024), ("lds", 32, 2048),
                                   ("global", 16, 40 * 1024)]:
             proj.set_tuning(march=march, chunk=chunk, lds_bytes=lds)
             got = proj.backward(gI, None).cpu().numpy()
-            assert rel_err(got, ref) < 1e-4, (march, chunk, lds)
+            err = float((np.abs(got.astype(np.float64) - ref) / np.maximum(scale, floor)).max())
+            assert err < 1e-4, (march, chunk, lds, err)
 
 
 @pytest.mark.parametrize("mode", ["gather", "scatter"])
