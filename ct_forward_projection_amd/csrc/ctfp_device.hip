@@ -1672,20 +1672,35 @@ struct GatherArgs {
 
 constexpr int kBrickX = 8, kBrickY = 8, kBrickZ = 4;
 
+// A brick's shadow in one view: the projection of its centre, the Jacobian
+// (pixels per voxel step), the half-extent of one voxel's support in pixels,
+// and the shadow's pixel rectangle.
+struct BrickView {
+    float u0, v0, jua, jub, juc, jva, jvb, jvc, hu, hv;
+    int ru0, ru1, rv0, rv1;
+};
+// Views whose brick shadows a workgroup computes at once (one lane each).
+constexpr int kGatherViews = 64;
+
+// Staged dwords per shadow pixel, one record per pixel: the support bounds
+// at the brick centre and the inverse step (3 axes each), k* split in two
+// (read for every pixel of a voxel's window), then the closest-approach
+// offset and the step (3 axes each) and dl/dL_m * dx (read for the pixels
+// that touch the voxel).  An odd record length keeps the lanes' reads of
+// different records on different LDS banks.
+__host__ __device__ constexpr int gather_pixel_words(int m) { return (17 + m) | 1; }
+constexpr int kGLo = 0, kGHi = 3, kGId = 6, kGKf = 9, kGKr = 10, kGE = 11, kGD = 14, kGG = 17;
+
 template <int M>
 __global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
 {
     extern __shared__ char lds_raw[];
     const MarchArgs &P = A.P;
     const int cap = A.cap;
-    // staged arrays (structure of arrays, `cap` entries each)
-    float *s_e = (float *)(lds_raw + 256);      // 3 x cap: offset of the closest approach
-    float *s_d = s_e + 3 * cap;                  // 3 x cap: step per sample
-    float *s_id = s_d + 3 * cap;                 // 3 x cap: 1 / step
-    int *s_kf = (int *)(s_id + 3 * cap);         // cap: floor(k*)
-    float *s_kr = (float *)(s_kf + cap);         // cap: k* - floor(k*)
-    float *s_g = s_kr + cap;                     // M x cap: dl/dL_m * dx
-    int *s_rect = (int *)lds_raw;                // scratch: the brick's pixel rectangle
+    constexpr int PW = gather_pixel_words(M);
+    BrickView *s_view = (BrickView *)(lds_raw + 256); // kGatherViews entries
+    // staged pixel records, `cap` of them
+    float *s_px = (float *)(lds_raw + 256 + kGatherViews * sizeof(BrickView));
     const int bx = (int)(blockIdx.x % (uint32_t)A.bricks_x);
     const int by = (int)((blockIdx.x / (uint32_t)A.bricks_x) % (uint32_t)A.bricks_y);
     const int bz = (int)(blockIdx.x / ((uint32_t)A.bricks_x * (uint32_t)A.bricks_y));
@@ -1702,102 +1717,128 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
     float acc[M];
 #pragma unroll
     for (int m = 0; m < M; m++) acc[m] = 0.0f;
-    for (int vw = 0; vw < P.n_views; vw++) {
-        const int view = (int)(P.view_begin + vw);
-        // shadow of the brick: centre projection and the Jacobian per voxel
-        double u0, v0, ua, va, ub, vb, uc, vc;
-        project_point(P, view, ca, cb, cc, u0, v0);
-        project_point(P, view, ca + 1.0, cb, cc, ua, va);
-        project_point(P, view, ca, cb + 1.0, cc, ub, vb);
-        project_point(P, view, ca, cb, cc + 1.0, uc, vc);
-        const float Jua = (float)(ua - u0), Jub = (float)(ub - u0), Juc = (float)(uc - u0);
-        const float Jva = (float)(va - v0), Jvb = (float)(vb - v0), Jvc = (float)(vc - v0);
-        // a voxel's support cube spans +-1 voxel: +-h pixels around its centre
-        const float hu = fabsf(Jua) + fabsf(Jub) + fabsf(Juc) + 0.25f;
-        const float hv = fabsf(Jva) + fabsf(Jvb) + fabsf(Jvc) + 0.25f;
-        const float eu = fabsf(Jua) * (0.5f * kBrickX + 0.5f) + fabsf(Jub) * (0.5f * kBrickY + 0.5f) +
-                         fabsf(Juc) * (0.5f * kBrickZ + 0.5f) + 0.5f;
-        const float ev = fabsf(Jva) * (0.5f * kBrickX + 0.5f) + fabsf(Jvb) * (0.5f * kBrickY + 0.5f) +
-                         fabsf(Jvc) * (0.5f * kBrickZ + 0.5f) + 0.5f;
-        const int ru0 = max(0, (int)ceil(u0 - eu)), ru1 = min(P.nu - 1, (int)floor(u0 + eu));
-        const int rv0 = max(0, (int)ceil(v0 - ev)), rv1 = min(P.nv - 1, (int)floor(v0 + ev));
-        if (ru0 > ru1 || rv0 > rv1) continue; // block-uniform
-        // the voxel's own pixel window (inside the brick's)
-        const float cu = (float)u0 + Jua * ox + Jub * oy + Juc * oz;
-        const float cv = (float)v0 + Jva * ox + Jvb * oy + Jvc * oz;
-        const int pu0 = max(ru0, (int)ceilf(cu - hu)), pu1 = min(ru1, (int)floorf(cu + hu));
-        const int pv0 = max(rv0, (int)ceilf(cv - hv)), pv1 = min(rv1, (int)floorf(cv + hv));
-        // the brick's shadow in pieces of at most `cap` pixels: column
-        // segments no wider than cap, each in bands of whole rows
-        for (int c0 = ru0; c0 <= ru1; c0 += cap) {
-            const int c1 = min(ru1, c0 + cap - 1);
-            const int rw = c1 - c0 + 1;
-            const int band = max(1, cap / rw);
-            for (int b0 = rv0; b0 <= rv1; b0 += band) {
-                const int b1 = min(rv1, b0 + band - 1);
-                const int count = (b1 - b0 + 1) * rw;
-                __syncthreads(); // the previous band's entries are consumed
-                for (int i = threadIdx.x; i < count; i += kBlock) {
-                    const int pv = b0 + i / rw, pu = c0 + i % rw;
-                    const RayLine Ln = ray_line(P, pu, pv, (int)(P.view_begin + vw));
-                    // position at sample k: q0 + k * d, relative to the brick centre
-                    const double dxa = Ln.da * P.dx, dxb = Ln.db * P.dx, dxc = Ln.dc * P.dx;
-                    const double qa = Ln.sa + Ln.da * P.t0 - ca, qb = Ln.sb + Ln.db * P.t0 - cb;
-                    const double qc = Ln.sc + Ln.dc * P.t0 - cc;
-                    const double ks = -(qa * dxa + qb * dxb + qc * dxc) / (dxa * dxa + dxb * dxb + dxc * dxc);
-                    const double kf = floor(ks);
-                    s_e[i] = (float)(qa + ks * dxa);
-                    s_e[cap + i] = (float)(qb + ks * dxb);
-                    s_e[2 * cap + i] = (float)(qc + ks * dxc);
-                    const float da = (float)dxa, db = (float)dxb, dc = (float)dxc;
-                    s_d[i] = da;
-                    s_d[cap + i] = db;
-                    s_d[2 * cap + i] = dc;
-                    // a zero step leaves that axis unconstrained (or empty) through +-huge bounds
-                    s_id[i] = 1.0f / (fabsf(da) > 1e-30f ? da : 1e-30f);
-                    s_id[cap + i] = 1.0f / (fabsf(db) > 1e-30f ? db : 1e-30f);
-                    s_id[2 * cap + i] = 1.0f / (fabsf(dc) > 1e-30f ? dc : 1e-30f);
-                    s_kf[i] = (int)kf;
-                    s_kr[i] = (float)(ks - kf);
-                    const int64_t pix = (int64_t)pv * P.nu + pu;
-    #pragma unroll
-                    for (int m = 0; m < M; m++) s_g[m * cap + i] = A.gL[((int64_t)vw * M + m) * npix + pix];
-                }
-                __syncthreads();
-                if (!inside) continue;
-                const int q0 = max(pv0, b0), q1 = min(pv1, b1);
-                const int w0 = max(pu0, c0), w1 = min(pu1, c1);
-                for (int pv = q0; pv <= q1; pv++) {
-                    for (int pu = w0; pu <= w1; pu++) {
-                        const int i = (pv - b0) * rw + (pu - c0);
-                        const float fa = s_e[i] - ox, fb = s_e[cap + i] - oy, fc = s_e[2 * cap + i] - oz;
-                        const float da = s_d[i], db = s_d[cap + i], dc = s_d[2 * cap + i];
-                        const float ia = s_id[i], ib = s_id[cap + i], ic = s_id[2 * cap + i];
-                        // samples with |f + s d| < 1 on every axis, s = k - k*
-                        const float a0 = (-1.0f - fa) * ia, a1 = (1.0f - fa) * ia;
-                        const float b0_ = (-1.0f - fb) * ib, b1_ = (1.0f - fb) * ib;
-                        const float c0 = (-1.0f - fc) * ic, c1 = (1.0f - fc) * ic;
-                        const float slo = fmaxf(fmaxf(fminf(a0, a1), fminf(b0_, b1_)), fminf(c0, c1));
-                        const float shi = fminf(fminf(fmaxf(a0, a1), fmaxf(b0_, b1_)), fmaxf(c0, c1));
-                        if (!(shi > slo)) continue;
-                        const int kf = s_kf[i];
-                        const float kr = s_kr[i];
-                        // j = k - kf over the open interval, and 0 <= k < K
-                        const int j0 = max((int)ceilf(kr + slo), -kf);
-                        const int j1 = min((int)floorf(kr + shi), P.K - 1 - kf);
-                        if (j0 > j1) continue;
-                        float g[M];
-    #pragma unroll
-                        for (int m = 0; m < M; m++) g[m] = s_g[m * cap + i];
-                        float wsum = 0.0f;
-                        for (int j = j0; j <= j1; j++) {
-                            const float sj = (float)j - kr;
-                            const float ra = fmaf(sj, da, fa), rb = fmaf(sj, db, fb), rc = fmaf(sj, dc, fc);
-                            wsum += fmaxf(0.0f, 1.0f - fabsf(ra)) * fmaxf(0.0f, 1.0f - fabsf(rb)) *
-                                    fmaxf(0.0f, 1.0f - fabsf(rc));
+    for (int vt = 0; vt < P.n_views; vt += kGatherViews) {
+        const int nvt = min(kGatherViews, P.n_views - vt);
+        __syncthreads(); // the previous views' table is consumed
+        if ((int)threadIdx.x < nvt) {
+            // the brick's shadow in view vt + lane: centre projection and
+            // the Jacobian per voxel step (one lane per view)
+            const int view = (int)(P.view_begin + vt + threadIdx.x);
+            double u0, v0, ua, va, ub, vb, uc, vc;
+            project_point(P, view, ca, cb, cc, u0, v0);
+            project_point(P, view, ca + 1.0, cb, cc, ua, va);
+            project_point(P, view, ca, cb + 1.0, cc, ub, vb);
+            project_point(P, view, ca, cb, cc + 1.0, uc, vc);
+            BrickView bv;
+            bv.u0 = (float)u0;
+            bv.v0 = (float)v0;
+            bv.jua = (float)(ua - u0);
+            bv.jub = (float)(ub - u0);
+            bv.juc = (float)(uc - u0);
+            bv.jva = (float)(va - v0);
+            bv.jvb = (float)(vb - v0);
+            bv.jvc = (float)(vc - v0);
+            // a voxel's support cube spans +-1 voxel: +-h pixels around its
+            // centre (plus a quarter pixel for the FP32 centre and Jacobian)
+            bv.hu = fabsf(bv.jua) + fabsf(bv.jub) + fabsf(bv.juc) + 0.25f;
+            bv.hv = fabsf(bv.jva) + fabsf(bv.jvb) + fabsf(bv.jvc) + 0.25f;
+            const double eu = fabs(ua - u0) * (0.5 * kBrickX + 0.5) + fabs(ub - u0) * (0.5 * kBrickY + 0.5) +
+                              fabs(uc - u0) * (0.5 * kBrickZ + 0.5) + 0.5;
+            const double ev = fabs(va - v0) * (0.5 * kBrickX + 0.5) + fabs(vb - v0) * (0.5 * kBrickY + 0.5) +
+                              fabs(vc - v0) * (0.5 * kBrickZ + 0.5) + 0.5;
+            bv.ru0 = max(0, (int)ceil(u0 - eu));
+            bv.ru1 = min(P.nu - 1, (int)floor(u0 + eu));
+            bv.rv0 = max(0, (int)ceil(v0 - ev));
+            bv.rv1 = min(P.nv - 1, (int)floor(v0 + ev));
+            s_view[threadIdx.x] = bv;
+        }
+        __syncthreads();
+        for (int vw = vt; vw < vt + nvt; vw++) {
+            const BrickView bv = s_view[vw - vt];
+            const int ru0 = bv.ru0, ru1 = bv.ru1, rv0 = bv.rv0, rv1 = bv.rv1;
+            if (ru0 > ru1 || rv0 > rv1) continue; // block-uniform
+            // the voxel's own pixel window (inside the brick's)
+            const float cu = bv.u0 + bv.jua * ox + bv.jub * oy + bv.juc * oz;
+            const float cv = bv.v0 + bv.jva * ox + bv.jvb * oy + bv.jvc * oz;
+            const int pu0 = max(ru0, (int)ceilf(cu - bv.hu)), pu1 = min(ru1, (int)floorf(cu + bv.hu));
+            const int pv0 = max(rv0, (int)ceilf(cv - bv.hv)), pv1 = min(rv1, (int)floorf(cv + bv.hv));
+            // the brick's shadow in pieces of at most `cap` pixels: column
+            // segments no wider than cap, each in bands of whole rows
+            for (int c0 = ru0; c0 <= ru1; c0 += cap) {
+                const int c1 = min(ru1, c0 + cap - 1);
+                const int rw = c1 - c0 + 1;
+                const int band = max(1, cap / rw);
+                for (int b0 = rv0; b0 <= rv1; b0 += band) {
+                    const int b1 = min(rv1, b0 + band - 1);
+                    const int count = (b1 - b0 + 1) * rw;
+                    __syncthreads(); // the previous band's entries are consumed
+                    for (int i = threadIdx.x; i < count; i += kBlock) {
+                        const int pv = b0 + i / rw, pu = c0 + i % rw;
+                        const RayLine Ln = ray_line(P, pu, pv, (int)(P.view_begin + vw));
+                        // position at sample k: q0 + k * d, relative to the brick centre
+                        const double dxa = Ln.da * P.dx, dxb = Ln.db * P.dx, dxc = Ln.dc * P.dx;
+                        const double qa = Ln.sa + Ln.da * P.t0 - ca, qb = Ln.sb + Ln.db * P.t0 - cb;
+                        const double qc = Ln.sc + Ln.dc * P.t0 - cc;
+                        const double ks =
+                            -(qa * dxa + qb * dxb + qc * dxc) / (dxa * dxa + dxb * dxb + dxc * dxc);
+                        const double kf = floor(ks);
+                        const float e[3] = {(float)(qa + ks * dxa), (float)(qb + ks * dxb), (float)(qc + ks * dxc)};
+                        const float d[3] = {(float)dxa, (float)dxb, (float)dxc};
+                        float *rec = s_px + i * PW;
+#pragma unroll
+                        for (int a = 0; a < 3; a++) {
+                            // a zero step leaves that axis unconstrained (or
+                            // empty) through +-huge bounds
+                            const float id = 1.0f / (fabsf(d[a]) > 1e-30f ? d[a] : 1e-30f);
+                            // samples s = k - k* with |e - o + s d| < 1 lie
+                            // between (+-1 - e) / d + o / d: the bounds at o = 0
+                            const float l = (-1.0f - e[a]) * id, h = (1.0f - e[a]) * id;
+                            rec[kGLo + a] = fminf(l, h);
+                            rec[kGHi + a] = fmaxf(l, h);
+                            rec[kGId + a] = id;
+                            rec[kGE + a] = e[a];
+                            rec[kGD + a] = d[a];
                         }
-    #pragma unroll
-                        for (int m = 0; m < M; m++) acc[m] = fmaf(wsum, g[m], acc[m]);
+                        rec[kGKf] = __int_as_float((int)kf);
+                        rec[kGKr] = (float)(ks - kf);
+                        const int64_t pix = (int64_t)pv * P.nu + pu;
+#pragma unroll
+                        for (int m = 0; m < M; m++) rec[kGG + m] = A.gL[((int64_t)vw * M + m) * npix + pix];
+                    }
+                    __syncthreads();
+                    if (!inside) continue;
+                    const int q0 = max(pv0, b0), q1 = min(pv1, b1);
+                    const int w0 = max(pu0, c0), w1 = min(pu1, c1);
+                    for (int pv = q0; pv <= q1; pv++) {
+                        const float *rec = s_px + ((pv - b0) * rw + (w0 - c0)) * PW;
+                        for (int pu = w0; pu <= w1; pu++, rec += PW) {
+                            // the voxel's support in s: the centre bounds
+                            // shifted by o / d on each axis
+                            const float ia = rec[kGId], ib = rec[kGId + 1], ic = rec[kGId + 2];
+                            const float slo = fmaxf(fmaxf(fmaf(ox, ia, rec[kGLo]), fmaf(oy, ib, rec[kGLo + 1])),
+                                                    fmaf(oz, ic, rec[kGLo + 2]));
+                            const float shi = fminf(fminf(fmaf(ox, ia, rec[kGHi]), fmaf(oy, ib, rec[kGHi + 1])),
+                                                    fmaf(oz, ic, rec[kGHi + 2]));
+                            if (!(shi > slo)) continue;
+                            const int kf = __float_as_int(rec[kGKf]);
+                            const float kr = rec[kGKr];
+                            // j = k - kf over the open interval, and 0 <= k < K
+                            const int j0 = max((int)ceilf(kr + slo), -kf);
+                            const int j1 = min((int)floorf(kr + shi), P.K - 1 - kf);
+                            if (j0 > j1) continue;
+                            const float fa = rec[kGE] - ox, fb = rec[kGE + 1] - oy, fc = rec[kGE + 2] - oz;
+                            const float da = rec[kGD], db = rec[kGD + 1], dc = rec[kGD + 2];
+                            float wsum = 0.0f, sj = (float)j0 - kr;
+                            for (int j = j0; j <= j1; j++, sj += 1.0f) {
+                                // hat(r) = max(0, 1 - |r|), as one clamp
+                                const float ra = __builtin_amdgcn_fmed3f(1.0f - fabsf(fmaf(sj, da, fa)), 0.0f, 1.0f);
+                                const float rb = __builtin_amdgcn_fmed3f(1.0f - fabsf(fmaf(sj, db, fb)), 0.0f, 1.0f);
+                                const float rc = __builtin_amdgcn_fmed3f(1.0f - fabsf(fmaf(sj, dc, fc)), 0.0f, 1.0f);
+                                wsum = fmaf(ra * rb, rc, wsum);
+                            }
+#pragma unroll
+                            for (int m = 0; m < M; m++) acc[m] = fmaf(wsum, rec[kGG + m], acc[m]);
+                        }
                     }
                 }
             }
@@ -2509,7 +2550,8 @@ int ctfp_backward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, con
     const int64_t bricks_z = (s.p_num_z + kBrickZ - 1) / kBrickZ;
     const int64_t bricks = (int64_t)A.bricks_x * A.bricks_y * bricks_z;
     if (bricks >= (1ll << 31)) return set_error(CTFP_ERR_UNSUPPORTED, "volume too large for the gather");
-    A.cap = (gather_lds - 256) / ((11 + ctx->n_mat) * 4);
+    A.cap = (int32_t)((gather_lds - 256 - kGatherViews * (int)sizeof(BrickView)) /
+                      (gather_pixel_words(ctx->n_mat) * 4));
     for (int64_t v0 = 0; v0 < view_count; v0 += group) {
         const int64_t n = std::min(group, view_count - v0);
         MarchArgs Q = P;
