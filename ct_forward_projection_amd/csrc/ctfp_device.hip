@@ -1031,47 +1031,6 @@ __device__ __forceinline__ void block_box(int (&lo)[3], int (&hi)[3], int *scrat
     }
 }
 
-// Block-wide min of lo[] and max of hi[] with one barrier.  The scratch holds
-// two buffers (ints [24 p, 24 p + 24), p = parity) that calls use in turn: a
-// buffer is written again only two calls later, when every wave has passed
-// the barrier of the call in between and so has read it.  The caller must
-// not touch LDS that another wave may still read from before this barrier.
-template <int D>
-__device__ __forceinline__ void block_range(int (&lo)[D], int (&hi)[D], int *scratch, int parity)
-{
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-#pragma unroll
-        for (int s = 32; s >= 1; s >>= 1) {
-            lo[d] = min(lo[d], __shfl_xor(lo[d], s));
-            hi[d] = max(hi[d], __shfl_xor(hi[d], s));
-        }
-    }
-    int *buf = scratch + parity * 24;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) {
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-            buf[wave * 6 + d] = lo[d];
-            buf[wave * 6 + 3 + d] = hi[d];
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-        int a = buf[d], b = buf[3 + d];
-#pragma unroll
-        for (int w = 1; w < kBlock / 64; w++) {
-            a = min(a, buf[w * 6 + d]);
-            b = max(b, buf[w * 6 + 3 + d]);
-        }
-        // equal in every lane: scalar registers, so what follows from the
-        // box runs on the scalar unit
-        lo[d] = __builtin_amdgcn_readfirstlane(a);
-        hi[d] = __builtin_amdgcn_readfirstlane(b);
-    }
-}
-
 // Min / max over each 16-lane row, in every lane of the row (DPP quad
 // permutes and row mirrors fused into v_min / v_max; all lanes active).
 __device__ __forceinline__ int row_min(int v)
