@@ -203,6 +203,40 @@ def test_anisotropic_volume_and_material_counts(require_gpu, n_mat, dtype):
             assert torch.equal(a, b_)
 
 
+@pytest.mark.parametrize("shape", [(7, 5, 1), (1, 6, 4), (3, 1, 9)])
+@pytest.mark.parametrize("dtype", [P.CTFP_F32, P.CTFP_F16])
+def test_degenerate_volume_shapes(require_gpu, shape, dtype):
+    """Volumes one voxel thick along an axis (nx, ny, nz as given) with two
+    materials: every sample sits within a voxel of the zero padding, so the
+    box fill's out-of-volume rows and the border weights carry the result."""
+    import torch
+    nx, ny, nz = shape
+    rng = np.random.default_rng(nx * 100 + ny * 10 + nz)
+    vols = [rng.uniform(0.2, 1.5, size=(nz, ny, nx)).astype(np.float32) for _ in range(2)]
+    s = S.config2()
+    s.materials = (S.WATER, S.BONE)
+    s.n_vox = (nx, ny, nz)
+    s.length_mm = (nx * 2.0, ny * 2.5, nz * 1.5)
+    s.det, s.d_size, s.views = (40, 36), 0.3, 5
+    kw = P.spectral_inputs(s, *s.det)
+    got = run_projector(s, vols, spec_kw=kw, dtype=dtype)
+    ref = oracle_expect(s, vols, kw, 0, s.views)
+    assert float(np.abs(ref[1][2]).max()) > 0.0  # some ray crosses the slab
+    if dtype == P.CTFP_F32:
+        check_all(got, ref, name=f"degenerate_{nx}x{ny}x{nz}")
+    else:
+        check_mixed(got, ref[1], f"degenerate_{nx}x{ny}x{nz}_f16", "f16")
+    with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
+        proj.set_volumes(vols, storage_dtype=dtype)
+        proj.set_spectral(**kw)
+        lds = proj.forward(want_L=True)
+        proj.set_tuning(march="global")
+        glob = proj.forward(want_L=True)
+        torch.cuda.synchronize()
+        for a, b_ in zip(lds, glob):
+            assert torch.equal(a, b_)
+
+
 def test_labels_multi_material_literal_method(require_gpu):
     """Label volume through ctfp_set_labels vs the reference's literal method
     (per-energy mu(E) volume re-march), including labels above MaterialNum."""
