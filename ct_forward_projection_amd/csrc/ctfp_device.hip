@@ -819,16 +819,20 @@ __device__ __forceinline__ TileLane tile_lane(const MarchArgs &P, uint32_t logic
 // < 2e-7 there) and below it the Taylor series to tau^7 in Horner form
 // (remainder < tau^8 / 8! : 2e-8 relative at 0.35), so thin rays keep their
 // absorbed fraction without cancellation.
-__device__ __forceinline__ void attenuate(float tau, float &ex, float &ab)
+__device__ __forceinline__ float absorb_series(float tau)
 {
-    ex = __builtin_amdgcn_exp2f(-tau * 1.44269504088896341f);
     float q = fmaf(tau, 1.0f / 5040.0f, -1.0f / 720.0f);
     q = fmaf(tau, q, 1.0f / 120.0f);
     q = fmaf(tau, q, -1.0f / 24.0f);
     q = fmaf(tau, q, 1.0f / 6.0f);
     q = fmaf(tau, q, -0.5f);
     q = fmaf(tau, q, 1.0f);
-    ab = fabsf(tau) < 0.35f ? tau * q : 1.0f - ex;
+    return tau * q;
+}
+__device__ __forceinline__ void attenuate(float tau, float &ex, float &ab)
+{
+    ex = __builtin_amdgcn_exp2f(-tau * 1.44269504088896341f);
+    ab = fabsf(tau) < 0.35f ? absorb_series(tau) : 1.0f - ex;
 }
 
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
@@ -843,8 +847,9 @@ typedef float float16_t __attribute__((ext_vector_type(16)));
 // by 2^8 (P.Aq, [energy rows padded to 32][16] FP16) so A_lo stays normal.
 // Lane l (half h = l / 32, column c = l % 32) receives tau for rays c and
 // 32 + c and the energy rows 8b + 4h + q (b, q < 4) of each 32-row block, runs
-// the same exp / weighted FP64 sums as the per-lane loop for those, and one
-// exchange with lane l ^ 32 completes the sums of its own ray.  Every lane
+// the exponentials and the weighted sums for those (each group of four bins
+// in FP32, then one compensated add), and one exchange with lane l ^ 32
+// completes the sums of its own ray.  Every lane
 // of the wavefront must call it; `active` marks rays inside the detector.
 template <int M>
 __device__ __forceinline__ void spectrum_mfma(const MarchArgs &P, const TileLane &t, const float (&Lm)[M],
@@ -909,13 +914,32 @@ __device__ __forceinline__ void spectrum_mfma(const MarchArgs &P, const TileLane
                 float wv[4];
 #pragma unroll
                 for (int q = 0; q < 4; q++) wv[q] = we[q * P.w_stride_e];
+                float tau[4], ex[4], ab[4];
+                bool small = false;
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    float ex, ab;
-                    attenuate(d[4 * bb + q] * (1.0f / 256.0f), ex, ab);
-                    ksum_add(si, wv[q] * ex);
-                    ksum_add(sa, wv[q] * ab);
+                    tau[q] = d[4 * bb + q] * (1.0f / 256.0f);
+                    ex[q] = __builtin_amdgcn_exp2f(-tau[q] * 1.44269504088896341f);
+                    ab[q] = 1.0f - ex[q];
+                    small |= fabsf(tau[q]) < 0.35f;
                 }
+                // the series only when a lane of the wavefront needs it
+                // (a uniform branch; the same values as attenuate)
+                if (__builtin_amdgcn_ballot_w64(small) != 0) {
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        if (fabsf(tau[q]) < 0.35f) ab[q] = absorb_series(tau[q]);
+                }
+                // the four bins summed in FP32, then one compensated add each
+                // (a few ulps of the total, as for bin-by-bin compensation)
+                float gI = wv[0] * ex[0], gA = wv[0] * ab[0];
+#pragma unroll
+                for (int q = 1; q < 4; q++) {
+                    gI = fmaf(wv[q], ex[q], gI);
+                    gA = fmaf(wv[q], ab[q], gA);
+                }
+                ksum_add(si, gI);
+                ksum_add(sa, gA);
                 we += 8 * P.w_stride_e;
             }
         }
