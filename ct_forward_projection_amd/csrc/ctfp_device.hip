@@ -1151,13 +1151,14 @@ __device__ __forceinline__ float small_rcp(int d) { return __builtin_amdgcn_rcpf
 constexpr int kFillBatch = 4;
 
 // Cooperative, row-coalesced load of the cell box [lo, lo + (sx, sy, sz)) into
-// LDS (cell i of the box at byte i * LB, x fastest).  Cells outside the
-// padded volume are zeros.  A thread keeps one x column and walks rows
-// (y, z) with a fixed stride, so the x test and offset are hoisted and the
-// loop holds no division.
+// LDS (cell (lx, row) of the box at byte (row * px + lx) * LB, x fastest;
+// row pitch px >= sx).  Cells outside the padded volume are zeros.  A thread
+// keeps one x column and walks rows (y, z) with a fixed stride, so the x test
+// and offset are hoisted and the loop holds no division.
 template <int M>
 __device__ __forceinline__ void fill_box(__amdgpu_buffer_rsrc_t rsrc, char *lds, const int (&lo)[3], int sx,
-                                         int sy, int sz, int nx, int ny, int nz, uint32_t oy, uint32_t oz)
+                                         int px, int sy, int sz, int nx, int ny, int nz, uint32_t oy,
+                                         uint32_t oz)
 {
     constexpr uint32_t LB = lds_cell_bytes<M>();
     const int rows = sy * sz;
@@ -1169,14 +1170,15 @@ __device__ __forceinline__ void fill_box(__amdgpu_buffer_rsrc_t rsrc, char *lds,
         const int tid = threadIdx.x;
         if (tid >= per * sx) return;
         const int r0 = small_div(tid, sx, inv_sx);
-        const int x = lo[0] + (tid - __mul24(r0, sx));
+        const int lx = tid - __mul24(r0, sx);
+        const int x = lo[0] + lx;
         int lz = small_div(r0, sy, inv_sy);
         int ly = r0 - __mul24(lz, sy);
         const bool xin = (uint32_t)(x + kVoxPadLo) < (uint32_t)(nx + kVoxPadLo + kVoxPadHi);
         const uint32_t xoff = ((uint32_t)(x + kVoxPadLo) & 0xFFFFFu) * (4u * M);
         const uint32_t ymax = (uint32_t)(ny + kVoxPadLo + kVoxPadHi), zmax = (uint32_t)(nz + kVoxPadLo + kVoxPadHi);
-        uint32_t loff = (uint32_t)tid * LB;
-        const uint32_t lstep = (uint32_t)(per * sx) * LB;
+        uint32_t loff = (uint32_t)(__mul24(r0, px) + lx) * LB;
+        const uint32_t lstep = (uint32_t)(per * px) * LB;
         // kFillBatch rows per pass of the loop: their loads are all in flight
         // before the first LDS write.  A row outside the padded volume (or
         // past the box) loads the padded cell at offset 0, which is a zero.
@@ -1244,7 +1246,56 @@ __device__ __forceinline__ void fill_box(__amdgpu_buffer_rsrc_t rsrc, char *lds,
 #pragma unroll
                 for (int m = 0; m < M; m++) d[m] = 0u;
             }
-            lds_write<M>(lds, (uint32_t)i * LB, d);
+            lds_write<M>(lds, (uint32_t)(row * px + lx) * LB, d);
+        }
+    }
+}
+
+// a * b + c for |a|, |b| < 2^23 (v_mad_i32_i24, one full-rate instruction;
+// written out because the compiler otherwise emits two multiplies and a
+// three-way add for the cell index).
+__device__ __forceinline__ int mad_i24(int a, int b, int c)
+{
+    int r;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// The samples of one chunk from its staged box (row pitch PX cells, or px
+// when PX is 0; plane pitch pitch * sy).  A sample's cell offset from the
+// anchor's cell becomes one cell index and then bytes; the y + 1 rows lie
+// one row pitch further (an immediate offset when PX is fixed) and the
+// z + 1 rows one plane further.
+template <int M, int ST, int PX>
+__device__ __forceinline__ void march_staged(const RaySetup &R, const Anchor &A0, FixPos q, const FixPos &d16,
+                                             int k0, int chunk, int c_lo, int c_hi, const int (&lo)[3], int px,
+                                             int sy, const char *lds, float (&acc)[M])
+{
+    constexpr uint32_t LB = lds_cell_bytes<M>();
+    const int pitch = PX > 0 ? PX : px;
+    const int plane = pitch * sy;
+    const uint32_t ry = (uint32_t)pitch * LB, rz = (uint32_t)plane * LB;
+    for (int ka = k0; ka < k0 + chunk; ka += kAnchor, fix_add(q, d16)) {
+        const int i1 = min(ka + kAnchor, c_hi) - ka;
+        int i = max(ka, c_lo) - ka;
+        if (i >= i1) continue;
+        const Anchor A = ka == k0 ? A0 : anchor_from(q, R);
+        // the anchor's cell in the box
+        const char *pa = lds + (((A.ic - lo[2]) * sy + (A.ib - lo[1])) * pitch + (A.ia - lo[0])) * (int)LB;
+        auto sample = [&](float fi) {
+            const Cell c = offset_cell(A, fi);
+            const int ci = mad_i24(c.z, plane, mad_i24(c.y, pitch, c.x));
+            const char *p0 = pa + ci * (int)LB;
+            const char *p2 = p0 + rz;
+            accumulate<M, ST>(c, [&](int r, uint32_t(&out)[M]) {
+                lds_read<M>((r & 2) ? p2 : p0, ((r & 1) ? ry : 0u) + (r < 4 ? 0u : LB), out);
+            }, acc);
+        };
+        if (i == 0 && i1 == kAnchor) { // a whole segment (most of them)
+#pragma unroll
+            for (int s = 0; s < kAnchor; s++) sample((float)s);
+        } else {
+            for (float fi = (float)i; i < i1; i++, fi += 1.0f) sample(fi);
         }
     }
 }
@@ -1261,7 +1312,6 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
                                              __amdgpu_buffer_rsrc_t rsrc, char *lds, int *scratch,
                                              float (&acc)[M])
 {
-    constexpr uint32_t LB = lds_cell_bytes<M>();
     const uint32_t oy = P.row_y & 0xFFFFFFu, oz = P.row_z & 0xFFFFFFu;
     const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
     const int k_hi = R.k_lo + R.n;
@@ -1307,34 +1357,16 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
         const int sx = hi[0] - lo[0] + (ST == kStF32 ? 1 : 0), sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
         const bool staged = (int64_t)sx * sy * sz <= (int64_t)P.lds_cells;
         if (staged) {
-            fill_box<M>(rsrc, lds, lo, sx, sy, sz, nx, ny, nz, oy, oz);
+            // Row pitch: 12 cells for boxes up to 12 wide when that still
+            // fits (then the y + 1 rows are an immediate LDS offset away),
+            // sx otherwise (block-uniform).
+            const int px = sx <= 12 && (int64_t)12 * sy * sz <= (int64_t)P.lds_cells ? 12 : sx;
+            fill_box<M>(rsrc, lds, lo, sx, px, sy, sz, nx, ny, nz, oy, oz);
             __syncthreads();
-            const int ry = sx * (int)LB, rz = sx * sy * (int)LB;
-            const uint32_t roff[4] = {0u, (uint32_t)ry, (uint32_t)rz, (uint32_t)(ry + rz)};
-            FixPos q = pk;
-            for (int ka = k0; ka < k0 + P.chunk; ka += kAnchor, fix_add(q, d16)) {
-                const int i1 = min(ka + kAnchor, c_hi) - ka;
-                int i = max(ka, c_lo) - ka;
-                if (i >= i1) continue;
-                const Anchor A = ka == k0 ? A0 : anchor_from(q, R);
-                // byte offset of the anchor's cell in the box; samples add
-                // their (small, signed) cell offsets with 24-bit multiplies
-                const char *pa =
-                    lds + (((A.ic - lo[2]) * sy + (A.ib - lo[1])) * sx + (A.ia - lo[0])) * (int)LB;
-                auto sample = [&](float fi) {
-                    const Cell c = offset_cell(A, fi);
-                    const char *pc = pa + (c.x * (int)LB + __mul24(c.y, ry) + __mul24(c.z, rz));
-                    accumulate<M, ST>(c, [&](int r, uint32_t(&out)[M]) {
-                        lds_read<M>(pc, roff[r & 3] + (r < 4 ? 0u : LB), out);
-                    }, acc);
-                };
-                if (i == 0 && i1 == kAnchor) { // a whole segment (most of them)
-#pragma unroll
-                    for (int q = 0; q < kAnchor; q++) sample((float)q);
-                } else {
-                    for (float fi = (float)i; i < i1; i++, fi += 1.0f) sample(fi);
-                }
-            }
+            if (px == 12)
+                march_staged<M, ST, 12>(R, A0, pk, d16, k0, P.chunk, c_lo, c_hi, lo, px, sy, lds, acc);
+            else
+                march_staged<M, ST, 0>(R, A0, pk, d16, k0, P.chunk, c_lo, c_hi, lo, px, sy, lds, acc);
         } else {
             FixPos q = pk;
             for (int ka = k0; ka < k0 + P.chunk; ka += kAnchor, fix_add(q, d16)) {
