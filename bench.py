@@ -416,15 +416,15 @@ def main():
                      "traffic": traffic},
     }
     if args.dtype != "f32" and not args.no_fp32:
-        # the FP32 projector with the same launch options, and the plain FP32
-        # projector (every corner gathered from HBM/L2, no LDS staging)
+        # the LDS-staged FP32 projector with the same launch options, and the
+        # plain FP32 projector (every corner gathered from HBM/L2, no staging)
         ref = P.Projector(P.scan_params(**setup.scan_kwargs()))
         ref.set_volumes(vols, storage_dtype=P.CTFP_F32)
         ref.set_spectral(**kw)
         ref.set_tuning(**tune)
         wall32, _ = timed(ref, "f32")
-        result["fp32_value"] = round(samples / (wall32 / args.steps) / 1e9, 3)
-        result["speedup_vs_fp32"] = round(result["value"] / result["fp32_value"], 3)
+        result["staged_fp32_value"] = round(samples / (wall32 / args.steps) / 1e9, 3)
+        result["speedup_vs_staged_fp32"] = round(result["value"] / result["staged_fp32_value"], 3)
         ref.set_tuning(march="global")
         wallg, _ = timed(ref, "f32 global")
         result["plain_fp32_value"] = round(samples / (wallg / args.steps) / 1e9, 3)

@@ -344,6 +344,12 @@ def test_errors_without_inputs(require_gpu):
         proj.forward()                       # no spectral weights
     with pytest.raises(P.CTFPError):
         proj.forward(1, 5)                   # view range outside the scan
+    with pytest.raises(P.CTFPError):
+        proj.forward(0, 0)                   # empty view range
+    with pytest.raises(P.CTFPError):
+        proj.forward(-1, 1)                  # negative first view
+    with pytest.raises(P.CTFPError):
+        proj.line_integrals(0, 0)            # empty view range
     kw = P.spectral_inputs(s, *s.det)
     kw["atten"] = kw["atten"][:, :2]
     with pytest.raises(P.CTFPError):
