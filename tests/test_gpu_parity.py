@@ -169,7 +169,7 @@ def test_single_view_and_negative_rotation(require_gpu):
 
 
 @pytest.mark.parametrize("n_mat", [1, 2, 4])
-@pytest.mark.parametrize("dtype", [P.CTFP_F32, P.CTFP_F16])
+@pytest.mark.parametrize("dtype", [P.CTFP_F32, P.CTFP_F16, P.CTFP_BF16])
 def test_anisotropic_volume_and_material_counts(require_gpu, n_mat, dtype):
     """A non-cubic volume (40 x 32 x 24) with a different voxel pitch on each
     axis (0.6 / 0.45 / 0.8 mm, so samples step 0.5 / 0.67 / 0.38 voxel), and
@@ -191,7 +191,8 @@ def test_anisotropic_volume_and_material_counts(require_gpu, n_mat, dtype):
     if dtype == P.CTFP_F32:
         check_all(got, ref, name=f"anisotropic_m{n_mat}")
     else:
-        check_mixed(got, ref[1], f"anisotropic_m{n_mat}_f16", "f16")
+        kind = "f16" if dtype == P.CTFP_F16 else "bf16"
+        check_mixed(got, ref[1], f"anisotropic_m{n_mat}_{kind}", kind)
     with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
         proj.set_volumes(vols, storage_dtype=dtype)
         proj.set_spectral(**kw)
