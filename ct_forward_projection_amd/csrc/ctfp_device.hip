@@ -288,8 +288,9 @@ __global__ void copy_kernel(const float4 *__restrict__ src, float4 *__restrict__
 struct MarchArgs {
     // volume (padded, interleaved, bytes)
     const void *vol;
-    uint32_t vol_bytes;
+    uint32_t vol_bytes;    // buffer-load range (all of it unless wide)
     uint32_t row_y, row_z; // byte strides of one y row / one z slice
+    int32_t wide;          // the volume spans 4 GiB or more
     float nx, ny, nz;      // clamp limits (= N per axis)
     // geometry (FP64, reference scalars)
     double sod, sdd, dsize, half_u, half_v, t0, dx;
@@ -532,6 +533,16 @@ __device__ __forceinline__ void load_raw(__amdgpu_buffer_rsrc_t r, uint32_t voff
     }
 }
 
+// The M dwords of one cell through a 64-bit address (global loads; a cell
+// is dword-aligned), for volumes past the 4 GiB buffer-load range.
+template <int M>
+__device__ __forceinline__ void load_glob(const char *p, uint32_t (&c)[M])
+{
+    const uint32_t *q = (const uint32_t *)p;
+#pragma unroll
+    for (int m = 0; m < M; m++) c[m] = q[m];
+}
+
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 typedef short bf16x2_t __attribute__((ext_vector_type(2)));
 
@@ -730,9 +741,9 @@ __device__ __forceinline__ void accumulate(const Cell &c, Fetch fetch, float (&a
 }
 
 // Global-memory gather of one sample (raw buffer loads; one VGPR offset).
-template <int M, int ST>
-__device__ __forceinline__ void accumulate_global(__amdgpu_buffer_rsrc_t rsrc, const Cell &c, uint32_t oy,
-                                                  uint32_t oz, float (&acc)[M])
+template <int M, int ST, bool W>
+__device__ __forceinline__ void accumulate_global(__amdgpu_buffer_rsrc_t rsrc, const char *vol, const Cell &c,
+                                                  uint32_t oy, uint32_t oz, float (&acc)[M])
 {
     constexpr uint32_t CB = 4u * M;
     // Byte offset with full-rate 24-bit multiply-adds: the masks (no-ops on
@@ -741,11 +752,20 @@ __device__ __forceinline__ void accumulate_global(__amdgpu_buffer_rsrc_t rsrc, c
     const uint32_t xi = (uint32_t)(c.x + kVoxPadLo) & 0xFFFFFu;
     const uint32_t yi = (uint32_t)(c.y + kVoxPadLo) & 0xFFFFFu;
     const uint32_t zi = (uint32_t)(c.z + kVoxPadLo) & 0xFFFFFu;
-    const uint32_t voff = zi * oz + (yi * oy + xi * CB);
     const uint32_t soff[4] = {0u, oy, oz, oy + oz};
-    accumulate<M, ST>(c, [&](int r, uint32_t(&out)[M]) {
-        load_raw<M>(rsrc, r < 4 ? voff : voff + CB, soff[r & 3], out);
-    }, acc);
+    if constexpr (W) {
+        // a volume of 4 GiB or more: a 64-bit z offset; within a slice
+        // (< 16 MiB) the y and x offsets stay 32-bit
+        const char *p = vol + ((uint64_t)zi * oz + (yi * oy + xi * CB));
+        accumulate<M, ST>(c, [&](int r, uint32_t(&out)[M]) {
+            load_glob<M>(p + soff[r & 3] + (r < 4 ? 0u : CB), out);
+        }, acc);
+    } else {
+        const uint32_t voff = zi * oz + (yi * oy + xi * CB);
+        accumulate<M, ST>(c, [&](int r, uint32_t(&out)[M]) {
+            load_raw<M>(rsrc, r < 4 ? voff : voff + CB, soff[r & 3], out);
+        }, acc);
+    }
 }
 
 // Bytes of one staged cell in LDS: M dwords, three padded to four so every
@@ -1012,21 +1032,15 @@ __device__ __forceinline__ uint32_t xcd_logical(const MarchArgs &P)
     return (bid & 7u) * P.per_xcd + (bid >> 3);
 }
 
-// Ray-march with direct global gathers: 8 (FP32) or 4 (16-bit) loads per sample.
-template <int M, int ST, bool kSpectrum>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void march_kernel(MarchArgs P)
+// One ray's samples, every corner gathered from the volume in HBM / L2 (W:
+// the volume spans 4 GiB or more).
+template <int M, int ST, bool W>
+__device__ __forceinline__ void march_global(const MarchArgs &P, const RaySetup &R, float (&acc)[M])
 {
-    const uint32_t logical = xcd_logical(P);
-    if (logical >= P.n_blocks) return;
-    const TileLane t = tile_lane(P, logical);
-    const RaySetup R = t.active ? setup_ray(P, t.u, t.v, (int)(P.view_begin + t.vw_local)) : empty_ray();
     __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(P.vol), 0, (int)P.vol_bytes, 0x00020000);
     const uint32_t oy = P.row_y & 0xFFFFFFu, oz = P.row_z & 0xFFFFFFu;
     const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
-    float acc[M];
-#pragma unroll
-    for (int m = 0; m < M; m++) acc[m] = 0.0f;
     const int k_hi = R.k_lo + R.n;
     const FixPos d16 = fix_steps(R, kAnchor);
     FixPos q = fix_at(R, R.k_lo & ~(kAnchor - 1));
@@ -1035,8 +1049,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const int i1 = min(ka + kAnchor, k_hi) - ka;
         int i = max(ka, R.k_lo) - ka;
         for (float fi = (float)i; i < i1; i++, fi += 1.0f)
-            accumulate_global<M, ST>(rsrc, absolute_cell(A, offset_cell(A, fi), nx, ny, nz), oy, oz, acc);
+            accumulate_global<M, ST, W>(rsrc, (const char *)P.vol, absolute_cell(A, offset_cell(A, fi), nx, ny, nz),
+                                        oy, oz, acc);
     }
+}
+
+// Ray-march with direct global gathers: 8 (FP32) or 4 (16-bit) loads per sample.
+template <int M, int ST, bool kSpectrum>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void march_kernel(MarchArgs P)
+{
+    const uint32_t logical = xcd_logical(P);
+    if (logical >= P.n_blocks) return;
+    const TileLane t = tile_lane(P, logical);
+    const RaySetup R = t.active ? setup_ray(P, t.u, t.v, (int)(P.view_begin + t.vw_local)) : empty_ray();
+    float acc[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) acc[m] = 0.0f;
+    if (P.wide)
+        march_global<M, ST, true>(P, R, acc);
+    else
+        march_global<M, ST, false>(P, R, acc);
     finish_ray<M, ST, kSpectrum>(P, t, acc);
 }
 
@@ -1155,13 +1187,24 @@ constexpr int kFillBatch = 4;
 // row pitch px >= sx).  Cells outside the padded volume are zeros.  A thread
 // keeps one x column and walks rows (y, z) with a fixed stride, so the x test
 // and offset are hoisted and the loop holds no division.
-template <int M>
-__device__ __forceinline__ void fill_box(__amdgpu_buffer_rsrc_t rsrc, char *lds, const int (&lo)[3], int sx,
-                                         int px, int sy, int sz, int nx, int ny, int nz, uint32_t oy,
-                                         uint32_t oz)
+template <int M, bool W>
+__device__ __forceinline__ void fill_box(__amdgpu_buffer_rsrc_t rsrc, const char *vol, char *lds,
+                                         const int (&lo)[3], int sx, int px, int sy, int sz, int nx, int ny,
+                                         int nz, uint32_t oy, uint32_t oz)
 {
     constexpr uint32_t LB = lds_cell_bytes<M>();
     const int rows = sy * sz;
+    // A volume of 4 GiB or more (W): the buffer starts at plane zb, the box's
+    // first plane inside the padded volume, so every row offset stays
+    // 32-bit; offset 0 is still a padding (zero) cell.
+    uint32_t zb = 0;
+    if constexpr (W) {
+        const int pzn = nz + kVoxPadLo + kVoxPadHi;
+        zb = (uint32_t)min(max(lo[2] + kVoxPadLo, 0), pzn - 1);
+        const uint64_t rest = (uint64_t)(pzn - (int)zb) * oz;
+        rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(vol) + (uint64_t)zb * oz, 0,
+                                                 (int)(rest < 0xFFFFFFFFull ? rest : 0xFFFFFFFFull), 0x00020000);
+    }
     if (sx <= kBlock) {
         const float inv_sx = small_rcp(sx), inv_sy = small_rcp(sy);
         // rows per pass and its (y, z) step: block-uniform
@@ -1185,7 +1228,7 @@ __device__ __forceinline__ void fill_box(__amdgpu_buffer_rsrc_t rsrc, char *lds,
         const bool inside = lo[0] >= -kVoxPadLo && lo[0] + sx <= nx + kVoxPadHi && lo[1] >= -kVoxPadLo &&
                             lo[1] + sy <= ny + kVoxPadHi && lo[2] >= -kVoxPadLo && lo[2] + sz <= nz + kVoxPadHi;
         if (inside) { // block-uniform: no row needs a bounds test
-            uint32_t off = __umul24((uint32_t)(lo[2] + lz + kVoxPadLo), oz) +
+            uint32_t off = __umul24((uint32_t)(lo[2] + lz + kVoxPadLo) - zb, oz) +
                            (__umul24((uint32_t)(lo[1] + ly + kVoxPadLo), oy) + xoff);
             const uint32_t step = (uint32_t)dy * oy + (uint32_t)dz * oz, wrap = oz - (uint32_t)sy * oy;
             for (int r = r0; r < rows; r += kFillBatch * per) {
@@ -1214,7 +1257,7 @@ __device__ __forceinline__ void fill_box(__amdgpu_buffer_rsrc_t rsrc, char *lds,
             for (int b = 0; b < kFillBatch; b++) {
                 const uint32_t yi = (uint32_t)(lo[1] + ly + kVoxPadLo), zi = (uint32_t)(lo[2] + lz + kVoxPadLo);
                 const bool in = xin && yi < ymax && zi < zmax;
-                load_raw<M>(rsrc, in ? __umul24(zi, oz) + (__umul24(yi, oy) + xoff) : 0u, 0, d[b]);
+                load_raw<M>(rsrc, in ? __umul24(zi - zb, oz) + (__umul24(yi, oy) + xoff) : 0u, 0, d[b]);
                 ly += dy;
                 lz += dz;
                 if (ly >= sy) {
@@ -1241,7 +1284,7 @@ __device__ __forceinline__ void fill_box(__amdgpu_buffer_rsrc_t rsrc, char *lds,
                 const uint32_t xi = (uint32_t)(x + kVoxPadLo) & 0xFFFFFu;
                 const uint32_t yi = (uint32_t)(y + kVoxPadLo) & 0xFFFFFu;
                 const uint32_t zi = (uint32_t)(z + kVoxPadLo) & 0xFFFFFu;
-                load_raw<M>(rsrc, zi * oz + (yi * oy + xi * (4u * M)), 0, d);
+                load_raw<M>(rsrc, (zi - zb) * oz + (yi * oy + xi * (4u * M)), 0, d);
             } else {
 #pragma unroll
                 for (int m = 0; m < M; m++) d[m] = 0u;
@@ -1307,7 +1350,7 @@ __device__ __forceinline__ void march_staged(const RaySetup &R, const Anchor &A0
 // box exceeds the LDS budget reads from global memory instead.  Each ray still
 // accumulates in ascending k, so the result equals march_kernel's bit for bit.
 // Every thread of the block must call it (it synchronises the block).
-template <int M, int ST>
+template <int M, int ST, bool W>
 __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup &R,
                                              __amdgpu_buffer_rsrc_t rsrc, char *lds, int *scratch,
                                              float (&acc)[M])
@@ -1355,13 +1398,16 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
         if (hi[0] < lo[0]) continue; // no lane of the block samples this chunk (block-uniform)
         // 16-bit cells are x-pairs that already hold the +1 x corner
         const int sx = hi[0] - lo[0] + (ST == kStF32 ? 1 : 0), sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
-        const bool staged = (int64_t)sx * sy * sz <= (int64_t)P.lds_cells;
+        // (a box of a volume past 4 GiB is staged only if its planes span
+        // less than 4 GiB, so its row offsets fit 32 bits)
+        const bool staged = (int64_t)sx * sy * sz <= (int64_t)P.lds_cells &&
+                            (!W || (uint64_t)sz * oz < 0xFFFFFFFFull);
         if (staged) {
             // Row pitch: 12 cells for boxes up to 12 wide when that still
             // fits (then the y + 1 rows are an immediate LDS offset away),
             // sx otherwise (block-uniform).
             const int px = sx <= 12 && (int64_t)12 * sy * sz <= (int64_t)P.lds_cells ? 12 : sx;
-            fill_box<M>(rsrc, lds, lo, sx, px, sy, sz, nx, ny, nz, oy, oz);
+            fill_box<M, W>(rsrc, (const char *)P.vol, lds, lo, sx, px, sy, sz, nx, ny, nz, oy, oz);
             __syncthreads();
             if (px == 12)
                 march_staged<M, ST, 12>(R, A0, pk, d16, k0, P.chunk, c_lo, c_hi, lo, px, sy, lds, acc);
@@ -1375,8 +1421,8 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
                 if (i >= i1) continue;
                 const Anchor A = ka == k0 ? A0 : anchor_from(q, R);
                 for (float fi = (float)i; i < i1; i++, fi += 1.0f)
-                    accumulate_global<M, ST>(rsrc, absolute_cell(A, offset_cell(A, fi), nx, ny, nz), oy,
-                                             oz, acc);
+                    accumulate_global<M, ST, W>(rsrc, (const char *)P.vol,
+                                                absolute_cell(A, offset_cell(A, fi), nx, ny, nz), oy, oz, acc);
             }
         }
     }
@@ -1399,7 +1445,10 @@ __device__ __forceinline__ void march_tile(const MarchArgs &P, uint32_t logical,
     float acc[M];
 #pragma unroll
     for (int m = 0; m < M; m++) acc[m] = 0.0f;
-    march_chunks<M, ST>(P, R, rsrc, lds_raw + 256, (int *)lds_raw, acc);
+    if (P.wide)
+        march_chunks<M, ST, true>(P, R, rsrc, lds_raw + 256, (int *)lds_raw, acc);
+    else
+        march_chunks<M, ST, false>(P, R, rsrc, lds_raw + 256, (int *)lds_raw, acc);
     finish_ray<M, ST, kSpectrum>(P, t, acc);
 }
 
@@ -1636,7 +1685,10 @@ __device__ __forceinline__ void ray_gradient(const MarchArgs &P, const AdjointAr
         float acc[M];
 #pragma unroll
         for (int m = 0; m < M; m++) acc[m] = 0.0f;
-        march_chunks<M, ST>(P, R, rsrc, lds_raw + 256, (int *)lds_raw, acc);
+        if (P.wide)
+            march_chunks<M, ST, true>(P, R, rsrc, lds_raw + 256, (int *)lds_raw, acc);
+        else
+            march_chunks<M, ST, false>(P, R, rsrc, lds_raw + 256, (int *)lds_raw, acc);
 #pragma unroll
         for (int m = 0; m < M; m++) Lm[m] = acc[m] * P.dx_f;
     }
@@ -2006,9 +2058,6 @@ int alloc_volume(ctfp_context *ctx, int n_mat)
     size_t bytes = (size_t)(px * py * pz) * (size_t)n_mat * sizeof(float);
     if ((size_t)(px * py) * n_mat * sizeof(float) >= (1u << 24))
         return set_error(CTFP_ERR_UNSUPPORTED, "one padded z slice exceeds 16 MiB");
-    if (bytes >= 0xFFFFFFFFull)
-        return set_error(CTFP_ERR_UNSUPPORTED,
-                         "padded volume of %zu bytes exceeds the 4 GiB buffer-load range", bytes);
     if (ctx->vol && (ctx->vol_bytes != bytes)) dfree(ctx->vol);
     if (!ctx->vol) HIP_TRY(hipMalloc(&ctx->vol, bytes));
     ctx->vol_bytes = bytes;
@@ -2040,7 +2089,10 @@ int fill_march_args(ctfp_context *ctx, int64_t view_begin, int64_t view_count, M
     std::memset(&P, 0, sizeof P);
     const uint32_t VB = (uint32_t)(ctx->n_mat * sizeof(float));
     P.vol = ctx->vol;
-    P.vol_bytes = (uint32_t)ctx->vol_bytes;
+    // volumes of 4 GiB and more read through per-box buffers and 64-bit
+    // addresses; below that, one buffer covers the volume
+    P.wide = ctx->vol_bytes >= 0xFFFFFFFFull ? 1 : 0;
+    P.vol_bytes = P.wide ? 0xFFFFFFFFu : (uint32_t)ctx->vol_bytes;
     P.row_y = (uint32_t)(ctx->px * VB);
     P.row_z = (uint32_t)(ctx->px * ctx->py * VB);
     P.nx = (float)s.p_num_x;

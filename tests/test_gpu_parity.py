@@ -238,6 +238,51 @@ def test_degenerate_volume_shapes(require_gpu, shape, dtype):
             assert torch.equal(a, b_)
 
 
+@pytest.fixture(scope="module")
+def wide_volume():
+    """2040 x 2040 x 520 voxels at 0.1 mm, one material: the padded layout
+    (2044 x 2044 x 524 x 4 B = 8.76 GB) spans more than 4 GiB, and planes
+    z >= 255 lie past it.  Zero except a seeded block around the centre that
+    straddles that plane."""
+    vol = np.zeros((520, 2040, 2040), dtype=np.float32)
+    rng = np.random.default_rng(12)
+    vol[230:290, 990:1050, 990:1050] = rng.uniform(0.1, 1.5, size=(60, 60, 60)).astype(np.float32)
+    return vol
+
+
+@pytest.mark.parametrize("dtype", [P.CTFP_F32, P.CTFP_F16])
+def test_volume_past_4gib(require_gpu, wide_volume, dtype):
+    """A volume past the 4 GiB buffer-load range: dense central rays (0.5
+    voxel apart, so chunks are LDS-staged) cross the plane at 4 GiB.  The
+    FP64 evaluation to TOL (FP32) or the FP16 bound, and the staged march
+    equal to the 64-bit global gather bit for bit."""
+    import torch
+    nz, ny, nx = wide_volume.shape
+    assert (nx + 4) * (ny + 4) * (nz + 4) * 4 > 2 ** 32
+    s = S.config2()
+    s.materials = (S.WATER,)
+    s.n_vox = (nx, ny, nz)
+    s.length_mm = (nx * 0.1, ny * 0.1, nz * 0.1)
+    s.det, s.d_size, s.views = (32, 48), 0.075, 10
+    kw = P.spectral_inputs(s, *s.det)
+    with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
+        proj.set_volumes([wide_volume], storage_dtype=dtype)
+        proj.set_spectral(**kw)
+        I, IA, L = proj.forward(0, 2, want_L=True)
+        I0 = proj.flat_field()
+        proj.set_tuning(march="global")
+        Ig, IAg, Lg = proj.forward(0, 2, want_L=True)
+        torch.cuda.synchronize()
+        assert torch.equal(I, Ig) and torch.equal(IA, IAg) and torch.equal(L, Lg)
+        got = to_np(I), to_np(IA), to_np(L), to_np(I0)
+    ref = oracle_expect(s, [wide_volume], kw, 0, 2)
+    assert float(np.abs(ref[1][2]).min()) > 0.0  # every ray crosses the block
+    if dtype == P.CTFP_F32:
+        check_all(got, ref, name="volume_past_4gib")
+    else:
+        check_mixed(got, ref[1], "volume_past_4gib_f16", "f16")
+
+
 def test_labels_multi_material_literal_method(require_gpu):
     """Label volume through ctfp_set_labels vs the reference's literal method
     (per-energy mu(E) volume re-march), including labels above MaterialNum."""
