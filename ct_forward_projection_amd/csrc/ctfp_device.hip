@@ -1435,7 +1435,9 @@ __device__ __forceinline__ RaySetup lane_ray(const MarchArgs &P, const TileLane 
 
 // One ray tile (logical block `logical` of P) through the LDS-staged march.
 // The first 256 bytes of `lds_raw` are block scratch, the rest staged cells.
-template <int M, int ST, bool kSpectrum>
+// kWide: also instantiate the march for volumes of 4 GiB or more (the batch
+// kernel, which inlines every material count, leaves it out).
+template <int M, int ST, bool kSpectrum, bool kWide = true>
 __device__ __forceinline__ void march_tile(const MarchArgs &P, uint32_t logical, char *lds_raw)
 {
     const TileLane t = tile_lane(P, logical);
@@ -1445,8 +1447,8 @@ __device__ __forceinline__ void march_tile(const MarchArgs &P, uint32_t logical,
     float acc[M];
 #pragma unroll
     for (int m = 0; m < M; m++) acc[m] = 0.0f;
-    if (P.wide)
-        march_chunks<M, ST, true>(P, R, rsrc, lds_raw + 256, (int *)lds_raw, acc);
+    if (kWide && P.wide)
+        march_chunks<M, ST, kWide>(P, R, rsrc, lds_raw + 256, (int *)lds_raw, acc);
     else
         march_chunks<M, ST, false>(P, R, rsrc, lds_raw + 256, (int *)lds_raw, acc);
     finish_ray<M, ST, kSpectrum>(P, t, acc);
@@ -1533,10 +1535,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             const uint32_t tiles = (uint32_t)(it.P.tiles_u * it.P.tiles_v);
             const uint32_t logical = ((r / tiles) * kXcds + x) * tiles + r % tiles;
             switch (it.n_mat) {
-            case 1: march_tile<1, ST, true>(it.P, logical, lds_raw); break;
-            case 2: march_tile<2, ST, true>(it.P, logical, lds_raw); break;
-            case 3: march_tile<3, ST, true>(it.P, logical, lds_raw); break;
-            default: march_tile<4, ST, true>(it.P, logical, lds_raw); break;
+            case 1: march_tile<1, ST, true, false>(it.P, logical, lds_raw); break;
+            case 2: march_tile<2, ST, true, false>(it.P, logical, lds_raw); break;
+            case 3: march_tile<3, ST, true, false>(it.P, logical, lds_raw); break;
+            default: march_tile<4, ST, true, false>(it.P, logical, lds_raw); break;
             }
         }
     }
@@ -2913,6 +2915,9 @@ int ctfp_batch_create(const ctfp_batch_item *items, int n_items, ctfp_batch **ou
             return set_error(CTFP_ERR_INVALID, "batch: item %d is tuned for the global march", i);
         if (c->spec_n_mat != c->n_mat || !c->W)
             return set_error(CTFP_ERR_STATE, "batch: item %d has no spectral tables for its volumes", i);
+        if (c->vol_bytes >= 0xFFFFFFFFull)
+            return set_error(CTFP_ERR_UNSUPPORTED,
+                             "batch: item %d's volume spans 4 GiB or more (use ctfp_forward for it)", i);
         int rc = forward_args(c, items[i].view_begin, items[i].view_count, items[i].I, items[i].IAbsorb,
                               nullptr, true, h[i].P);
         if (rc) return rc;
