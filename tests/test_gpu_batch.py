@@ -89,6 +89,28 @@ def test_batch_rejects_changed_inputs(require_gpu):
     proj.close()
 
 
+def test_batch_rejects_volume_past_4gib(require_gpu):
+    """A scan whose padded volume spans 4 GiB or more (2040 x 2040 x 520,
+    8.76 GB) runs through ctfp_forward but is refused as a batch item."""
+    import torch
+    s = S.config2()
+    s.materials = (S.WATER,)
+    s.n_vox, s.length_mm = (2040, 2040, 520), (204.0, 204.0, 52.0)
+    s.det, s.d_size, s.views = (32, 32), 0.075, 2
+    proj = P.Projector(P.scan_params(**s.scan_kwargs()))
+    vol = torch.zeros((520, 2040, 2040), dtype=torch.float32, device="cuda")
+    proj.set_volumes([vol])
+    del vol
+    proj.set_spectral(**P.spectral_inputs(s, *s.det))
+    I, IA = proj.forward(0, 2)
+    I0 = proj.flat_field()
+    torch.cuda.synchronize()
+    assert torch.equal(I, I0.expand_as(I))  # an empty volume: the flat field
+    with pytest.raises(P.CTFPError):
+        P.Batch([(proj, 0, 2, torch.empty_like(I), torch.empty_like(IA))])
+    proj.close()
+
+
 def test_autotune_keeps_results(require_gpu):
     import torch
     proj = make_item(64, 3, (128, 96), 12, 120.0, True, seed=4, dtype=P.CTFP_F16)
