@@ -8,10 +8,21 @@ applied per ray; outputs I and IAbsorb stay in HBM.  Inputs are synthetic
 HBM before timing starts.
 
 The headline is the mixed-precision projector (volumes stored in FP16,
-interpolation, line integrals and exp in FP32, energy sums in FP64); the
-same scan through the FP32 projector with the same launch options and
-through the plain FP32 projector (corners gathered from HBM/L2, no LDS
+interpolation, line integrals and exp in FP32, energy sums in compensated
+FP32); the same scan through the FP32 projector with the same launch options
+and through the plain FP32 projector (corners gathered from HBM/L2, no LDS
 staging) is timed beside it.  --dtype f32 / bf16 time the other storage types.
+
+Outside the timed region the line also carries, unless --no-extras:
+  - the error against the FP64 oracle of the FP16, BF16 and FP32 projectors
+    over 8 views spread over the scan (max and mean);
+  - the 128^3 single-material scan: the plain and staged FP32 GPU projectors
+    and the oracle's FP32 mirror on this host's cores;
+  - the differentiable 3 x 256^3 projector: backward / forward time, peak
+    device memory;
+  - the batched workload of 8 uneven scans: aggregate Gsamples/s of the
+    persistent graph launch against the same scans run one after the other
+    through the plain FP32 projector.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -60,35 +71,128 @@ def cpu_baseline(setup, vols, kw, target_s: float):
                       f"march + spectrum, {secs:.1f} s on {threads} threads"}
 
 
-def accuracy(proj, setup, vols, kw):
-    """Max relative error of view 0 against the FP64 evaluation of the method
-    (oracle, CPU): p = ln(I0/I) floored at 1e-2 max|p|, I floored at 1e-3 max."""
+def det_table(kw):
+    import numpy as np
+    from oracle import oracle as O
+    return np.stack([O.det_response(kw.get("det_factor", 1.0), float(m), kw["scint_thickness"])
+                     for m in kw["scint_mu"]])
+
+
+ACC_VIEWS = (0, 45, 90, 135, 180, 225, 270, 315)
+
+
+def accuracy(projs, setup, vols, kw, views=ACC_VIEWS):
+    """Relative error against the FP64 evaluation of the method (oracle,
+    CPU) over `views` of the scan, for each projector of `projs` (name ->
+    Projector): |d| / max(|ref|, f max|ref|) with f = 1e-3 for I and 1e-2 for
+    p = ln(I0/I), max and mean over every ray of those views."""
     import numpy as np
     import torch
     from oracle import oracle as O
-    I, IA = proj.forward(0, 1)
-    I0 = proj.flat_field()
-    torch.cuda.synchronize()
-    I, I0 = I.cpu().numpy().astype(np.float64), I0.cpu().numpy().astype(np.float64)
     g = O.geom_from_setup(setup)
     p = O.paras(g)
-    L64 = O.march_f64(g, p, vols, 0, 1, threads=O.num_threads())
-    det = np.stack([O.det_response(1.0, float(m), kw["scint_thickness"]) for m in kw["scint_mu"]])
-    I64, _, I064 = O.spectrum_f64(L64, kw["atten"], kw["spectrum"], kw["i0_val"], kw["grid"], det)
+    det = det_table(kw)
+    ref_I, ref_p = [], []
+    for v in views:
+        L64 = O.march_f64(g, p, vols, v, 1, threads=O.num_threads())
+        I64, _, I064 = O.spectrum_f64(L64, kw["atten"], kw["spectrum"], kw["i0_val"], kw["grid"], det)
+        ref_I.append(I64[0])
+        ref_p.append(np.log(I064 / I64[0]))
+        log(f"accuracy: FP64 oracle view {v} done")
+    ref_I, ref_p = np.stack(ref_I), np.stack(ref_p)
 
     def rel(a, b, f):
-        return float((np.abs(a - b) / np.maximum(np.abs(b), f * np.abs(b).max())).max())
+        return np.abs(a - b) / np.maximum(np.abs(b), f * np.abs(b).max())
 
-    pg, pr = np.log(I0[None] / I), np.log(I064[None] / I64)
-    return {"p": rel(pg, pr, 1e-2), "I": rel(I, I64, 1e-3), "sample": "view 0 of 360 vs FP64 oracle"}
+    out = {"views": list(views), "metric": "|d| / max(|ref|, f max|ref|) vs FP64 oracle, f = 1e-3 (I), "
+                                           "1e-2 (p = ln(I0/I)), over every ray of the views"}
+    for name, pr in projs.items():
+        I = np.stack([pr.forward(v, 1)[0][0].cpu().numpy().astype(np.float64) for v in views])
+        I0 = pr.flat_field().cpu().numpy().astype(np.float64)
+        torch.cuda.synchronize()
+        eI, ep = rel(I, ref_I, 1e-3), rel(np.log(I0[None] / I), ref_p, 1e-2)
+        out[name] = {"I_max": float(eI.max()), "I_mean": float(eI.mean()),
+                     "p_max": float(ep.max()), "p_mean": float(ep.mean())}
+        log(f"accuracy {name}: {out[name]}")
+    return out
+
+
+def config1_figures(steps, warmup, cpu_seconds):
+    """The 128^3 single-material scan (256x256, 90 views, K = 272, 61 bins,
+    energy-integrating CsI, no grid): the plain FP32 GPU projector (every
+    corner from HBM/L2) and the LDS-staged one, against the oracle's FP32
+    mirror on this host's cores (a bounded sample of views), with the GPU's
+    error against that mirror on the views the CPU ran."""
+    import numpy as np
+    import torch
+
+    import ct_forward_projection_amd as P
+    from ct_forward_projection_amd import synthetic as S
+    from oracle import oracle as O
+    s = S.config1()
+    vol = S.cylinder_phantom()
+    kw = P.spectral_inputs(s, *s.det)
+    out = {"workload": "128^3 water/ellipsoid phantom (1 mm), 256x256 @ 0.8 mm, 90 views, K=272, "
+                       "80 kVp 61 bins, CsI E*eta, no grid"}
+    with P.Projector(P.scan_params(**s.scan_kwargs())) as pr:
+        pr.set_volumes([vol])
+        pr.set_spectral(**kw)
+        samples = s.views * pr.samples_per_view
+        out["samples_per_scan"] = samples
+        stream = torch.cuda.current_stream()
+        for march in ("global", "lds"):
+            pr.set_tuning(march=march)
+            for _ in range(warmup):
+                pr.forward()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(steps):
+                I, IA = pr.forward()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / steps
+            key = "gpu_fp32_plain" if march == "global" else "gpu_fp32_staged"
+            out[key] = {"gsamples_per_s": round(samples / ms / 1e6, 2), "views_per_s": round(s.views / ms * 1e3, 1),
+                        "ms_per_scan": round(ms, 3)}
+            log(f"config 1 {key}: {out[key]}")
+        I_gpu = I.cpu().numpy()
+    g = O.geom_from_setup(s)
+    p = O.paras(g)
+    threads = O.num_threads()
+    spec = np.asarray(kw["spectrum"], np.float32) * np.arange(1, len(kw["spectrum"]) + 1, dtype=np.float32)
+    det = det_table(kw)
+    t = time.perf_counter()
+    n = 0
+    errs = []
+    while n < s.views and (n == 0 or time.perf_counter() - t < cpu_seconds):
+        L = O.march_f32(g, p, [vol], n, 1, threads=threads)
+        Ic, _, _ = O.spectrum_f32(L, kw["atten"], spec, kw["i0_val"], None, det, threads=threads)
+        ref = Ic[0].astype(np.float64)
+        errs.append(float((np.abs(I_gpu[n] - ref) / np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max())).max()))
+        n += 1
+    secs = time.perf_counter() - t
+    spv = s.det[0] * s.det[1] * p.n_steps
+    out["cpu_fp32_mirror"] = {"gsamples_per_s": round(n * spv / secs / 1e9, 4),
+                              "views_per_s": round(n / secs, 3), "cores": threads,
+                              "sample": f"views 0-{n - 1} of {s.views}, march + spectrum, {secs:.1f} s"}
+    out["gpu_vs_cpu_fp32_max_rel_err"] = max(errs)
+    out["speedup_gpu_plain_vs_cpu"] = round(out["gpu_fp32_plain"]["gsamples_per_s"] /
+                                            out["cpu_fp32_mirror"]["gsamples_per_s"], 1)
+    log(f"config 1 cpu: {out['cpu_fp32_mirror']}, gpu vs cpu {max(errs):.2e}")
+    return out
 
 
 def run_adjoint(args):
+    print(json.dumps(adjoint_figures(args.steps, args.warmup)), flush=True)
+
+
+def adjoint_figures(steps, warmup):
     """Differentiable projector (3 x 256^3, 512x512, 180 views, grid + CsI):
     one weighted-least-squares step on log projections against a Poisson-noisy
     target (1e5 photons per pixel).  Forward = layout + projection launch;
-    backward = layout + adjoint launch + the loss's elementwise ops.  Prints
-    one JSON line with both times, their ratio and peak device memory."""
+    backward = layout + adjoint launch + the loss's elementwise ops.  Returns
+    both times, their ratio and peak device memory."""
     import numpy as np
     import torch
 
@@ -127,12 +231,12 @@ def run_adjoint(args):
         torch.cuda.synchronize()
         return ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), float(loss)
 
-    for i in range(args.warmup):
+    for i in range(warmup):
         step(False)
-        log(f"adjoint: warmup {i + 1}/{args.warmup} done")
+        log(f"adjoint: warmup {i + 1}/{warmup} done")
     torch.cuda.reset_peak_memory_stats(dev)
     fw, bw, losses = [], [], []
-    for i in range(args.steps):
+    for i in range(steps):
         f, b, l = step(True)
         fw.append(f)
         bw.append(b)
@@ -142,7 +246,7 @@ def run_adjoint(args):
     f, b = sum(fw) / len(fw), sum(bw) / len(bw)
     samples = setup.views * proj.samples_per_view
     out = {"metric": "differentiable projector: backward / forward time", "value": round(b / f, 3),
-           "unit": "ratio", "higher_is_better": False, "steps": args.steps, "warmup": args.warmup,
+           "unit": "ratio", "higher_is_better": False, "steps": steps, "warmup": warmup,
            "forward_ms": round(f, 2), "backward_ms": round(b, 2),
            "forward_gsamples_per_s": round(samples / (f / 1e3) / 1e9, 2),
            "peak_device_bytes": int(peak), "projector_device_bytes": proj.device_bytes,
@@ -152,46 +256,27 @@ def run_adjoint(args):
                                   "Poisson target at 1e5 photons/pixel",
                       "tuning": proj.get_tuning()}}
     proj.close()
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def batch_specs(seed: int = 2024):
-    """The eight uneven scans of the batched workload (seeded): cubic phantoms
-    of {128, 192, 256, 384, 512} voxels at 0.5 mm with 1-3 materials,
-    detectors from 256x256 to 1024x768, 60-360 views, 80-140 kVp spectra
-    (61-121 active 1-keV bins from 20 keV), grid on or off."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    dets = [(256, 256), (512, 384), (512, 512), (768, 576), (1024, 768)]
-    out = []
-    for k in range(8):
-        n = int(rng.choice([128, 192, 256, 384, 512]))
-        out.append(dict(n=n, n_mat=int(rng.integers(1, 4)), det=dets[int(rng.integers(0, len(dets)))],
-                        views=int(rng.integers(60, 361)), kvp=float(rng.choice(np.arange(80, 141, 10))),
-                        grid=bool(rng.random() < 0.5), seed=100 + k))
-    return out
+    from ct_forward_projection_amd import synthetic as S
+    return S.config5_specs(seed)
 
 
 def batch_setup(spec):
     from ct_forward_projection_amd import synthetic as S
-    s = S.config2()
-    n = spec["n"]
-    s.name = f"batch{spec['seed']}"
-    s.n_vox, s.length_mm = (n, n, n), (n * 0.5,) * 3
-    s.det, s.views, s.kvp = spec["det"], spec["views"], spec["kvp"]
-    # the detector spans the phantom's diagonal at 1.5x magnification
-    s.d_size = float(n * 0.5 * 1.45 * 1.5 / min(spec["det"]))
-    s.materials = s.materials[:spec["n_mat"]]
-    if not spec["grid"]:
-        s.grid = None
-    vols = list(S.three_material_phantom(n, seed=spec["seed"]))[:spec["n_mat"]]
-    return s, vols
+    return S.config5_setup(spec)
 
 
 def run_batch(args):
+    print(json.dumps(batch_figures(args.steps, args.warmup)), flush=True)
+
+
+def batch_figures(steps, warmup):
     """Eight uneven scans in one persistent, graph-captured launch (FP16
     storage, each scan autotuned), against the same eight run one after the
-    other through the plain FP32 projector.  Prints one JSON line."""
+    other through the plain FP32 projector."""
     import torch
 
     import ct_forward_projection_amd as P
@@ -221,16 +306,16 @@ def run_batch(args):
     stream = torch.cuda.current_stream()
 
     def timed(fn, label):
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        for _ in range(args.steps):
+        for _ in range(steps):
             fn()
         e1.record(stream)
         torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / args.steps
+        ms = e0.elapsed_time(e1) / steps
         log(f"{label}: {ms:.2f} ms per pass")
         return ms
 
@@ -252,7 +337,7 @@ def run_batch(args):
                       "n_diff": int((d != 0).sum()), "nan": int(torch.isnan(I).sum())})
     log(f"batch vs per-scan launches: {diffs}")
     out = {"metric": "batched uneven workload: aggregate Gsamples/s", "value": round(samples / ms_batch / 1e6, 2),
-           "unit": "Gsamples/s", "higher_is_better": True, "steps": args.steps, "warmup": args.warmup,
+           "unit": "Gsamples/s", "higher_is_better": True, "steps": steps, "warmup": warmup,
            "ms_per_pass": round(ms_batch, 2), "samples_per_pass": samples,
            "plain_fp32_sequential_value": round(samples / ms_plain / 1e6, 2),
            "speedup_vs_plain_fp32_sequential": round(ms_plain / ms_batch, 3),
@@ -265,7 +350,7 @@ def run_batch(args):
     batch.close()
     for p in projs + plain:
         p.close()
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -282,7 +367,11 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="samples per staged chunk (0 = library default)")
     ap.add_argument("--lds-kib", type=int, default=0, help="LDS per workgroup in KiB (0 = default)")
     ap.add_argument("--no-accuracy", action="store_true",
-                    help="skip the one-view error check against the FP64 oracle")
+                    help="skip the 8-view error check against the FP64 oracle")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the 128^3, differentiable 256^3 and batched figures")
+    ap.add_argument("--compare-steps", type=int, default=5,
+                    help="timed steps of the FP32 projectors beside the headline (at most --steps)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true",
                     help="do not time the FP32 projectors beside a 16-bit run")
@@ -347,27 +436,27 @@ def main():
         torch.cuda.synchronize()
         log(f"copy kernel: {nb} bytes read, {nb} bytes written")
 
-    def timed(pr, label):
-        for i in range(args.warmup):
+    def timed(pr, label, steps, warmup):
+        for i in range(warmup):
             pr.forward(0, views, I, IA, stream=stream)
             torch.cuda.synchronize()
-            log(f"{label}: warmup {i + 1}/{args.warmup} done")
-        starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-        ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+            log(f"{label}: warmup {i + 1}/{warmup} done")
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for i in range(args.steps):
+        for i in range(steps):
             starts[i].record(stream)
             pr.forward(0, views, I, IA, stream=stream)
             ends[i].record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         kern = [s.elapsed_time(e) for s, e in zip(starts, ends)]
-        log(f"{label}: timed {args.steps} steps, wall {wall:.3f} s, per-launch ms {kern}")
+        log(f"{label}: timed {steps} steps, wall {wall:.3f} s, per-launch ms {kern}")
         return wall, kern
 
     marched = proj.marched_samples(0, views)
-    wall, kern_ms = timed(proj, args.dtype)
+    wall, kern_ms = timed(proj, args.dtype, args.steps, args.warmup)
     ms_per_step = wall * 1e3 / args.steps
 
     # stage split: the same launch without the spectrum (line integrals of
@@ -388,13 +477,17 @@ def main():
     avg_launch_s = sum(kern_ms) / len(kern_ms) / 1e3
     achieved = samples * bytes_per_sample / avg_launch_s / 1e9
 
-    traffic = None
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs of this workload (not measured in this run)
+    traffic, traffic_src = None, None
     tfile = os.path.join(ROOT, "profiles", "march_config2_traffic.json")
     if os.path.exists(tfile) and views == setup.views:
         with open(tfile) as f:
             entry = json.load(f).get(f"{args.dtype}_{args.march}")
         if entry:
             traffic = entry.get("bytes_per_launch")
+            traffic_src = "profiles/march_config2_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by " \
+                          "the copy-kernel calibration; a committed measurement, not this run)"
 
     result = {
         "metric": METRIC, "value": round(gsps, 3), "unit": "Gsamples/s", "n_gpus": 1,
@@ -413,7 +506,9 @@ def main():
                       "spectrum": round(sorted(kern_ms)[len(kern_ms) // 2] - sorted(march_ms)[1], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic},
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "limiter": "VALU instruction issue (rocprofv3 SQ counters: "
+                                "profiles/march_config2_f16.md), not HBM bandwidth"},
     }
     if args.dtype != "f32" and not args.no_fp32:
         # the LDS-staged FP32 projector with the same launch options, and the
@@ -422,16 +517,26 @@ def main():
         ref.set_volumes(vols, storage_dtype=P.CTFP_F32)
         ref.set_spectral(**kw)
         ref.set_tuning(**tune)
-        wall32, _ = timed(ref, "f32")
-        result["staged_fp32_value"] = round(samples / (wall32 / args.steps) / 1e9, 3)
+        cs = max(1, min(args.steps, args.compare_steps))
+        wall32, _ = timed(ref, "f32", cs, 1)
+        result["staged_fp32_value"] = round(samples / (wall32 / cs) / 1e9, 3)
         result["speedup_vs_staged_fp32"] = round(result["value"] / result["staged_fp32_value"], 3)
         ref.set_tuning(march="global")
-        wallg, _ = timed(ref, "f32 global")
-        result["plain_fp32_value"] = round(samples / (wallg / args.steps) / 1e9, 3)
+        wallg, _ = timed(ref, "f32 global", cs, 1)
+        result["plain_fp32_value"] = round(samples / (wallg / cs) / 1e9, 3)
         result["speedup_vs_plain_fp32"] = round(result["value"] / result["plain_fp32_value"], 3)
+        result["compare_steps"] = cs
+        ref.set_tuning(**tune)
+        if not args.no_accuracy:
+            other = "bf16" if args.dtype == "f16" else "f16"
+            proj2 = P.Projector(P.scan_params(**setup.scan_kwargs()))
+            proj2.set_volumes(vols, storage_dtype=dtypes[other])
+            proj2.set_spectral(**kw)
+            result["rel_err_vs_fp64"] = accuracy({args.dtype: proj, other: proj2, "f32": ref}, setup, vols, kw)
+            proj2.close()
         ref.close()
-    if not args.no_accuracy:
-        result["rel_err_vs_fp64"] = accuracy(proj, setup, vols, kw)
+    elif not args.no_accuracy:
+        result["rel_err_vs_fp64"] = accuracy({args.dtype: proj}, setup, vols, kw)
     del I, IA
     proj.close()
     if not args.no_cpu_baseline:
@@ -439,6 +544,17 @@ def main():
         result["cpu_baseline"] = cpu_baseline(setup, vols, kw, args.cpu_seconds)
     else:
         result["cpu_baseline"] = None
+    if not args.no_extras:
+        del vols
+        torch.cuda.empty_cache()
+        log("extras: 128^3 single-material scan ...")
+        result["config1_single_material"] = config1_figures(max(3, min(args.steps, 10)), 1,
+                                                            min(args.cpu_seconds, 8.0))
+        log("extras: differentiable 3 x 256^3 projector ...")
+        result["config4_differentiable"] = adjoint_figures(3, 1)
+        torch.cuda.empty_cache()
+        log("extras: batched uneven workload ...")
+        result["config5_batched"] = batch_figures(3, 1)
     print(json.dumps(result), flush=True)
 
 

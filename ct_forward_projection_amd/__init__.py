@@ -8,12 +8,13 @@ inputs used for measurement (``synthetic``).
 """
 from ._lib import CTFP_BF16, CTFP_F16, CTFP_F32, CTFPError, lib  # noqa: F401
 from .projector import (Batch, Projector, compute_paras, det_response,  # noqa: F401
-                        focal_spot_offsets, grid_period_index, grid_transmission, postprocess,
+                        focal_spot_offsets, grid_period_index, grid_transmission, grid_unit_heights,
+                        postprocess,
                         scan_params, scint_thickness, spectral_inputs)
 
 __all__ = [
     "Batch", "Projector", "compute_paras", "det_response", "focal_spot_offsets",
-    "grid_period_index", "grid_transmission",
+    "grid_period_index", "grid_transmission", "grid_unit_heights",
     "postprocess", "scan_params", "scint_thickness", "spectral_inputs", "lib", "CTFPError",
     "CTFP_F32", "CTFP_F16", "CTFP_BF16",
 ]

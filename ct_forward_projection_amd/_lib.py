@@ -36,6 +36,7 @@ EXPORTED = (
     "ctfp_get_scan", "ctfp_get_device", "ctfp_device_bytes", "ctfp_autotune",
     "ctfp_batch_create", "ctfp_batch_run", "ctfp_batch_samples", "ctfp_batch_destroy",
     "ctfp_count_samples", "ctfp_set_focal_spot", "ctfp_focal_spot_offsets",
+    "ctfp_grid_unit_heights", "ctfp_count_marched",
 )
 
 CTFP_MARCH_GLOBAL = 0
@@ -99,7 +100,8 @@ class GridParams(C.Structure):
 class Tuning(C.Structure):
     """ctfp_tuning: ray-march launch options (results do not depend on them)."""
     _fields_ = [("march", C.c_int32), ("chunk", C.c_int32), ("lds_bytes", C.c_int32),
-                ("wave_u", C.c_int32), ("block_u", C.c_int32), ("adjoint", C.c_int32)]
+                ("wave_u", C.c_int32), ("block_u", C.c_int32), ("adjoint", C.c_int32),
+                ("skip_empty", C.c_int32)]
 
 
 class BatchItem(C.Structure):
@@ -140,6 +142,7 @@ def _declare(lib):
         "ctfp_det_response": (i32, [i64, C.c_float, C.c_float, _FP, _FP]),
         "ctfp_grid_transmission": (i32, [P(GridParams), i32, _FP, _FP, _FP]),
         "ctfp_grid_period_index": (i32, [i64, _FP, P(i64), i64, P(i64)]),
+        "ctfp_grid_unit_heights": (i32, [P(GridParams), _FP, i64, P(i64)]),
         "ctfp_bench_copy": (i32, [vp, vp, i64, vp]),
         "ctfp_set_tuning": (i32, [vp, P(Tuning)]),
         "ctfp_backward": (i32, [vp, i64, i64, vp, vp, vp, vp, vp]),
@@ -150,6 +153,7 @@ def _declare(lib):
         "ctfp_device_bytes": (i32, [vp, P(i64)]),
         "ctfp_autotune": (i32, [vp, i32, P(Tuning)]),
         "ctfp_count_samples": (i32, [vp, i64, i64, P(i64)]),
+        "ctfp_count_marched": (i32, [vp, i64, i64, P(i64)]),
         "ctfp_set_focal_spot": (i32, [vp, _FP, _FP]),
         "ctfp_focal_spot_offsets": (i32, [i64, C.c_float, C.c_uint32, _FP, _FP]),
         "ctfp_batch_create": (i32, [P(BatchItem), i32, P(vp)]),

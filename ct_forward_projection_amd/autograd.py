@@ -7,14 +7,21 @@ torch::autograd::Function in the `_ctfp_torch` extension): the gradient
 is the backprojection of the polyenergetic residual
   dl/dL_m = (dl/dIA - dl/dI) * sum_E A[E][m] W_E exp(-sum_m' A[E][m'] L_m')
 into per-material densities, with every ray's line integrals recomputed in
-the backward pass rather than stored.
+the backward pass rather than stored.  `project_batch` and `PolyProjection`
+do the same for several phantoms, each with its own projector (scan
+geometry, detector, spectrum) or one shared projector.
+
+The graph of every call keeps its projector: the context stays alive until
+the graph is released, and Projector.close() on a projector that a graph
+still uses only marks it closed (the context goes with the last graph).
 
 The reference has no gradient; the forward is its projection call
 (CBCTSinMatPolyForwardProjGrid::computePolyForwProj,
 Source/CBCTSinMatPolyForwardProjGrid.cpp:82-105).
 
-The module needs the extension built (`make -C ct_forward_projection_amd/csrc
-torch`, done by the package build); there is no fallback.
+The module needs the extension, which __graft_entry__.build() compiles
+(`make -C ct_forward_projection_amd/csrc torch` alone also does); there is no
+fallback.
 """
 from __future__ import annotations
 
@@ -24,6 +31,7 @@ from . import _lib as L
 from .projector import Projector
 
 _ext = None
+_keep = None
 
 
 def _extension():
@@ -40,6 +48,28 @@ def _extension():
     return _ext
 
 
+def _keep_function():
+    """An identity on the densities whose graph node holds the projector and
+    its graph token, so that the context outlives every graph built on it."""
+    global _keep
+    if _keep is None:
+        import torch
+
+        class _Keep(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, rho, proj, token):
+                ctx.proj = proj
+                ctx.token = token
+                return rho.view_as(rho)
+
+            @staticmethod
+            def backward(ctx, grad):
+                return grad, None, None
+
+        _keep = _Keep
+    return _keep
+
+
 def project(rho, proj: Projector, view_begin: int = 0, view_count: int | None = None,
             storage_dtype: int = L.CTFP_F32):
     """(I, IAbsorb) [views][nv][nu] of densities rho, differentiable in rho.
@@ -49,8 +79,52 @@ def project(rho, proj: Projector, view_begin: int = 0, view_count: int | None = 
     share one projector must not overlap on different streams."""
     ext = _extension()
     vc = -1 if view_count is None else int(view_count)
-    I, IA = ext.project(rho, proj.handle, int(view_begin), vc, int(storage_dtype))
+    handle = proj.handle  # raises on a closed projector
+    if rho.requires_grad:
+        rho = _keep_function().apply(rho, proj, proj._graph_token())
+    I, IA = ext.project(rho, handle, int(view_begin), vc, int(storage_dtype))
     return I, IA
+
+
+def project_batch(rhos, projs, view_begin: int = 0, view_count: int | None = None,
+                  storage_dtype: int = L.CTFP_F32):
+    """[(I_b, IA_b)] for phantoms rhos[b] ([n_mat][nz][ny][nx] each), each
+    through projs[b], or all through one projector when `projs` is a single
+    Projector.  The phantoms run one after the other on the current stream
+    (each launch fills the GPU); every output is differentiable in its own
+    phantom, and the backward runs the adjoint per phantom."""
+    rhos = list(rhos)
+    plist = [projs] * len(rhos) if isinstance(projs, Projector) else list(projs)
+    if len(plist) != len(rhos):
+        raise ValueError(f"{len(rhos)} phantoms but {len(plist)} projectors")
+    return [project(r, p, view_begin, view_count, storage_dtype) for r, p in zip(rhos, plist)]
+
+
+def _module_base():
+    import torch
+    return torch.nn.Module
+
+
+class PolyProjection(_module_base()):
+    """torch.nn.Module over project_batch: forward(rho) with rho a tensor
+    [B][n_mat][nz][ny][nx] (or a list of B tensors) returns (I, IAbsorb),
+    stacked to [B][views][nv][nu] when every projector has the same detector
+    and view count, else as lists.  `projs` is one Projector shared by the
+    batch or one per phantom."""
+
+    def __init__(self, projs, view_begin: int = 0, view_count: int | None = None,
+                 storage_dtype: int = L.CTFP_F32):
+        super().__init__()
+        self.projs = projs
+        self.view_begin, self.view_count, self.storage_dtype = view_begin, view_count, storage_dtype
+
+    def forward(self, rho):
+        import torch
+        out = project_batch(list(rho), self.projs, self.view_begin, self.view_count, self.storage_dtype)
+        I, IA = [o[0] for o in out], [o[1] for o in out]
+        if all(t.shape == I[0].shape for t in I):
+            return torch.stack(I), torch.stack(IA)
+        return I, IA
 
 
 def log_projection(I, I0, eps: float = 1e-12):

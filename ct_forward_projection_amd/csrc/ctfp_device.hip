@@ -1,12 +1,14 @@
 // ctfp_device.hip -- gfx950 kernels and the device half of the C ABI (include/ctfp.h).
 //
 // Data layout in HBM
-//   volume : the n_mat density channels interleaved per voxel ([z][y][x][m]),
-//            padded with two zero voxels on both sides of every axis so the
-//            8-corner gather of any integer cell index clamped to [-2, N]
-//            stays in bounds and reads only zeros outside the support;
-//            read with raw buffer loads (one VGPR byte offset per sample, the
-//            three other corner rows as uniform SGPR offsets).
+//   volume : the n_mat density channels in groups of up to four, each group
+//            interleaved per voxel ([z][y][x][m]; 16-bit storage holds the
+//            x-pair (v(x), v(x+1)) per dword), padded with two zero voxels on
+//            both sides of every axis so the 8-corner gather of any integer
+//            cell index clamped to [-2, N] stays in bounds and reads only
+//            zeros outside the support; read with raw buffer loads.
+//   occupancy: per group, one byte per brick of 8^3 padded cells (any
+//            non-zero cell), for skipping chunks that would add only zeros.
 //   weights: W[e][v][u] = I0Val * S_e * T_e(u) * eta_e(v,u) for the active
 //            (non-zero) bins; the v stride is 0 when eta does not vary per pixel.
 //   outputs: I, IAbsorb [view][v][u]; optional L [view][m][v][u].
@@ -17,9 +19,10 @@
 // the sample index position is affine in k, so each ray carries its FP64-derived
 // start and step and its support interval [k_lo, k_hi) where any channel can
 // be non-zero; samples outside it are exact zeros in the reference and are
-// skipped.  The sample position itself stays in FP64 (FP64 FMA runs at the
-// unpacked FP32 rate on CDNA): an FP32 index near 512 cannot hold better than
-// 3e-5 voxel, which shows up as ~1e-4 relative error on grazing rays.
+// skipped.  The start and step become 32.32 fixed point: every 16th sample is
+// an anchor reached by exact integer steps, and the 16 samples after it are
+// FP32 offsets from it (error ~3e-6 voxel, against ~1e-4 for the reference's
+// FP32 t * d - SOD), so no FP64 runs in the march itself.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
@@ -33,30 +36,10 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "ctfp.h"
 #include "ctfp_internal.h"
-
-// ---------------------------------------------------------------- errors ---
-
-namespace ctfp {
-
-static thread_local std::string g_last_error;
-
-int set_error(int code, const char *fmt, ...)
-{
-    char buf[1024];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    g_last_error = buf;
-    return code;
-}
-
-const char *last_error() { return g_last_error.c_str(); }
-void clear_error() { g_last_error.clear(); }
-
-} // namespace ctfp
 
 using ctfp::set_error;
 
@@ -70,21 +53,42 @@ using ctfp::set_error;
 
 // --------------------------------------------------------------- context ---
 
+// Materials per scan, and channels per interleaved volume group (one march
+// handles up to four).
+constexpr int kMaxMaterials = CTFP_MAX_MATERIALS;
+constexpr int kGroupMats = 4;
+constexpr int kMaxGroups = (kMaxMaterials + kGroupMats - 1) / kGroupMats;
+
 struct ctfp_context {
     int device = 0;
     ctfp_scan_params scan{};
     ctfp_system_info sys{};
-    // volume
+    // volume: the n_mat channels in groups of up to four, each group its own
+    // interleaved layout (one march per group)
     int n_mat = 0;
     int dtype = CTFP_F32;
     void *vol = nullptr;
     size_t vol_bytes = 0;
     int64_t px = 0, py = 0, pz = 0; // padded extents
+    int n_groups = 0;
+    int group_mats[kMaxGroups] = {};  // channels of each group
+    int group_moff[kMaxGroups] = {};  // its first channel
+    size_t group_off[kMaxGroups] = {}; // its byte offset in vol
+    size_t group_bytes[kMaxGroups] = {};
+    int *d_flag = nullptr; // device flag: a source voxel was not finite
+    // occupancy of each group's layout: one byte per brick of 8^3 padded
+    // cells, non-zero when a cell of the brick holds a non-zero channel
+    uint8_t *occ = nullptr;
+    int64_t ob_x = 0, ob_y = 0, ob_z = 0; // bricks per axis
+    // line-integral workspace of grouped forwards and adjoints
+    float *l_ws = nullptr;
+    size_t l_ws_bytes = 0;
     // spectral
     int n_act = 0;
     int spec_n_mat = 0;  // materials the spectral tables were built for
     float *A = nullptr;  // [n_act][n_mat]
-    _Float16 *Aq = nullptr; // [n_act rounded up to 32][16]: MFMA operand of 2^8 A
+    _Float16 *Aq = nullptr; // [n_act rounded up to 32][16]: MFMA operand of 2^s A
+    float aq_inv_scale = 1.0f; // 2^-s
     float *W = nullptr;  // [n_act][nv or 1][nu]
     int64_t w_stride_e = 0;
     int32_t w_stride_v = 0;
@@ -97,7 +101,7 @@ struct ctfp_context {
     float *dI = nullptr, *dIA = nullptr;
     int64_t staged_views = 0;
     // launch tuning
-    ctfp_tuning tune{CTFP_MARCH_LDS, 16, 40 * 1024, 16, 1, CTFP_ADJOINT_GATHER};
+    ctfp_tuning tune{CTFP_MARCH_LDS, 16, 40 * 1024, 16, 1, CTFP_ADJOINT_GATHER, 1};
     // dl/dL workspace of the gather adjoint
     float *adj_ws = nullptr;
     size_t adj_ws_bytes = 0;
@@ -106,6 +110,13 @@ struct ctfp_context {
 };
 
 namespace {
+
+// A roctx range over one entry point (visible to rocprofv3 --marker-trace;
+// a call that returns at once when no profiler is attached).
+struct Range {
+    explicit Range(const char *name) { roctxRangePushA(name); }
+    ~Range() { roctxRangePop(); }
+};
 
 struct DeviceGuard {
     int prev = -1;
@@ -131,59 +142,141 @@ template <typename T> void dfree(T *&p)
 constexpr int kVoxPadLo = 2; // zero voxels below index 0 on each axis
 constexpr int kVoxPadHi = 2; // zero voxels above index N-1 on each axis
 
-// Per-material FP32 source volumes [z][y][x] (device pointers).
+// The FP32 source volumes [z][y][x] (device pointers) of one group of up to
+// four channels.
 struct VolPtrs {
     const float *p[4];
 };
 
+// Source voxel of channel m, zero outside the volume; a value that is not
+// finite raises *bad (when given).
 __device__ __forceinline__ float src_voxel(const VolPtrs &src, int m, int64_t x, int64_t y, int64_t z,
-                                           int64_t nx, int64_t ny, int64_t nz)
+                                           int64_t nx, int64_t ny, int64_t nz, int *bad)
 {
     if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return 0.0f;
-    return src.p[m][(z * ny + y) * nx + x];
+    const float v = src.p[m][(z * ny + y) * nx + x];
+    if (bad && !isfinite(v)) *bad = 1;
+    return v;
 }
 
 // FP32 resident layout: channels interleaved per voxel, zero padding.
 __global__ void layout_voxels_f32(VolPtrs src, float *__restrict__ dst, int n_mat, int64_t nx,
-                                  int64_t ny, int64_t nz, int64_t px, int64_t py, int64_t pz)
+                                  int64_t ny, int64_t nz, int64_t px, int64_t py, int64_t pz, int *bad)
 {
     const int64_t total = px * py * pz;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
          i += (int64_t)gridDim.x * blockDim.x) {
         int64_t x = i % px - kVoxPadLo, y = (i / px) % py - kVoxPadLo, z = i / (px * py) - kVoxPadLo;
-        for (int m = 0; m < n_mat; m++) dst[i * n_mat + m] = src_voxel(src, m, x, y, z, nx, ny, nz);
+        for (int m = 0; m < n_mat; m++) dst[i * n_mat + m] = src_voxel(src, m, x, y, z, nx, ny, nz, bad);
     }
+}
+
+// 32-bit integer hash (xor-shift-multiply; the "lowbias32" constants of
+// C. Wellons' hash search).
+__device__ __forceinline__ uint32_t hash32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// Dither threshold in [0, 1) of voxel (x, y, z) >= 0 of channel `mat`,
+// stratified over 2 x 2 x 2 blocks: the block's eight voxels take the ranks
+// 0..7 of a pseudo-random permutation plus one shared uniform offset, so each
+// block rounds up close to 8 f of its voxels when all hold a fraction f of a
+// step (an unbiased rounding whose error nearly cancels within a block).
+__device__ __forceinline__ float dither_threshold(int64_t x, int64_t y, int64_t z, int mat)
+{
+    const uint32_t key =
+        hash32((uint32_t)(x >> 1) * 0x9E3779B1u ^
+               hash32((uint32_t)(y >> 1) * 0x85EBCA77u ^ hash32((uint32_t)(z >> 1) * 0xC2B2AE3Du ^
+                                                                  (uint32_t)mat * 0x27D4EB2Fu)));
+    const uint32_t self = (uint32_t)((x & 1) | ((y & 1) << 1) | ((z & 1) << 2));
+    const uint32_t hs = hash32(key + self);
+    uint32_t rank = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) {
+        const uint32_t hj = hash32(key + j);
+        rank += (hj < hs || (hj == hs && j < self)) ? 1u : 0u;
+    }
+    const float u = (float)(hash32(key ^ 0xA511E9B3u) >> 8) * 0x1p-24f;
+    return ((float)rank + u) * 0.125f;
+}
+
+// v rounded to one of its two neighbouring BF16 values: the one toward zero,
+// or the next one away from zero when the fraction of the step beyond the
+// first exceeds the threshold t (exact values stay exact).  Over uniform
+// thresholds the mean is v; the error stays below one step (2^-7 relative).
+template <bool kBF16> __device__ __forceinline__ uint32_t quantize16(float v, float t)
+{
+    static_assert(kBF16, "FP16 storage rounds to nearest");
+    const uint32_t b = __float_as_uint(v), lo = b & 0xFFFF0000u;
+    if (lo == b || !isfinite(v)) return b >> 16;
+    const float flo = __uint_as_float(lo), fhi = __uint_as_float(lo + 0x10000u);
+    const float f = (v - flo) / (fhi - flo);
+    return (f > t ? lo + 0x10000u : lo) >> 16;
 }
 
 // 16-bit resident layout: cell (x, y, z) holds, per channel, the pair
 // (v(x), v(x+1)) in one dword, so one row load serves both x corners.
+// FP16 rounds to nearest (|error| <= 2^-11 relative).  BF16 (8 significant
+// bits) rounds each voxel up or down by the stratified dither, the same way
+// in both cells that hold it: a uniform insert then keeps its mean along a
+// ray instead of one systematic offset of up to 2^-8 relative.
+// mat0: index of the group's first channel in the scan (seeds its dither).
 template <bool kBF16>
-__global__ void layout_pairs_16(VolPtrs src, uint32_t *__restrict__ dst, int n_mat, int64_t nx,
-                                int64_t ny, int64_t nz, int64_t px, int64_t py, int64_t pz)
+__global__ void layout_pairs_16(VolPtrs src, uint32_t *__restrict__ dst, int n_mat, int mat0, int64_t nx,
+                                int64_t ny, int64_t nz, int64_t px, int64_t py, int64_t pz, int *bad)
 {
     const int64_t total = px * py * pz;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
          i += (int64_t)gridDim.x * blockDim.x) {
         int64_t x = i % px - kVoxPadLo, y = (i / px) % py - kVoxPadLo, z = i / (px * py) - kVoxPadLo;
         for (int m = 0; m < n_mat; m++) {
-            float lo = src_voxel(src, m, x, y, z, nx, ny, nz);
-            float hi = src_voxel(src, m, x + 1, y, z, nx, ny, nz);
+            const float lo = src_voxel(src, m, x, y, z, nx, ny, nz, bad);
+            const float hi = src_voxel(src, m, x + 1, y, z, nx, ny, nz, nullptr);
             uint32_t bits;
             if constexpr (kBF16) {
-                bits = (uint32_t)__bfloat16_as_ushort(__float2bfloat16(lo)) |
-                       ((uint32_t)__bfloat16_as_ushort(__float2bfloat16(hi)) << 16);
+                const uint32_t qlo = lo == 0.0f ? 0u : quantize16<true>(lo, dither_threshold(x, y, z, mat0 + m));
+                const uint32_t qhi =
+                    hi == 0.0f ? 0u : quantize16<true>(hi, dither_threshold(x + 1, y, z, mat0 + m));
+                bits = (qlo & 0xFFFFu) | (qhi << 16);
             } else {
-                __half2 h = __floats2half2_rn(lo, hi);
-                bits = __builtin_bit_cast(uint32_t, h);
+                bits = __builtin_bit_cast(uint32_t, __floats2half2_rn(lo, hi));
             }
             dst[i * n_mat + m] = bits;
         }
     }
 }
 
+// Occupancy of a group's resident layout: one byte per brick of
+// kOccBrick^3 padded cells, 1 when any dword of any cell in it is non-zero
+// (a 16-bit cell holds v(x) and v(x+1)).  One workgroup per brick.
+constexpr int kOccBrick = 8;
+__global__ __launch_bounds__(256) void occupancy_kernel(const uint32_t *__restrict__ vol, int words, int64_t px,
+                                                        int64_t py, int64_t pz, int64_t obx, int64_t oby,
+                                                        uint8_t *__restrict__ occ)
+{
+    const int64_t b = blockIdx.x;
+    const int64_t bx = b % obx, by = (b / obx) % oby, bz = b / (obx * oby);
+    int any = 0;
+    for (int c = threadIdx.x; c < kOccBrick * kOccBrick * kOccBrick; c += blockDim.x) {
+        const int64_t x = bx * kOccBrick + (c % kOccBrick), y = by * kOccBrick + (c / kOccBrick) % kOccBrick;
+        const int64_t z = bz * kOccBrick + c / (kOccBrick * kOccBrick);
+        if (x >= px || y >= py || z >= pz) continue;
+        const uint32_t *cell = vol + ((z * py + y) * px + x) * words;
+        for (int w = 0; w < words; w++) any |= cell[w] != 0u;
+    }
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) occ[b] = any ? 1 : 0;
+}
+
 // Density channels [m][z][y][x] from a label volume.
 struct Densities {
-    float d[4];
+    float d[kMaxMaterials];
 };
 __global__ void labels_to_raw(const uint8_t *__restrict__ labels, float *__restrict__ raw, int n_mat,
                               int single_material, Densities rho, int64_t nvox)
@@ -307,15 +400,22 @@ struct MarchArgs {
     // spectrum
     int32_t n_act;
     const float *A; // [n_act][M]
-    const _Float16 *Aq; // [n_act rounded up to 32][16]: 2^8 A split into FP16 hi / lo (MFMA operand)
+    const _Float16 *Aq; // [n_act rounded up to 32][16]: 2^s A split into FP16 hi / lo (MFMA operand)
+    float aq_inv_scale; // 2^-s
     const float *W;
     int64_t w_stride_e;
     int32_t w_stride_v;
-    // outputs
+    // outputs; L rows [view][l_mats][v][u], this launch's channels from l_moff
     float *I, *IA, *L;
+    int32_t l_mats, l_moff;
     // LDS staging (march_lds_kernel)
     int32_t chunk;     // samples per staged chunk
     int32_t lds_cells; // staged cells that fit the LDS budget
+    // empty chunks: occupancy of 8^3-cell bricks of the padded volume (null:
+    // no skipping), its bricks along x and y, and the padded cells per axis
+    const uint8_t *occ;
+    int32_t ob_x, ob_y;
+    int32_t pcx, pcy, pcz;
     // ray tile: wavefronts of (1 << wu_log2) x (64 >> wu_log2) pixels,
     // (1 << bu_log2) x (4 >> bu_log2) wavefronts per workgroup
     int32_t wu_log2, bu_log2;
@@ -864,7 +964,10 @@ typedef float float16_t __attribute__((ext_vector_type(16)));
 // operands are split into FP16 hi + lo parts and K holds, per material,
 // (A_hi, L_hi), (A_hi, L_lo), (A_lo, L_hi): the products are exact in the FP32
 // accumulator and only A_lo L_lo (~2^-22 relative) is left out.  A is scaled
-// by 2^8 (P.Aq, [energy rows padded to 32][16] FP16) so A_lo stays normal.
+// by a power of two 2^s (P.Aq, [energy rows padded to 32][16] FP16) chosen
+// from the largest attenuation so that 2^s A stays below 2^15 (no FP16
+// overflow) and A_lo of the large entries stays normal; tau is the FP32 sum
+// times 2^-s (exact).
 // Lane l (half h = l / 32, column c = l % 32) receives tau for rays c and
 // 32 + c and the energy rows 8b + 4h + q (b, q < 4) of each 32-row block, runs
 // the exponentials and the weighted sums for those (each group of four bins
@@ -938,7 +1041,7 @@ __device__ __forceinline__ void spectrum_mfma(const MarchArgs &P, const TileLane
                 bool small = false;
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    tau[q] = d[4 * bb + q] * (1.0f / 256.0f);
+                    tau[q] = d[4 * bb + q] * P.aq_inv_scale;
                     ex[q] = __builtin_amdgcn_exp2f(-tau[q] * 1.44269504088896341f);
                     ab[q] = 1.0f - ex[q];
                     small |= fabsf(tau[q]) < 0.35f;
@@ -990,7 +1093,7 @@ __device__ __forceinline__ void finish_ray(const MarchArgs &P, const TileLane &t
     for (int m = 0; m < M; m++) Lm[m] = acc[m] * P.dx_f;
     if (P.L && t.active) {
 #pragma unroll
-        for (int m = 0; m < M; m++) P.L[((int64_t)t.vw_local * M + m) * npix + pix] = Lm[m];
+        for (int m = 0; m < M; m++) P.L[((int64_t)t.vw_local * P.l_mats + P.l_moff + m) * npix + pix] = Lm[m];
     }
     if constexpr (kSpectrum && ST != kStF32 && 3 * M <= 16) {
         // the mixed-precision projector: exponents on the matrix cores
@@ -1073,7 +1176,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 }
 
 // Block-wide min / max of six ints (cell box of a chunk).
-__device__ __forceinline__ void block_box(int (&lo)[3], int (&hi)[3], int *scratch)
+__device__ __forceinline__ void block_box(int (&lo)[3], int (&hi)[3], int *boxbuf)
 {
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -1087,18 +1190,18 @@ __device__ __forceinline__ void block_box(int (&lo)[3], int (&hi)[3], int *scrat
     if (lane == 0) {
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            scratch[wave * 8 + d] = lo[d];
-            scratch[wave * 8 + 4 + d] = hi[d];
+            boxbuf[wave * 8 + d] = lo[d];
+            boxbuf[wave * 8 + 4 + d] = hi[d];
         }
     }
     __syncthreads();
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        int a = scratch[d], b = scratch[4 + d];
+        int a = boxbuf[d], b = boxbuf[4 + d];
 #pragma unroll
         for (int w = 1; w < kBlock / 64; w++) {
-            a = min(a, scratch[w * 8 + d]);
-            b = max(b, scratch[w * 8 + 4 + d]);
+            a = min(a, boxbuf[w * 8 + d]);
+            b = max(b, boxbuf[w * 8 + 4 + d]);
         }
         lo[d] = a;
         hi[d] = b;
@@ -1143,9 +1246,9 @@ __device__ __forceinline__ void box_reset(int *buf)
 // resets buffer call - 1 (mod 3), whose last reads all came before this
 // barrier, for use two calls later.
 template <int D>
-__device__ __forceinline__ void block_box_atomic(int (&lo)[D], int (&hi)[D], int *scratch, int call)
+__device__ __forceinline__ void block_box_atomic(int (&lo)[D], int (&hi)[D], int *boxbuf, int call)
 {
-    int *buf = scratch + call * 8;
+    int *buf = boxbuf + call * 8;
 #pragma unroll
     for (int d = 0; d < D; d++) {
         lo[d] = row_min(lo[d]);
@@ -1164,7 +1267,7 @@ __device__ __forceinline__ void block_box_atomic(int (&lo)[D], int (&hi)[D], int
         lo[d] = __builtin_amdgcn_readfirstlane(buf[d]);
         hi[d] = __builtin_amdgcn_readfirstlane(buf[4 + d]);
     }
-    if (threadIdx.x == 0) box_reset(scratch + (call == 0 ? kBoxBufs - 1 : call - 1) * 8);
+    if (threadIdx.x == 0) box_reset(boxbuf + (call == 0 ? kBoxBufs - 1 : call - 1) * 8);
 }
 
 // Exact q = n / d for 0 <= n < 2^22, 1 <= d < 2^10, from a float reciprocal.
@@ -1343,6 +1446,30 @@ __device__ __forceinline__ void march_staged(const RaySetup &R, const Anchor &A0
     }
 }
 
+// True when every cell the chunk's box [lo, hi] stages (16-bit cells hold
+// the x + 1 voxel, so one cell fewer in x) is zero in every channel: all
+// bricks of the occupancy grid that overlap the box, clipped to the padded
+// volume, are empty (cells outside it read the padding's zeros).  The chunk
+// would add exact zeros to every ray, so it can be skipped.  Every wave
+// computes the same answer from the block-uniform box, one brick per lane.
+template <int ST>
+__device__ __forceinline__ bool chunk_empty(const MarchArgs &P, const int (&lo)[3], const int (&hi)[3])
+{
+    const int x0 = max(lo[0] + kVoxPadLo, 0), x1 = min(hi[0] + kVoxPadLo - (ST == kStF32 ? 0 : 1), P.pcx - 1);
+    const int y0 = max(lo[1] + kVoxPadLo, 0), y1 = min(hi[1] + kVoxPadLo, P.pcy - 1);
+    const int z0 = max(lo[2] + kVoxPadLo, 0), z1 = min(hi[2] + kVoxPadLo, P.pcz - 1);
+    if (x0 > x1 || y0 > y1 || z0 > z1) return true;
+    const int bx0 = x0 / kOccBrick, by0 = y0 / kOccBrick, bz0 = z0 / kOccBrick;
+    const int nbx = x1 / kOccBrick - bx0 + 1, nby = y1 / kOccBrick - by0 + 1, nbz = z1 / kOccBrick - bz0 + 1;
+    if (nbx > 4 || nby > 4 || nbz > 4) return false; // a box this large is marched
+    const int lane = threadIdx.x & 63;
+    const int jx = lane & 3, jy = (lane >> 2) & 3, jz = lane >> 4;
+    bool any = false;
+    if (jx < nbx && jy < nby && jz < nbz)
+        any = P.occ[((int64_t)(bz0 + jz) * P.ob_y + (by0 + jy)) * P.ob_x + (bx0 + jx)] != 0;
+    return __builtin_amdgcn_ballot_w64(any) == 0;
+}
+
 // Ray-march of one block's rays with the volume staged in LDS per chunk of
 // samples: the rays move together along the common sample lattice; for each
 // chunk of `chunk` samples the box of cells they touch is loaded once,
@@ -1350,10 +1477,12 @@ __device__ __forceinline__ void march_staged(const RaySetup &R, const Anchor &A0
 // box exceeds the LDS budget reads from global memory instead.  Each ray still
 // accumulates in ascending k, so the result equals march_kernel's bit for bit.
 // Every thread of the block must call it (it synchronises the block).
-template <int M, int ST, bool W>
+// kCount: no fill and no march; `marched` receives the lane's samples in the
+// chunks that would be marched (the decisions are the march's own).
+template <int M, int ST, bool W, bool kCount = false>
 __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup &R,
-                                             __amdgpu_buffer_rsrc_t rsrc, char *lds, int *scratch,
-                                             float (&acc)[M])
+                                             __amdgpu_buffer_rsrc_t rsrc, char *lds, int *boxbuf,
+                                             float (&acc)[M], int *marched = nullptr)
 {
     const uint32_t oy = P.row_y & 0xFFFFFFu, oz = P.row_z & 0xFFFFFFu;
     const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
@@ -1363,11 +1492,11 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
     // ago; the barrier below publishes the reset buffers
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int b = 0; b < kBoxBufs; b++) box_reset(scratch + 8 * b);
+        for (int b = 0; b < kBoxBufs; b++) box_reset(boxbuf + 8 * b);
     }
     __syncthreads();
     int call = 0;
-    block_box_atomic<1>(kr_lo, kr_hi, scratch, call);
+    block_box_atomic<1>(kr_lo, kr_hi, boxbuf, call);
     const int kb_hi = kr_hi[0];
     // the chunk's first anchor walks in exact fixed-point steps
     const int kstart = kr_lo[0] & ~(kAnchor - 1);
@@ -1394,8 +1523,13 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
         }
         // every wave has finished the previous chunk past this barrier
         call = call == kBoxBufs - 1 ? 0 : call + 1;
-        block_box_atomic<3>(lo, hi, scratch, call);
+        block_box_atomic<3>(lo, hi, boxbuf, call);
         if (hi[0] < lo[0]) continue; // no lane of the block samples this chunk (block-uniform)
+        if (P.occ && chunk_empty<ST>(P, lo, hi)) continue; // all zeros (block-uniform)
+        if constexpr (kCount) {
+            *marched += max(0, c_hi - c_lo);
+            continue;
+        }
         // 16-bit cells are x-pairs that already hold the +1 x corner
         const int sx = hi[0] - lo[0] + (ST == kStF32 ? 1 : 0), sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
         // (a box of a volume past 4 GiB is staged only if its planes span
@@ -1484,6 +1618,34 @@ __global__ __launch_bounds__(kBlock) void count_kernel(MarchArgs P, unsigned lon
     }
 }
 
+// Samples in the chunks the LDS march evaluates (rays' windows, less the
+// chunks it skips as empty), summed per workgroup into *count.
+template <int ST>
+__global__ __launch_bounds__(kBlock) void count_marched_kernel(MarchArgs P, unsigned long long *count)
+{
+    __shared__ int boxbuf[64];
+    __shared__ unsigned long long part[kBlock / 64];
+    const uint32_t logical = xcd_logical(P);
+    if (logical >= P.n_blocks) return;
+    const TileLane t = tile_lane(P, logical);
+    const RaySetup R = lane_ray(P, t);
+    __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(P.vol), 0, (int)P.vol_bytes, 0x00020000);
+    float acc[1] = {0.0f};
+    int marched = 0;
+    march_chunks<1, ST, false, true>(P, R, rsrc, nullptr, boxbuf, acc, &marched);
+    unsigned long long n = (unsigned long long)marched;
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) n += __shfl_xor(n, s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int w = 0; w < kBlock / 64; w++) tot += part[w];
+        atomicAdd(count, tot);
+    }
+}
+
 // ------------------------------------------------------------ batches ---
 //
 // Several scans (contexts, each with its own volume, detector, views and
@@ -1561,7 +1723,7 @@ struct AdjointArgs {
 // whose box exceeds the LDS budget adds straight to global memory.
 template <int M>
 __device__ __forceinline__ void scatter_chunks(const MarchArgs &P, const RaySetup &R, const float (&g)[M],
-                                               float *grad, char *lds, int *scratch)
+                                               float *grad, char *lds, int *boxbuf)
 {
     constexpr uint32_t LW = lds_cell_bytes<M>() / 4; // floats per staged cell
     float *box = (float *)lds;
@@ -1570,7 +1732,7 @@ __device__ __forceinline__ void scatter_chunks(const MarchArgs &P, const RaySetu
     const int k_hi = R.k_lo + R.n;
     int kr_lo[3] = {R.n > 0 ? R.k_lo : INT_MAX, 0, 0}, kr_hi[3] = {R.n > 0 ? k_hi : INT_MIN, 0, 0};
     __syncthreads();
-    block_box(kr_lo, kr_hi, scratch);
+    block_box(kr_lo, kr_hi, boxbuf);
     const int kb_hi = kr_hi[0];
     auto add_global = [&](int x, int y, int z, float w) {
         if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz || w == 0.0f) return;
@@ -1593,7 +1755,7 @@ __device__ __forceinline__ void scatter_chunks(const MarchArgs &P, const RaySetu
             }
         }
         __syncthreads();
-        block_box(lo, hi, scratch);
+        block_box(lo, hi, boxbuf);
         if (hi[0] < lo[0]) continue; // no lane of the block samples this chunk (block-uniform)
         const int sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
         const bool staged = (int64_t)sx * sy * sz <= (int64_t)P.lds_cells;
@@ -1751,9 +1913,12 @@ __global__ __launch_bounds__(kBlock) void ray_grad_kernel(MarchArgs P, AdjointAr
 // voxel writes its own gradient: no atomics, and the result is deterministic.
 struct GatherArgs {
     MarchArgs P;       // geometry of the scan (views [view_begin, +n_views))
-    const float *gL;   // [n_views][M][nv][nu]
-    float *grad;       // [M][nz][ny][nx], accumulated
+    const float *gL;   // [n_views][gl_mats][nv][nu]
+    float *grad;       // [gl_mats][nz][ny][nx], accumulated
+    int32_t gl_mats;   // materials of the scan
+    int32_t mat_off;   // this launch's first material (M of them)
     int32_t bricks_x, bricks_y;
+    uint32_t brick_base; // brick of this launch's first workgroup
     int32_t cap;       // staged pixels that fit the LDS budget
 };
 
@@ -1788,9 +1953,10 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
     BrickView *s_view = (BrickView *)(lds_raw + 256); // kGatherViews entries
     // staged pixel records, `cap` of them
     float *s_px = (float *)(lds_raw + 256 + kGatherViews * sizeof(BrickView));
-    const int bx = (int)(blockIdx.x % (uint32_t)A.bricks_x);
-    const int by = (int)((blockIdx.x / (uint32_t)A.bricks_x) % (uint32_t)A.bricks_y);
-    const int bz = (int)(blockIdx.x / ((uint32_t)A.bricks_x * (uint32_t)A.bricks_y));
+    const uint32_t brick = A.brick_base + blockIdx.x;
+    const int bx = (int)(brick % (uint32_t)A.bricks_x);
+    const int by = (int)((brick / (uint32_t)A.bricks_x) % (uint32_t)A.bricks_y);
+    const int bz = (int)(brick / ((uint32_t)A.bricks_x * (uint32_t)A.bricks_y));
     const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
     const int x = bx * kBrickX + lx, y = by * kBrickY + ly, z = bz * kBrickZ + lz;
     const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
@@ -1890,7 +2056,8 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
                         rec[kGKr] = (float)(ks - kf);
                         const int64_t pix = (int64_t)pv * P.nu + pu;
 #pragma unroll
-                        for (int m = 0; m < M; m++) rec[kGG + m] = A.gL[((int64_t)vw * M + m) * npix + pix];
+                        for (int m = 0; m < M; m++)
+                            rec[kGG + m] = A.gL[((int64_t)vw * A.gl_mats + A.mat_off + m) * npix + pix];
                     }
                     __syncthreads();
                     if (!inside) continue;
@@ -1935,7 +2102,7 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
         const int64_t nvox = (int64_t)nx * ny * nz;
         const int64_t q = ((int64_t)z * ny + y) * nx + x;
 #pragma unroll
-        for (int m = 0; m < M; m++) A.grad[m * nvox + q] += acc[m];
+        for (int m = 0; m < M; m++) A.grad[(int64_t)(A.mat_off + m) * nvox + q] += acc[m];
     }
 }
 
@@ -1950,6 +2117,67 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(MarchArgs P, AdjointArg
     float g[M];
     ray_gradient<M, ST>(P, G, t, R, lds_raw, g);
     scatter_chunks<M>(P, R, g, G.grad, lds_raw + 256, (int *)lds_raw);
+}
+
+// The spectrum of every ray from its line integrals L [view][n_mat][v][u]
+// (scans of more than four materials, whose groups are marched one after the
+// other): the arithmetic of finish_ray's per-ray FP32 spectrum for any
+// n_mat.  The line integrals of the block's rays wait in LDS.
+__global__ __launch_bounds__(kBlock) void spectrum_kernel(MarchArgs P, const float *__restrict__ L, int n_mat)
+{
+    __shared__ float sL[kMaxMaterials][kBlock];
+    const int64_t npix = (int64_t)P.nu * P.nv;
+    const int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (q >= (int64_t)P.n_views * npix) return;
+    const int64_t vw = q / npix, pix = q % npix;
+    const int u = (int)(pix % P.nu), v = (int)(pix / P.nu);
+    for (int m = 0; m < n_mat; m++) sL[m][threadIdx.x] = L[(vw * n_mat + m) * npix + pix];
+    const float *wp = P.W + (int64_t)v * P.w_stride_v + u;
+    KSum sI = ksum_zero(), sA = ksum_zero();
+    for (int e = 0; e < P.n_act; e++) {
+        float tau = 0.0f;
+        for (int m = 0; m < n_mat; m++) tau = fmaf(P.A[e * n_mat + m], sL[m][threadIdx.x], tau);
+        float ex, ab;
+        attenuate(tau, ex, ab);
+        const float w = wp[(int64_t)e * P.w_stride_e];
+        ksum_add(sI, w * ex);
+        ksum_add(sA, w * ab);
+    }
+    P.I[q] = ksum_value(sI);
+    P.IA[q] = ksum_value(sA);
+}
+
+// dl/dL_m * dx of every ray from its line integrals L [view][n_mat][v][u] and
+// the upstream gradients (the arithmetic of ray_gradient for any n_mat), into
+// gL [view][n_mat][v][u].  The per-material sums wait in LDS.
+__global__ __launch_bounds__(kBlock) void ray_grad_from_l_kernel(MarchArgs P, AdjointArgs G, const float *__restrict__ L,
+                                                                 int n_mat, float *__restrict__ gL)
+{
+    __shared__ float sL[kMaxMaterials][kBlock];
+    __shared__ KSum sm[kMaxMaterials][kBlock];
+    const int64_t npix = (int64_t)P.nu * P.nv;
+    const int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (q >= (int64_t)P.n_views * npix) return;
+    const int64_t vw = q / npix, pix = q % npix;
+    const int u = (int)(pix % P.nu), v = (int)(pix / P.nu);
+    const float c = (G.gIA ? G.gIA[q] : 0.0f) - (G.gI ? G.gI[q] : 0.0f);
+    for (int m = 0; m < n_mat; m++) {
+        sL[m][threadIdx.x] = L[(vw * n_mat + m) * npix + pix];
+        sm[m][threadIdx.x] = ksum_zero();
+    }
+    if (c != 0.0f) {
+        const float *wp = P.W + (int64_t)v * P.w_stride_v + u;
+        for (int e = 0; e < P.n_act; e++) {
+            float tau = 0.0f;
+            for (int m = 0; m < n_mat; m++) tau = fmaf(P.A[e * n_mat + m], sL[m][threadIdx.x], tau);
+            float ex, ab;
+            attenuate(tau, ex, ab);
+            const float we = wp[(int64_t)e * P.w_stride_e] * ex;
+            for (int m = 0; m < n_mat; m++) ksum_add(sm[m][threadIdx.x], P.A[e * n_mat + m] * we);
+        }
+    }
+    for (int m = 0; m < n_mat; m++)
+        gL[(vw * n_mat + m) * npix + pix] = c != 0.0f ? c * ksum_value(sm[m][threadIdx.x]) * P.dx_f : 0.0f;
 }
 
 using MarchFn = void (*)(MarchArgs);
@@ -2049,25 +2277,79 @@ int check_ctx(const ctfp_context *ctx)
     return CTFP_OK;
 }
 
+// Channel groups of a scan of n_mat materials: fours, then the rest.
+void plan_groups(int n_mat, int &n_groups, int (&mats)[kMaxGroups], int (&moff)[kMaxGroups])
+{
+    n_groups = 0;
+    for (int m = 0; m < n_mat; m += kGroupMats) {
+        mats[n_groups] = std::min(kGroupMats, n_mat - m);
+        moff[n_groups] = m;
+        n_groups++;
+    }
+}
+
+int check_n_mat(int n_mat)
+{
+    if (n_mat < 1 || n_mat > kMaxMaterials)
+        return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d: 1..%d materials are supported", n_mat, kMaxMaterials);
+    return CTFP_OK;
+}
+
 int alloc_volume(ctfp_context *ctx, int n_mat)
 {
-    if (n_mat < 1 || n_mat > 4)
-        return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d: 1..4 materials are supported", n_mat);
+    if (check_n_mat(n_mat)) return CTFP_ERR_UNSUPPORTED;
     const ctfp_scan_params &s = ctx->scan;
     int64_t px = s.p_num_x + kVoxPadLo + kVoxPadHi;
     int64_t py = s.p_num_y + kVoxPadLo + kVoxPadHi;
     int64_t pz = s.p_num_z + kVoxPadLo + kVoxPadHi;
-    size_t bytes = (size_t)(px * py * pz) * (size_t)n_mat * sizeof(float);
-    if ((size_t)(px * py) * n_mat * sizeof(float) >= (1u << 24))
-        return set_error(CTFP_ERR_UNSUPPORTED, "one padded z slice exceeds 16 MiB");
+    int n_groups = 0, mats[kMaxGroups] = {}, moff[kMaxGroups] = {};
+    plan_groups(n_mat, n_groups, mats, moff);
+    size_t off[kMaxGroups] = {}, gbytes[kMaxGroups] = {};
+    size_t bytes = 0;
+    for (int g = 0; g < n_groups; g++) {
+        if ((size_t)(px * py) * mats[g] * sizeof(float) >= (1u << 24))
+            return set_error(CTFP_ERR_UNSUPPORTED, "one padded z slice exceeds 16 MiB");
+        off[g] = bytes;
+        gbytes[g] = (size_t)(px * py * pz) * (size_t)mats[g] * sizeof(float);
+        bytes += (gbytes[g] + 255) / 256 * 256;
+    }
     if (ctx->vol && (ctx->vol_bytes != bytes)) dfree(ctx->vol);
     if (!ctx->vol) HIP_TRY(hipMalloc(&ctx->vol, bytes));
     ctx->vol_bytes = bytes;
+    const int64_t obx = (px + kOccBrick - 1) / kOccBrick, oby = (py + kOccBrick - 1) / kOccBrick;
+    const int64_t obz = (pz + kOccBrick - 1) / kOccBrick;
+    if (ctx->occ && (ctx->ob_x != obx || ctx->ob_y != oby || ctx->ob_z != obz || ctx->n_groups != n_groups))
+        dfree(ctx->occ);
+    if (!ctx->occ) HIP_TRY(hipMalloc(&ctx->occ, (size_t)(obx * oby * obz) * n_groups));
+    ctx->ob_x = obx;
+    ctx->ob_y = oby;
+    ctx->ob_z = obz;
     ctx->n_mat = n_mat;
+    ctx->n_groups = n_groups;
+    for (int g = 0; g < n_groups; g++) {
+        ctx->group_mats[g] = mats[g];
+        ctx->group_moff[g] = moff[g];
+        ctx->group_off[g] = off[g];
+        ctx->group_bytes[g] = gbytes[g];
+    }
     ctx->px = px;
     ctx->py = py;
     ctx->pz = pz;
     return CTFP_OK;
+}
+
+// Workgroups of one launch: the grid's work-items (blocks x 256) must stay
+// below 2^32, and the march launches a multiple of 8 blocks.
+constexpr uint64_t kMaxLaunchBlocks = ((1ull << 32) - 1) / kBlock / 8 * 8;
+
+// Views per march launch so that no launch exceeds kMaxLaunchBlocks (every
+// view has the same tiles).
+int64_t views_per_launch(const ctfp_context *ctx)
+{
+    const int64_t tile_pu = (int64_t)ctx->tune.wave_u * ctx->tune.block_u;
+    const int64_t tile_pv = (int64_t)(64 / ctx->tune.wave_u) * (4 / ctx->tune.block_u);
+    const int64_t tiles = ((ctx->scan.d_num_u + tile_pu - 1) / tile_pu) * ((ctx->scan.d_num_v + tile_pv - 1) / tile_pv);
+    return std::max<int64_t>(1, (int64_t)kMaxLaunchBlocks / std::max<int64_t>(1, tiles));
 }
 
 int ilog2(int v)
@@ -2077,7 +2359,9 @@ int ilog2(int v)
     return r;
 }
 
-int fill_march_args(ctfp_context *ctx, int64_t view_begin, int64_t view_count, MarchArgs &P)
+// Launch arguments of the march over views [view_begin, +view_count) of
+// channel group `group` (its line integrals go to rows group_moff.. of L).
+int fill_march_args(ctfp_context *ctx, int64_t view_begin, int64_t view_count, MarchArgs &P, int group = 0)
 {
     const ctfp_scan_params &s = ctx->scan;
     const ctfp_system_info &y = ctx->sys;
@@ -2088,13 +2372,22 @@ int fill_march_args(ctfp_context *ctx, int64_t view_begin, int64_t view_count, M
                          (long long)s.proj_num);
     if (y.int_num < 1 || y.int_num > (1 << 30))
         return set_error(CTFP_ERR_INVALID, "samples per ray %lld out of range", (long long)y.int_num);
+    if (group < 0 || group >= ctx->n_groups) return set_error(CTFP_ERR_INVALID, "material group %d", group);
     std::memset(&P, 0, sizeof P);
-    const uint32_t VB = (uint32_t)(ctx->n_mat * sizeof(float));
-    P.vol = ctx->vol;
+    const uint32_t VB = (uint32_t)(ctx->group_mats[group] * sizeof(float));
+    P.vol = (const char *)ctx->vol + ctx->group_off[group];
     // volumes of 4 GiB and more read through per-box buffers and 64-bit
     // addresses; below that, one buffer covers the volume
-    P.wide = ctx->vol_bytes >= 0xFFFFFFFFull ? 1 : 0;
-    P.vol_bytes = P.wide ? 0xFFFFFFFFu : (uint32_t)ctx->vol_bytes;
+    P.wide = ctx->group_bytes[group] >= 0xFFFFFFFFull ? 1 : 0;
+    P.vol_bytes = P.wide ? 0xFFFFFFFFu : (uint32_t)ctx->group_bytes[group];
+    P.l_mats = ctx->n_mat;
+    P.l_moff = ctx->group_moff[group];
+    P.occ = ctx->tune.skip_empty && ctx->occ ? ctx->occ + (size_t)group * ctx->ob_x * ctx->ob_y * ctx->ob_z : nullptr;
+    P.ob_x = (int32_t)ctx->ob_x;
+    P.ob_y = (int32_t)ctx->ob_y;
+    P.pcx = (int32_t)ctx->px;
+    P.pcy = (int32_t)ctx->py;
+    P.pcz = (int32_t)ctx->pz;
     P.row_y = (uint32_t)(ctx->px * VB);
     P.row_z = (uint32_t)(ctx->px * ctx->py * VB);
     P.nx = (float)s.p_num_x;
@@ -2127,8 +2420,9 @@ int fill_march_args(ctfp_context *ctx, int64_t view_begin, int64_t view_count, M
     P.tiles_u = (int32_t)((s.d_num_u + tile_pu - 1) / tile_pu);
     P.tiles_v = (int32_t)((s.d_num_v + tile_pv - 1) / tile_pv);
     uint64_t nb = (uint64_t)view_count * P.tiles_u * P.tiles_v;
-    if (nb * 1ull >= (1ull << 31))
-        return set_error(CTFP_ERR_INVALID, "too many tiles in one launch (%llu)", (unsigned long long)nb);
+    if (nb > kMaxLaunchBlocks)
+        return set_error(CTFP_ERR_INVALID, "too many tiles in one launch (%llu > %llu)", (unsigned long long)nb,
+                         (unsigned long long)kMaxLaunchBlocks);
     P.n_blocks = (uint32_t)nb;
     P.per_xcd = (uint32_t)((nb + 7) / 8);
     P.dx_f = y.dx;
@@ -2141,8 +2435,6 @@ int fill_march_args(ctfp_context *ctx, int64_t view_begin, int64_t view_count, M
 
 extern "C" {
 
-const char *ctfp_last_error(void) { return ctfp::last_error(); }
-const char *ctfp_version(void) { return "ctfp 0.1.0 (gfx950)"; }
 
 int ctfp_create(const ctfp_scan_params *scan, int device, ctfp_context **out)
 {
@@ -2183,8 +2475,12 @@ int ctfp_destroy(ctfp_context *ctx)
 {
     if (!ctx) return CTFP_OK;
     DeviceGuard g(ctx->device);
-    (void)hipDeviceSynchronize();
+    // hipFree waits for the device's work before it releases memory; the
+    // caller must not free a context that other threads still launch on
     dfree(ctx->vol);
+    dfree(ctx->occ);
+    dfree(ctx->d_flag);
+    dfree(ctx->l_ws);
     dfree(ctx->adj_ws);
     dfree(ctx->focal);
     dfree(ctx->Aq);
@@ -2205,28 +2501,56 @@ int ctfp_get_system_info(const ctfp_context *ctx, ctfp_system_info *out)
     return CTFP_OK;
 }
 
-// Writes the resident layout for `dtype` from n_mat FP32 device volumes on
-// `stream`; waits for it unless `async`.
-static int write_layout(ctfp_context *ctx, const VolPtrs &vp, int n_mat, int dtype,
+// Writes the resident layout for `dtype` from n_mat FP32 device volumes
+// (chan[m], [z][y][x]) on `stream`, one interleaved volume per channel group.
+// Unless `async` it waits for the layout (on `stream` only) and rejects
+// volumes holding a NaN or an infinity (the context then holds no volume).
+static int write_layout(ctfp_context *ctx, const float *const *chan, int n_mat, int dtype,
                         hipStream_t stream = nullptr, bool async = false)
 {
     int rc = alloc_volume(ctx, n_mat);
     if (rc) return rc;
+    if (!async && !ctx->d_flag) HIP_TRY(hipMalloc(&ctx->d_flag, sizeof(int)));
+    int *bad = async ? nullptr : ctx->d_flag;
+    if (bad) HIP_TRY(hipMemsetAsync(bad, 0, sizeof(int), stream));
     const ctfp_scan_params &s = ctx->scan;
     const int64_t total = ctx->px * ctx->py * ctx->pz;
-    if (dtype == CTFP_F32)
-        layout_voxels_f32<<<grid_for(total), 256, 0, stream>>>(vp, (float *)ctx->vol, n_mat, s.p_num_x, s.p_num_y,
-                                                    s.p_num_z, ctx->px, ctx->py, ctx->pz);
-    else if (dtype == CTFP_F16)
-        layout_pairs_16<false><<<grid_for(total), 256, 0, stream>>>(vp, (uint32_t *)ctx->vol, n_mat, s.p_num_x,
-                                                         s.p_num_y, s.p_num_z, ctx->px, ctx->py, ctx->pz);
-    else
-        layout_pairs_16<true><<<grid_for(total), 256, 0, stream>>>(vp, (uint32_t *)ctx->vol, n_mat, s.p_num_x,
-                                                        s.p_num_y, s.p_num_z, ctx->px, ctx->py, ctx->pz);
-    HIP_TRY(hipGetLastError());
-    if (!async) HIP_TRY(hipDeviceSynchronize());
+    for (int g = 0; g < ctx->n_groups; g++) {
+        VolPtrs vp{};
+        const int mg = ctx->group_mats[g], m0 = ctx->group_moff[g];
+        for (int m = 0; m < mg; m++) vp.p[m] = chan[m0 + m];
+        char *dst = (char *)ctx->vol + ctx->group_off[g];
+        if (dtype == CTFP_F32)
+            layout_voxels_f32<<<grid_for(total), 256, 0, stream>>>(vp, (float *)dst, mg, s.p_num_x, s.p_num_y,
+                                                                   s.p_num_z, ctx->px, ctx->py, ctx->pz, bad);
+        else if (dtype == CTFP_F16)
+            layout_pairs_16<false><<<grid_for(total), 256, 0, stream>>>(vp, (uint32_t *)dst, mg, m0, s.p_num_x,
+                                                                        s.p_num_y, s.p_num_z, ctx->px, ctx->py,
+                                                                        ctx->pz, bad);
+        else
+            layout_pairs_16<true><<<grid_for(total), 256, 0, stream>>>(vp, (uint32_t *)dst, mg, m0, s.p_num_x,
+                                                                       s.p_num_y, s.p_num_z, ctx->px, ctx->py,
+                                                                       ctx->pz, bad);
+        HIP_TRY(hipGetLastError());
+        const int64_t obn = ctx->ob_x * ctx->ob_y * ctx->ob_z;
+        occupancy_kernel<<<(unsigned)obn, 256, 0, stream>>>((const uint32_t *)dst, mg, ctx->px, ctx->py, ctx->pz,
+                                                            ctx->ob_x, ctx->ob_y, ctx->occ + g * obn);
+        HIP_TRY(hipGetLastError());
+    }
     ctx->dtype = dtype;
     ctx->generation++;
+    if (!async) {
+        int h = 0;
+        HIP_TRY(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (h) {
+            dfree(ctx->vol);
+            ctx->vol_bytes = 0;
+            ctx->n_mat = 0;
+            ctx->n_groups = 0;
+            return set_error(CTFP_ERR_INVALID, "volumes: a density is NaN or infinite");
+        }
+    }
     return CTFP_OK;
 }
 
@@ -2240,20 +2564,20 @@ static int check_dtype(int dtype)
 int ctfp_set_volumes(ctfp_context *ctx, int n_mat, const float *const *volumes,
                      int volumes_on_device, int storage_dtype)
 {
+    Range range_("ctfp_set_volumes");
     if (check_ctx(ctx)) return CTFP_ERR_INVALID;
     if (!volumes) return set_error(CTFP_ERR_INVALID, "null volume list");
     if (check_dtype(storage_dtype)) return CTFP_ERR_INVALID;
-    if (n_mat < 1 || n_mat > 4)
-        return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d: 1..4 materials are supported", n_mat);
+    if (check_n_mat(n_mat)) return CTFP_ERR_UNSUPPORTED;
     for (int m = 0; m < n_mat; m++)
         if (!volumes[m]) return set_error(CTFP_ERR_INVALID, "volume %d is null", m);
     DeviceGuard g(ctx->device);
     const ctfp_scan_params &s = ctx->scan;
     const int64_t nvox = s.p_num_x * s.p_num_y * s.p_num_z;
-    VolPtrs vp{};
+    const float *chan[kMaxMaterials] = {};
     float *stage = nullptr;
     if (volumes_on_device) {
-        for (int m = 0; m < n_mat; m++) vp.p[m] = volumes[m];
+        for (int m = 0; m < n_mat; m++) chan[m] = volumes[m];
     } else {
         HIP_TRY(hipMalloc(&stage, (size_t)nvox * n_mat * sizeof(float)));
         for (int m = 0; m < n_mat; m++) {
@@ -2263,10 +2587,10 @@ int ctfp_set_volumes(ctfp_context *ctx, int n_mat, const float *const *volumes,
                 (void)hipFree(stage);
                 return set_error(CTFP_ERR_DEVICE, "volume upload: %s", hipGetErrorString(e));
             }
-            vp.p[m] = stage + m * nvox;
+            chan[m] = stage + m * nvox;
         }
     }
-    int rc = write_layout(ctx, vp, n_mat, storage_dtype);
+    int rc = write_layout(ctx, chan, n_mat, storage_dtype);
     if (stage) (void)hipFree(stage);
     return rc;
 }
@@ -2274,18 +2598,15 @@ int ctfp_set_volumes(ctfp_context *ctx, int n_mat, const float *const *volumes,
 int ctfp_set_volumes_async(ctfp_context *ctx, int n_mat, const float *const *volumes,
                            int storage_dtype, void *stream)
 {
+    Range range_("ctfp_set_volumes_async");
     if (check_ctx(ctx)) return CTFP_ERR_INVALID;
     if (!volumes) return set_error(CTFP_ERR_INVALID, "null volume list");
     if (check_dtype(storage_dtype)) return CTFP_ERR_INVALID;
-    if (n_mat < 1 || n_mat > 4)
-        return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d: 1..4 materials are supported", n_mat);
-    VolPtrs vp{};
-    for (int m = 0; m < n_mat; m++) {
+    if (check_n_mat(n_mat)) return CTFP_ERR_UNSUPPORTED;
+    for (int m = 0; m < n_mat; m++)
         if (!volumes[m]) return set_error(CTFP_ERR_INVALID, "volume %d is null", m);
-        vp.p[m] = volumes[m];
-    }
     DeviceGuard g(ctx->device);
-    return write_layout(ctx, vp, n_mat, storage_dtype, (hipStream_t)stream, true);
+    return write_layout(ctx, volumes, n_mat, storage_dtype, (hipStream_t)stream, true);
 }
 
 int ctfp_set_focal_spot(ctfp_context *ctx, const float *offset_u, const float *offset_v)
@@ -2329,6 +2650,8 @@ int ctfp_device_bytes(const ctfp_context *ctx, int64_t *bytes)
     if (ctx->view_cs) b += ctx->scan.proj_num * (int64_t)sizeof(double2);
     if (ctx->dI) b += 2 * ctx->staged_views * npix * (int64_t)sizeof(float);
     if (ctx->adj_ws) b += (int64_t)ctx->adj_ws_bytes;
+    if (ctx->l_ws) b += (int64_t)ctx->l_ws_bytes;
+    if (ctx->occ) b += ctx->ob_x * ctx->ob_y * ctx->ob_z * ctx->n_groups;
     *bytes = b;
     return CTFP_OK;
 }
@@ -2346,8 +2669,7 @@ int ctfp_set_labels(ctfp_context *ctx, const uint8_t *labels, int labels_on_devi
     if (check_ctx(ctx)) return CTFP_ERR_INVALID;
     if (!labels || !density) return set_error(CTFP_ERR_INVALID, "null labels or densities");
     if (check_dtype(storage_dtype)) return CTFP_ERR_INVALID;
-    if (n_mat < 1 || n_mat > 4)
-        return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d: 1..4 materials are supported", n_mat);
+    if (check_n_mat(n_mat)) return CTFP_ERR_UNSUPPORTED;
     DeviceGuard g(ctx->device);
     const int channels = single_material ? 1 : n_mat;
     const ctfp_scan_params &s = ctx->scan;
@@ -2370,9 +2692,9 @@ int ctfp_set_labels(ctfp_context *ctx, const uint8_t *labels, int labels_on_devi
     }
     int rc = CTFP_OK;
     if (e == hipSuccess) {
-        VolPtrs vp{};
-        for (int m = 0; m < channels; m++) vp.p[m] = raw + m * nvox;
-        rc = write_layout(ctx, vp, channels, storage_dtype);
+        const float *chan[kMaxMaterials] = {};
+        for (int m = 0; m < channels; m++) chan[m] = raw + m * nvox;
+        rc = write_layout(ctx, chan, channels, storage_dtype);
     }
     if (stage) (void)hipFree(stage);
     if (raw) (void)hipFree(raw);
@@ -2384,13 +2706,14 @@ int ctfp_set_labels(ctfp_context *ctx, const uint8_t *labels, int labels_on_devi
 
 int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
 {
+    Range range_("ctfp_set_spectral");
     if (check_ctx(ctx)) return CTFP_ERR_INVALID;
     if (!sp || !sp->spectrum || !sp->atten)
         return set_error(CTFP_ERR_INVALID, "spectral: spectrum and atten are required");
     if (sp->n_energy < 1 || sp->n_energy > 4096)
         return set_error(CTFP_ERR_INVALID, "spectral: n_energy=%d", sp->n_energy);
-    if (sp->n_mat < 1 || sp->n_mat > 4)
-        return set_error(CTFP_ERR_UNSUPPORTED, "spectral: n_mat=%d", sp->n_mat);
+    if (sp->n_mat < 1 || sp->n_mat > kMaxMaterials)
+        return set_error(CTFP_ERR_UNSUPPORTED, "spectral: n_mat=%d (1..%d)", sp->n_mat, kMaxMaterials);
     if (ctx->vol && sp->n_mat != ctx->n_mat)
         return set_error(CTFP_ERR_INVALID, "spectral: n_mat=%d but volumes have %d channels",
                          sp->n_mat, ctx->n_mat);
@@ -2399,6 +2722,29 @@ int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
     DeviceGuard g(ctx->device);
     const ctfp_scan_params &s = ctx->scan;
     const int64_t nu = s.d_num_u, nv = s.d_num_v, npix = nu * nv;
+    // every table entry must be a finite number (a NaN or an infinity would
+    // reach every ray through the energy sums)
+    {
+        const int64_t E = sp->n_energy;
+        struct Table {
+            const char *name;
+            const float *p;
+            int64_t n;
+        } tables[] = {{"spectrum", sp->spectrum, E},
+                      {"atten", sp->atten, E * sp->n_mat},
+                      {"grid", sp->grid, E * nu},
+                      {"det", sp->det, E * npix},
+                      {"scint_mu", sp->det ? nullptr : sp->scint_mu, E},
+                      {"scint_thickness", sp->det ? nullptr : (sp->scint_mu ? sp->scint_thickness : nullptr), npix}};
+        for (const Table &t : tables) {
+            if (!t.p) continue;
+            for (int64_t i = 0; i < t.n; i++)
+                if (!std::isfinite(t.p[i]))
+                    return set_error(CTFP_ERR_INVALID, "spectral: %s[%lld] is not finite", t.name, (long long)i);
+        }
+        if (!std::isfinite(sp->i0_val) || !std::isfinite(sp->det_factor))
+            return set_error(CTFP_ERR_INVALID, "spectral: i0_val or det_factor is not finite");
+    }
     // Active bins: a zero spectrum entry contributes exactly zero.
     std::vector<int> act;
     for (int e = 0; e < sp->n_energy; e++)
@@ -2420,13 +2766,21 @@ int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
     HIP_TRY(hipMalloc(&ctx->A, A.size() * sizeof(float)));
     HIP_TRY(hipMemcpy(ctx->A, A.data(), A.size() * sizeof(float), hipMemcpyHostToDevice));
     {
-        // MFMA operand: row e, k = 3m + (0, 1, 2) = (hi, hi, lo) of 2^8 A[e][m]
+        // MFMA operand: row e, k = 3m + (0, 1, 2) = (hi, hi, lo) of 2^s A[e][m],
+        // 2^s the largest power of two with 2^s max|A| <= 2^15 (FP16 holds
+        // 2^s A without overflow; the split's lo part of the largest entries
+        // stays normal)
+        double amax = 0.0;
+        for (float a : A) amax = std::max(amax, (double)std::fabs(a));
+        const int sc = amax > 0.0 ? std::min(40, std::max(-40, (int)std::floor(std::log2(32768.0 / amax)))) : 0;
+        const float scale = std::ldexp(1.0f, sc);
+        ctx->aq_inv_scale = std::ldexp(1.0f, -sc);
         const int rows = std::max(32, (na + 31) / 32 * 32);
         std::vector<_Float16> Aq((size_t)rows * 16, (_Float16)0.0f);
         if (3 * sp->n_mat <= 16) {
             for (int i = 0; i < na; i++) {
                 for (int m = 0; m < sp->n_mat; m++) {
-                    const float a = A[(size_t)i * sp->n_mat + m] * 256.0f;
+                    const float a = A[(size_t)i * sp->n_mat + m] * scale;
                     const _Float16 hi = (_Float16)a;
                     Aq[(size_t)i * 16 + 3 * m] = hi;
                     Aq[(size_t)i * 16 + 3 * m + 1] = hi;
@@ -2491,57 +2845,162 @@ int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
                                                    nv, ctx->I0);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipDeviceSynchronize();
+    // the tables are built on the null stream; wait for that stream only
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     cleanup();
     if (e != hipSuccess) return set_error(CTFP_ERR_DEVICE, "spectral set-up: %s", hipGetErrorString(e));
     return CTFP_OK;
 }
 
-// Launch arguments of a forward over views [view_begin, view_begin + count).
+// The spectral tables of the context, checked against its volumes.
+static int spectral_args(ctfp_context *ctx, MarchArgs &P)
+{
+    if (!ctx->W || !ctx->A) return set_error(CTFP_ERR_STATE, "spectral weights not set");
+    if (ctx->spec_n_mat != ctx->n_mat)
+        return set_error(CTFP_ERR_STATE, "spectral tables are for %d materials, volumes have %d", ctx->spec_n_mat,
+                         ctx->n_mat);
+    P.n_act = ctx->n_act;
+    P.A = ctx->A;
+    P.Aq = ctx->Aq;
+    P.aq_inv_scale = ctx->aq_inv_scale;
+    P.W = ctx->W;
+    P.w_stride_e = ctx->w_stride_e;
+    P.w_stride_v = ctx->w_stride_v;
+    return CTFP_OK;
+}
+
+// Launch arguments of the march of channel group `group` over views
+// [view_begin, view_begin + count): with the spectrum (I, IA; one group
+// only) and / or the line integrals into L [count][n_mat][v][u].
 static int forward_args(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *I,
-                        float *IA, float *L, bool spectrum, MarchArgs &P)
+                        float *IA, float *L, bool spectrum, MarchArgs &P, int group = 0)
 {
     if (check_ctx(ctx)) return CTFP_ERR_INVALID;
-    int rc = fill_march_args(ctx, view_begin, view_count, P);
+    int rc = fill_march_args(ctx, view_begin, view_count, P, group);
     if (rc) return rc;
     if (spectrum) {
-        if (!ctx->W || !ctx->A) return set_error(CTFP_ERR_STATE, "spectral weights not set");
-        if (ctx->spec_n_mat != ctx->n_mat)
-            return set_error(CTFP_ERR_STATE, "spectral tables are for %d materials, volumes have %d",
-                             ctx->spec_n_mat, ctx->n_mat);
+        if ((rc = spectral_args(ctx, P))) return rc;
         if (!I || !IA) return set_error(CTFP_ERR_INVALID, "null output");
-        P.n_act = ctx->n_act;
-        P.A = ctx->A;
-        P.Aq = ctx->Aq;
-        P.W = ctx->W;
-        P.w_stride_e = ctx->w_stride_e;
-        P.w_stride_v = ctx->w_stride_v;
     } else if (!L) {
         return set_error(CTFP_ERR_INVALID, "null output");
     }
     P.I = I;
     P.IA = IA;
     P.L = L;
-    const uint32_t lb = ctx->n_mat == 3 ? 16u : 4u * (uint32_t)ctx->n_mat;
+    const int mg = ctx->group_mats[group];
+    const uint32_t lb = mg == 3 ? 16u : 4u * (uint32_t)mg;
     P.chunk = ctx->tune.chunk;
     P.lds_cells = (int32_t)((ctx->tune.lds_bytes - 256) / lb);
     return CTFP_OK;
 }
 
+// The march of one channel group, in launches of at most views_per_launch
+// views.  I / IA / L point at the first view's rows.
+static int launch_group(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *I, float *IA, float *L,
+                        void *stream, bool spectrum, int group)
+{
+    const bool lds = ctx->tune.march == CTFP_MARCH_LDS;
+    if (!ctx->vol) return set_error(CTFP_ERR_STATE, "no volumes uploaded");
+    MarchFn fn = pick_march(ctx->group_mats[group], ctx->dtype, spectrum, lds);
+    if (!fn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->group_mats[group]);
+    const size_t shmem = lds ? (size_t)ctx->tune.lds_bytes : 0;
+    const int64_t npix = ctx->scan.d_num_u * ctx->scan.d_num_v;
+    const int64_t per = views_per_launch(ctx);
+    DeviceGuard g(ctx->device);
+    for (int64_t v0 = 0; v0 < std::max<int64_t>(view_count, 1); v0 += per) {
+        const int64_t n = std::min(per, view_count - v0);
+        MarchArgs P;
+        int rc = forward_args(ctx, view_begin + v0, n, I ? I + v0 * npix : nullptr, IA ? IA + v0 * npix : nullptr,
+                              L ? L + v0 * ctx->n_mat * npix : nullptr, spectrum, P, group);
+        if (rc) return rc;
+        hipLaunchKernelGGL(fn, dim3(P.per_xcd * 8), dim3(kBlock), shmem, (hipStream_t)stream, P);
+        HIP_TRY(hipGetLastError());
+    }
+    return CTFP_OK;
+}
+
+// Line-integral workspace of the context for `views` views (grouped scans).
+static int ensure_l_ws(ctfp_context *ctx, int64_t views)
+{
+    const size_t need = (size_t)views * ctx->n_mat * ctx->scan.d_num_u * ctx->scan.d_num_v * sizeof(float);
+    if (ctx->l_ws_bytes >= need) return CTFP_OK;
+    dfree(ctx->l_ws);
+    ctx->l_ws_bytes = 0;
+    HIP_TRY(hipMalloc(&ctx->l_ws, need));
+    ctx->l_ws_bytes = need;
+    return CTFP_OK;
+}
+
+// Views of a grouped scan per pass through a line-integral buffer of `bytes`.
+static int64_t l_views(const ctfp_context *ctx, size_t bytes, int64_t view_count)
+{
+    const int64_t per_view = (int64_t)ctx->n_mat * ctx->scan.d_num_u * ctx->scan.d_num_v * (int64_t)sizeof(float);
+    return std::min<int64_t>({view_count, (int64_t)bytes / per_view, views_per_launch(ctx)});
+}
+
+constexpr size_t kLWorkspaceBytes = 512ull << 20;
+
+static int grouped_forward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *I, float *IA, float *L,
+                           void *stream, float *lws, size_t lws_bytes);
+
+// ctfp_forward (spectrum) and ctfp_line_integrals.  Up to four materials:
+// one march per launch with the spectrum fused.  More: every group marched
+// into L (the caller's, or the context's workspace in view chunks), then
+// spectrum_kernel from L.
 static int launch_march(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *I,
                         float *IA, float *L, void *stream, bool spectrum)
 {
-    MarchArgs P;
-    int rc = forward_args(ctx, view_begin, view_count, I, IA, L, spectrum, P);
-    if (rc) return rc;
-    const bool lds = ctx->tune.march == CTFP_MARCH_LDS;
-    MarchFn fn = pick_march(ctx->n_mat, ctx->dtype, spectrum, lds);
-    if (!fn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->n_mat);
-    const size_t shmem = lds ? (size_t)ctx->tune.lds_bytes : 0;
-    DeviceGuard g(ctx->device);
-    dim3 grid(P.per_xcd * 8), block(kBlock);
-    hipLaunchKernelGGL(fn, grid, block, shmem, (hipStream_t)stream, P);
-    HIP_TRY(hipGetLastError());
+    if (check_ctx(ctx)) return CTFP_ERR_INVALID;
+    if (!ctx->vol) return set_error(CTFP_ERR_STATE, "no volumes uploaded");
+    if (view_begin < 0 || view_count < 1 || view_begin + view_count > ctx->scan.proj_num)
+        return set_error(CTFP_ERR_INVALID, "view range [%lld, %lld) outside [0, %lld)", (long long)view_begin,
+                         (long long)(view_begin + view_count), (long long)ctx->scan.proj_num);
+    if (ctx->n_groups <= 1) return launch_group(ctx, view_begin, view_count, I, IA, L, stream, spectrum, 0);
+    int rc;
+    if (!spectrum) {
+        for (int g = 0; g < ctx->n_groups; g++)
+            if ((rc = launch_group(ctx, view_begin, view_count, nullptr, nullptr, L, stream, false, g))) return rc;
+        return CTFP_OK;
+    }
+    if (!L) {
+        const int64_t chunk = l_views(ctx, kLWorkspaceBytes, view_count);
+        if (chunk < 1) return set_error(CTFP_ERR_UNSUPPORTED, "line integrals of one view exceed the workspace");
+        if ((rc = ensure_l_ws(ctx, chunk))) return rc;
+    }
+    return grouped_forward(ctx, view_begin, view_count, I, IA, L, stream, ctx->l_ws, ctx->l_ws_bytes);
+}
+
+// The spectrum of a scan of more than four materials: per chunk of views,
+// every group's line integrals into L (the caller's, else `lws` of
+// lws_bytes), then spectrum_kernel.  Allocates nothing.
+static int grouped_forward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *I, float *IA, float *L,
+                           void *stream, float *lws, size_t lws_bytes)
+{
+    if (!I || !IA) return set_error(CTFP_ERR_INVALID, "null output");
+    int rc;
+    MarchArgs S;
+    if ((rc = fill_march_args(ctx, view_begin, std::min(view_count, views_per_launch(ctx)), S))) return rc;
+    if ((rc = spectral_args(ctx, S))) return rc;
+    const int64_t npix = ctx->scan.d_num_u * ctx->scan.d_num_v;
+    const int64_t chunk = L ? std::min<int64_t>(view_count, views_per_launch(ctx)) : l_views(ctx, lws_bytes, view_count);
+    if (chunk < 1 || (!L && !lws))
+        return set_error(CTFP_ERR_UNSUPPORTED, "line integrals of one view exceed the workspace");
+    DeviceGuard dg(ctx->device);
+    for (int64_t v0 = 0; v0 < view_count; v0 += chunk) {
+        const int64_t n = std::min(chunk, view_count - v0);
+        float *Lc = L ? L + v0 * ctx->n_mat * npix : lws;
+        for (int g = 0; g < ctx->n_groups; g++)
+            if ((rc = launch_group(ctx, view_begin + v0, n, nullptr, nullptr, Lc, stream, false, g))) return rc;
+        MarchArgs P = S;
+        P.view_begin = view_begin + v0;
+        P.n_views = (int32_t)n;
+        P.I = I + v0 * npix;
+        P.IA = IA + v0 * npix;
+        const int64_t blocks = (n * npix + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(spectrum_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, (hipStream_t)stream, P,
+                           (const float *)Lc, ctx->n_mat);
+        HIP_TRY(hipGetLastError());
+    }
     return CTFP_OK;
 }
 
@@ -2562,6 +3021,8 @@ int ctfp_set_tuning(ctfp_context *ctx, const ctfp_tuning *t)
         return set_error(CTFP_ERR_INVALID, "tuning: block_u %d is not 1, 2 or 4", t->block_u);
     if (t->adjoint != CTFP_ADJOINT_GATHER && t->adjoint != CTFP_ADJOINT_SCATTER)
         return set_error(CTFP_ERR_INVALID, "tuning: adjoint mode %d", t->adjoint);
+    if (t->skip_empty != 0 && t->skip_empty != 1)
+        return set_error(CTFP_ERR_INVALID, "tuning: skip_empty %d is not 0 or 1", t->skip_empty);
     ctx->tune = *t;
     return CTFP_OK;
 }
@@ -2576,96 +3037,158 @@ int ctfp_get_tuning(const ctfp_context *ctx, ctfp_tuning *t)
 int ctfp_forward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *I,
                  float *IAbsorb, float *L, void *stream)
 {
+    Range range_("ctfp_forward");
     return launch_march(ctx, view_begin, view_count, I, IAbsorb, L, stream, true);
 }
 
-int ctfp_backward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, const float *gI,
-                  const float *gIAbsorb, const float *L, float *grad, void *stream)
+// The adjoint of both backward entry points.  ws == nullptr: the gather's
+// dl/dL workspace (and, for more than four materials without L, the line
+// integrals) are the context's, allocated on first use for up to 512 MiB of
+// views per group; otherwise `ws` of ws_bytes holds them and nothing is
+// allocated.
+static int backward_impl(ctfp_context *ctx, int64_t view_begin, int64_t view_count, const float *gI,
+                         const float *gIAbsorb, const float *L, float *grad, void *stream, float *ws,
+                         size_t ws_bytes)
 {
     if (check_ctx(ctx)) return CTFP_ERR_INVALID;
     MarchArgs P;
     int rc = fill_march_args(ctx, view_begin, view_count, P);
     if (rc) return rc;
-    if (!ctx->W || !ctx->A) return set_error(CTFP_ERR_STATE, "spectral weights not set");
-    if (ctx->spec_n_mat != ctx->n_mat)
-        return set_error(CTFP_ERR_STATE, "spectral tables are for %d materials, volumes have %d",
-                         ctx->spec_n_mat, ctx->n_mat);
+    if ((rc = spectral_args(ctx, P))) return rc;
     if (!grad) return set_error(CTFP_ERR_INVALID, "null gradient output");
-    P.n_act = ctx->n_act;
-    P.A = ctx->A;
-    P.W = ctx->W;
-    P.w_stride_e = ctx->w_stride_e;
-    P.w_stride_v = ctx->w_stride_v;
-    const uint32_t lb = ctx->n_mat == 3 ? 16u : 4u * (uint32_t)ctx->n_mat;
+    const bool grouped = ctx->n_groups > 1;
     // CTFP_MARCH_GLOBAL: no chunk is staged; every sample gathers and
     // scatters through global memory (the LDS then only holds the box scratch)
     const bool global = ctx->tune.march == CTFP_MARCH_GLOBAL;
     const size_t shmem = global ? 256 : (size_t)ctx->tune.lds_bytes;
-    P.chunk = ctx->tune.chunk;
-    P.lds_cells = global ? 0 : (int32_t)((ctx->tune.lds_bytes - 256) / lb);
+    const ctfp_scan_params &s = ctx->scan;
+    const int64_t npix = s.d_num_u * s.d_num_v;
     DeviceGuard g(ctx->device);
     if (ctx->tune.adjoint == CTFP_ADJOINT_SCATTER) {
-        AdjointArgs G{gI, gIAbsorb, L, grad};
+        if (grouped)
+            return set_error(CTFP_ERR_UNSUPPORTED, "the scatter adjoint takes up to %d materials (%d given); use "
+                             "the gather", kGroupMats, ctx->n_mat);
         AdjointFn fn = pick_adjoint(ctx->n_mat, ctx->dtype);
         if (!fn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->n_mat);
-        hipLaunchKernelGGL(fn, dim3(P.per_xcd * 8), dim3(kBlock), shmem, (hipStream_t)stream, P, G);
-        HIP_TRY(hipGetLastError());
+        const int64_t per = views_per_launch(ctx);
+        for (int64_t v0 = 0; v0 < view_count; v0 += per) {
+            const int64_t n = std::min(per, view_count - v0);
+            MarchArgs Q;
+            if ((rc = fill_march_args(ctx, view_begin + v0, n, Q))) return rc;
+            if ((rc = spectral_args(ctx, Q))) return rc;
+            const uint32_t lb = ctx->n_mat == 3 ? 16u : 4u * (uint32_t)ctx->n_mat;
+            Q.chunk = ctx->tune.chunk;
+            Q.lds_cells = global ? 0 : (int32_t)((ctx->tune.lds_bytes - 256) / lb);
+            AdjointArgs G{gI ? gI + v0 * npix : nullptr, gIAbsorb ? gIAbsorb + v0 * npix : nullptr,
+                          L ? L + v0 * ctx->n_mat * npix : nullptr, grad};
+            hipLaunchKernelGGL(fn, dim3(Q.per_xcd * 8), dim3(kBlock), shmem, (hipStream_t)stream, Q, G);
+            HIP_TRY(hipGetLastError());
+        }
         return CTFP_OK;
     }
     // gather: per group of views, the ray pass writes dl/dL * dx into the
     // workspace and the voxel pass adds its backprojection to grad
-    RayGradFn rfn = pick_ray_grad(ctx->n_mat, ctx->dtype);
-    GatherFn gfn = pick_gather(ctx->n_mat);
-    if (!rfn || !gfn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->n_mat);
-    const ctfp_scan_params &s = ctx->scan;
-    const int64_t npix = s.d_num_u * s.d_num_v;
+    RayGradFn rfn = grouped ? nullptr : pick_ray_grad(ctx->n_mat, ctx->dtype);
+    if (!grouped && !rfn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->n_mat);
     const int64_t per_view = (int64_t)ctx->n_mat * npix * (int64_t)sizeof(float);
-    const int64_t group = std::max<int64_t>(1, std::min<int64_t>(view_count, (512ll << 20) / per_view));
-    const size_t ws = (size_t)(group * per_view);
-    if (ctx->adj_ws_bytes < ws) {
-        dfree(ctx->adj_ws);
-        ctx->adj_ws_bytes = 0;
-        HIP_TRY(hipMalloc(&ctx->adj_ws, ws));
-        ctx->adj_ws_bytes = ws;
+    // more than four materials without L: the line integrals of the group
+    // wait in a second buffer of the same size
+    const bool need_l = grouped && !L;
+    const int64_t per_view_ws = per_view * (need_l ? 2 : 1);
+    int64_t group;
+    float *lbuf = nullptr;
+    if (ws) {
+        group = std::min<int64_t>({view_count, (int64_t)ws_bytes / per_view_ws, views_per_launch(ctx)});
+        if (group < 1)
+            return set_error(CTFP_ERR_UNSUPPORTED,
+                             "dl/dL of one view (%lld bytes) exceeds the %lld-byte static workspace",
+                             (long long)per_view_ws, (long long)ws_bytes);
+        if (need_l) lbuf = ws + group * (per_view / (int64_t)sizeof(float));
+    } else {
+        group = std::max<int64_t>(1, std::min<int64_t>({view_count, (512ll << 20) / per_view, views_per_launch(ctx)}));
+        const size_t need = (size_t)(group * per_view);
+        if (ctx->adj_ws_bytes < need) {
+            dfree(ctx->adj_ws);
+            ctx->adj_ws_bytes = 0;
+            HIP_TRY(hipMalloc(&ctx->adj_ws, need));
+            ctx->adj_ws_bytes = need;
+        }
+        ws = ctx->adj_ws;
+        if (need_l) {
+            if ((rc = ensure_l_ws(ctx, group))) return rc;
+            lbuf = ctx->l_ws;
+        }
     }
     const int gather_lds = 40 * 1024;
     GatherArgs A;
     A.grad = grad;
-    A.gL = ctx->adj_ws;
+    A.gL = ws;
+    A.gl_mats = ctx->n_mat;
     A.bricks_x = (int32_t)((s.p_num_x + kBrickX - 1) / kBrickX);
     A.bricks_y = (int32_t)((s.p_num_y + kBrickY - 1) / kBrickY);
     const int64_t bricks_z = (s.p_num_z + kBrickZ - 1) / kBrickZ;
     const int64_t bricks = (int64_t)A.bricks_x * A.bricks_y * bricks_z;
     if (bricks >= (1ll << 31)) return set_error(CTFP_ERR_UNSUPPORTED, "volume too large for the gather");
-    A.cap = (int32_t)((gather_lds - 256 - kGatherViews * (int)sizeof(BrickView)) /
-                      (gather_pixel_words(ctx->n_mat) * 4));
     for (int64_t v0 = 0; v0 < view_count; v0 += group) {
         const int64_t n = std::min(group, view_count - v0);
-        MarchArgs Q = P;
-        rc = fill_march_args(ctx, view_begin + v0, n, Q);
-        if (rc) return rc;
-        Q.n_act = P.n_act;
-        Q.A = P.A;
-        Q.W = P.W;
-        Q.w_stride_e = P.w_stride_e;
-        Q.w_stride_v = P.w_stride_v;
-        Q.chunk = P.chunk;
-        Q.lds_cells = P.lds_cells;
         AdjointArgs G{gI ? gI + v0 * npix : nullptr, gIAbsorb ? gIAbsorb + v0 * npix : nullptr,
                       L ? L + v0 * ctx->n_mat * npix : nullptr, grad};
-        hipLaunchKernelGGL(rfn, dim3(Q.per_xcd * 8), dim3(kBlock), shmem, (hipStream_t)stream, Q, G,
-                           ctx->adj_ws);
-        HIP_TRY(hipGetLastError());
-        A.P = Q;
-        hipLaunchKernelGGL(gfn, dim3((unsigned)bricks), dim3(kBlock), (size_t)gather_lds, (hipStream_t)stream, A);
-        HIP_TRY(hipGetLastError());
+        if (!grouped) {
+            MarchArgs Q;
+            if ((rc = fill_march_args(ctx, view_begin + v0, n, Q))) return rc;
+            if ((rc = spectral_args(ctx, Q))) return rc;
+            const uint32_t lb = ctx->n_mat == 3 ? 16u : 4u * (uint32_t)ctx->n_mat;
+            Q.chunk = ctx->tune.chunk;
+            Q.lds_cells = global ? 0 : (int32_t)((ctx->tune.lds_bytes - 256) / lb);
+            hipLaunchKernelGGL(rfn, dim3(Q.per_xcd * 8), dim3(kBlock), shmem, (hipStream_t)stream, Q, G, ws);
+            HIP_TRY(hipGetLastError());
+        } else {
+            const float *Lc = G.L;
+            if (need_l) {
+                for (int gi = 0; gi < ctx->n_groups; gi++)
+                    if ((rc = launch_group(ctx, view_begin + v0, n, nullptr, nullptr, lbuf, stream, false, gi)))
+                        return rc;
+                Lc = lbuf;
+            }
+            MarchArgs Q;
+            if ((rc = fill_march_args(ctx, view_begin + v0, n, Q))) return rc;
+            if ((rc = spectral_args(ctx, Q))) return rc;
+            const int64_t blocks = (n * npix + kBlock - 1) / kBlock;
+            hipLaunchKernelGGL(ray_grad_from_l_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, (hipStream_t)stream,
+                               Q, G, Lc, ctx->n_mat, ws);
+            HIP_TRY(hipGetLastError());
+        }
+        // the voxel pass, per channel group, in launches of at most
+        // kMaxLaunchBlocks bricks
+        for (int gi = 0; gi < ctx->n_groups; gi++) {
+            GatherFn gfn = pick_gather(ctx->group_mats[gi]);
+            if (!gfn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->group_mats[gi]);
+            if ((rc = fill_march_args(ctx, view_begin + v0, n, A.P, gi))) return rc;
+            A.mat_off = ctx->group_moff[gi];
+            A.cap = (int32_t)((gather_lds - 256 - kGatherViews * (int)sizeof(BrickView)) /
+                              (gather_pixel_words(ctx->group_mats[gi]) * 4));
+            for (int64_t b0 = 0; b0 < bricks; b0 += (int64_t)kMaxLaunchBlocks) {
+                A.brick_base = (uint32_t)b0;
+                const int64_t nb = std::min<int64_t>((int64_t)kMaxLaunchBlocks, bricks - b0);
+                hipLaunchKernelGGL(gfn, dim3((unsigned)nb), dim3(kBlock), (size_t)gather_lds, (hipStream_t)stream, A);
+                HIP_TRY(hipGetLastError());
+            }
+        }
     }
     return CTFP_OK;
+}
+
+int ctfp_backward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, const float *gI,
+                  const float *gIAbsorb, const float *L, float *grad, void *stream)
+{
+    Range range_("ctfp_backward");
+    return backward_impl(ctx, view_begin, view_count, gI, gIAbsorb, L, grad, stream, nullptr, 0);
 }
 
 int ctfp_line_integrals(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *L,
                         void *stream)
 {
+    Range range_("ctfp_line_integrals");
     return launch_march(ctx, view_begin, view_count, nullptr, nullptr, L, stream, false);
 }
 
@@ -2683,6 +3206,7 @@ int ctfp_flat_field(ctfp_context *ctx, float *I0, void *stream)
 
 int ctfp_forward_host(ctfp_context *ctx, float *I, float *IAbsorb, float *I0)
 {
+    Range range_("ctfp_forward_host");
     if (check_ctx(ctx)) return CTFP_ERR_INVALID;
     if (!I || !IAbsorb) return set_error(CTFP_ERR_INVALID, "null output");
     DeviceGuard g(ctx->device);
@@ -2736,16 +3260,58 @@ int ctfp_count_samples(ctfp_context *ctx, int64_t view_begin, int64_t view_count
 {
     if (check_ctx(ctx) || !marched) return set_error(CTFP_ERR_INVALID, "null argument");
     MarchArgs P;
-    int rc = fill_march_args(ctx, view_begin, view_count, P);
+    int rc = fill_march_args(ctx, view_begin, std::min(view_count, views_per_launch(ctx)), P);
     if (rc) return rc;
+    if (view_count > 0 && (rc = fill_march_args(ctx, view_begin + view_count - 1, 1, P))) return rc;
     DeviceGuard g(ctx->device);
     unsigned long long *d = nullptr;
     HIP_TRY(hipMalloc(&d, sizeof(unsigned long long)));
     unsigned long long h = 0;
     hipError_t e = hipMemcpy(d, &h, sizeof h, hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
+    const int64_t per = views_per_launch(ctx);
+    for (int64_t v0 = 0; v0 < view_count && e == hipSuccess; v0 += per) {
+        if ((rc = fill_march_args(ctx, view_begin + v0, std::min(per, view_count - v0), P))) {
+            (void)hipFree(d);
+            return rc;
+        }
         count_kernel<<<P.n_blocks, kBlock>>>(P, d);
         e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return set_error(CTFP_ERR_DEVICE, "count: %s", hipGetErrorString(e));
+    *marched = (int64_t)h;
+    return CTFP_OK;
+}
+
+int ctfp_count_marched(ctfp_context *ctx, int64_t view_begin, int64_t view_count, int64_t *marched)
+{
+    if (check_ctx(ctx) || !marched) return set_error(CTFP_ERR_INVALID, "null argument");
+    if (!ctx->vol) return set_error(CTFP_ERR_STATE, "no volumes uploaded");
+    if (view_begin < 0 || view_count < 1 || view_begin + view_count > ctx->scan.proj_num)
+        return set_error(CTFP_ERR_INVALID, "view range [%lld, %lld) outside [0, %lld)", (long long)view_begin,
+                         (long long)(view_begin + view_count), (long long)ctx->scan.proj_num);
+    DeviceGuard g(ctx->device);
+    unsigned long long *d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(unsigned long long)));
+    unsigned long long h = 0;
+    hipError_t e = hipMemcpy(d, &h, sizeof h, hipMemcpyHostToDevice);
+    const int64_t per = views_per_launch(ctx);
+    for (int gi = 0; gi < ctx->n_groups; gi++) {
+        for (int64_t v0 = 0; v0 < view_count && e == hipSuccess; v0 += per) {
+            MarchArgs P;
+            int rc = forward_args(ctx, view_begin + v0, std::min(per, view_count - v0), nullptr, nullptr,
+                                  (float *)d, false, P, gi);
+            if (rc) {
+                (void)hipFree(d);
+                return rc;
+            }
+            if (ctx->dtype == CTFP_F32)
+                count_marched_kernel<kStF32><<<P.per_xcd * 8, kBlock>>>(P, d);
+            else
+                count_marched_kernel<kStF16><<<P.per_xcd * 8, kBlock>>>(P, d);
+            e = hipGetLastError();
+        }
     }
     if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost);
     (void)hipFree(d);
@@ -2758,6 +3324,7 @@ int ctfp_count_samples(ctfp_context *ctx, int64_t view_begin, int64_t view_count
 
 int ctfp_autotune(ctfp_context *ctx, int max_lds_bytes, ctfp_tuning *best)
 {
+    Range range_("ctfp_autotune");
     if (check_ctx(ctx)) return CTFP_ERR_INVALID;
     if (!ctx->vol || !ctx->W) return set_error(CTFP_ERR_STATE, "autotune needs volumes and spectral tables");
     const ctfp_scan_params &s = ctx->scan;
@@ -2862,6 +3429,11 @@ struct ctfp_batch {
     std::vector<uint64_t> gens; // generation of each context at creation (checked per run)
     std::vector<Group> groups;
     uint32_t *d_taken = nullptr; // [groups][kXcds]
+    // items of more than four materials: their group marches and spectrum
+    // pass follow the persistent launches in the graph, through l_ws
+    std::vector<int> grouped;
+    float *l_ws = nullptr;
+    size_t l_ws_bytes = 0;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -2880,6 +3452,7 @@ void batch_free(ctfp_batch *b)
         dfree(g.d_base);
     }
     dfree(b->d_taken);
+    dfree(b->l_ws);
     delete b;
 }
 
@@ -2905,6 +3478,8 @@ int ctfp_batch_create(const ctfp_batch_item *items, int n_items, ctfp_batch **ou
     const ctfp_context *c0 = items[0].ctx;
     if (check_ctx(c0)) return CTFP_ERR_INVALID;
     std::vector<BatchItem> h(n_items);
+    std::vector<int> grouped;
+    size_t l_ws_bytes = 0;
     int64_t samples = 0;
     for (int i = 0; i < n_items; i++) {
         ctfp_context *c = items[i].ctx;
@@ -2915,19 +3490,37 @@ int ctfp_batch_create(const ctfp_batch_item *items, int n_items, ctfp_batch **ou
             return set_error(CTFP_ERR_INVALID, "batch: item %d is tuned for the global march", i);
         if (c->spec_n_mat != c->n_mat || !c->W)
             return set_error(CTFP_ERR_STATE, "batch: item %d has no spectral tables for its volumes", i);
-        if (c->vol_bytes >= 0xFFFFFFFFull)
-            return set_error(CTFP_ERR_UNSUPPORTED,
-                             "batch: item %d's volume spans 4 GiB or more (use ctfp_forward for it)", i);
-        int rc = forward_args(c, items[i].view_begin, items[i].view_count, items[i].I, items[i].IAbsorb,
-                              nullptr, true, h[i].P);
-        if (rc) return rc;
-        h[i].n_mat = c->n_mat;
+        if (c->n_groups > 1) {
+            // more than four materials: checked here, run after the
+            // persistent launches through a line-integral workspace
+            MarchArgs P;
+            const int64_t vb = items[i].view_begin, vc = items[i].view_count;
+            if (vb < 0 || vc < 1 || vb + vc > c->scan.proj_num)
+                return set_error(CTFP_ERR_INVALID, "batch: item %d's view range is outside its scan", i);
+            int rc = fill_march_args(c, vb, 1, P);
+            if (!rc) rc = spectral_args(c, P);
+            if (rc) return rc;
+            if (!items[i].I || !items[i].IAbsorb) return set_error(CTFP_ERR_INVALID, "batch: null output");
+            const size_t per_view = (size_t)c->n_mat * c->scan.d_num_u * c->scan.d_num_v * sizeof(float);
+            l_ws_bytes = std::max(l_ws_bytes, per_view * (size_t)std::max<int64_t>(1, l_views(c, kLWorkspaceBytes, vc)));
+            grouped.push_back(i);
+        } else {
+            if (c->vol_bytes >= 0xFFFFFFFFull)
+                return set_error(CTFP_ERR_UNSUPPORTED,
+                                 "batch: item %d's volume spans 4 GiB or more (use ctfp_forward for it)", i);
+            int rc = forward_args(c, items[i].view_begin, items[i].view_count, items[i].I, items[i].IAbsorb,
+                                  nullptr, true, h[i].P);
+            if (rc) return rc;
+            h[i].n_mat = c->n_mat;
+        }
         samples += items[i].view_count * c->scan.d_num_u * c->scan.d_num_v * c->sys.int_num;
     }
+    auto is_grouped = [&](int i) { return std::find(grouped.begin(), grouped.end(), i) != grouped.end(); };
     // groups by LDS budget, in order of first appearance
     std::vector<int> budgets;
     for (int i = 0; i < n_items; i++)
-        if (std::find(budgets.begin(), budgets.end(), items[i].ctx->tune.lds_bytes) == budgets.end())
+        if (!is_grouped(i) &&
+            std::find(budgets.begin(), budgets.end(), items[i].ctx->tune.lds_bytes) == budgets.end())
             budgets.push_back(items[i].ctx->tune.lds_bytes);
     DeviceGuard g(c0->device);
     BatchFn fn = pick_batch(c0->dtype);
@@ -2945,10 +3538,16 @@ int ctfp_batch_create(const ctfp_batch_item *items, int n_items, ctfp_batch **ou
         batch_free(b);
         return set_error(code, "batch: %s", what);
     };
+    b->grouped = grouped;
+    if (l_ws_bytes) {
+        e = hipMalloc(&b->l_ws, l_ws_bytes);
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? CTFP_ERR_NOMEM : CTFP_ERR_DEVICE, hipGetErrorString(e));
+        b->l_ws_bytes = l_ws_bytes;
+    }
     for (int budget : budgets) {
         std::vector<BatchItem> gi;
         for (int i = 0; i < n_items; i++)
-            if (items[i].ctx->tune.lds_bytes == budget) gi.push_back(h[i]);
+            if (!is_grouped(i) && items[i].ctx->tune.lds_bytes == budget) gi.push_back(h[i]);
         const int n = (int)gi.size();
         std::vector<uint32_t> base((size_t)kXcds * (n + 1), 0u);
         for (int i = 0; i < n; i++) {
@@ -2979,22 +3578,37 @@ int ctfp_batch_create(const ctfp_batch_item *items, int n_items, ctfp_batch **ou
                                          hipGetErrorString(e));
     }
     const int ng = (int)b->groups.size();
-    e = hipMalloc(&b->d_taken, (size_t)ng * kXcds * sizeof(uint32_t));
-    // the launch sequence (zero every queue counter, then one persistent
-    // kernel per group) is captured once on a private stream and replayed
+    if (ng > 0) e = hipMalloc(&b->d_taken, (size_t)ng * kXcds * sizeof(uint32_t));
+    // the launch sequence (zero every queue counter, one persistent kernel
+    // per group, then the items of more than four materials) is captured
+    // once on a private stream and replayed
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal);
+    int grc = CTFP_OK;
     if (e == hipSuccess) {
-        batch_reset<<<1, 256, 0, b->cap_stream>>>(b->d_taken, ng * kXcds);
-        hipError_t launch = hipGetLastError();
+        hipError_t launch = hipSuccess;
+        if (ng > 0) {
+            batch_reset<<<1, 256, 0, b->cap_stream>>>(b->d_taken, ng * kXcds);
+            launch = hipGetLastError();
+        }
         for (int k = 0; k < ng && launch == hipSuccess; k++) {
             const ctfp_batch::Group &grp = b->groups[k];
             BatchQueues Q{b->d_taken + k * kXcds, grp.d_base, grp.n};
             hipLaunchKernelGGL(fn, dim3(grp.grid), dim3(kBlock), grp.shmem, b->cap_stream, grp.d_items, Q);
             launch = hipGetLastError();
         }
+        for (size_t k = 0; k < grouped.size() && launch == hipSuccess && grc == CTFP_OK; k++) {
+            const ctfp_batch_item &it = items[grouped[k]];
+            grc = grouped_forward(it.ctx, it.view_begin, it.view_count, it.I, it.IAbsorb, nullptr, b->cap_stream,
+                                  b->l_ws, b->l_ws_bytes);
+        }
         e = hipStreamEndCapture(b->cap_stream, &b->graph);
         if (e == hipSuccess) e = launch;
+    }
+    if (grc != CTFP_OK) {
+        const std::string msg = ctfp::last_error();
+        batch_free(b);
+        return set_error(grc, "batch: %s", msg.c_str());
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
     if (e != hipSuccess)
@@ -3005,6 +3619,7 @@ int ctfp_batch_create(const ctfp_batch_item *items, int n_items, ctfp_batch **ou
 
 int ctfp_batch_run(ctfp_batch *b, void *stream)
 {
+    Range range_("ctfp_batch_run");
     if (!b) return set_error(CTFP_ERR_INVALID, "null batch");
     for (int i = 0; i < b->n_items; i++)
         if (b->items[i].ctx->generation != b->gens[i])

@@ -212,6 +212,24 @@ int ctfp_grid_transmission(const ctfp_grid_params *g, int n_energy, const float 
     return CTFP_OK;
 }
 
+int ctfp_grid_unit_heights(const ctfp_grid_params *g, float *H, int64_t capacity, int64_t *count)
+{
+    GridLayout L;
+    int rc = grid_layout(g, L);
+    if (rc) return rc;
+    if (!count || (capacity > 0 && !H)) return set_error(CTFP_ERR_INVALID, "grid heights: bad output");
+    *count = L.G;
+    // Slant heights (um), Grid.cpp:76-87, in the same FP32 arithmetic as the
+    // transmission's units.
+    const float h = g->height_mm * 1000, FD = g->focal_mm * 1000;
+    const double half = (double)((float)L.G / 2);
+    for (int64_t i = 0; i < std::min(capacity, L.G); i++) {
+        float dc = (float)((double)i + 0.5 - half);
+        H[i] = h / FD * sqrtf(FD * FD + dc * dc);
+    }
+    return CTFP_OK;
+}
+
 int ctfp_grid_period_index(int64_t d_num, const float *T, int64_t *out, int64_t capacity, int64_t *count)
 {
     if (d_num < 2 || !T || !count) return set_error(CTFP_ERR_INVALID, "period index: bad arguments");

@@ -51,8 +51,8 @@ void check_densities(const torch::Tensor &rho, const Scan &sc)
     TORCH_CHECK(rho.is_cuda(), "ctfp: densities must be on the GPU");
     TORCH_CHECK(rho.scalar_type() == torch::kFloat32, "ctfp: densities must be float32");
     TORCH_CHECK(rho.is_contiguous(), "ctfp: densities must be contiguous");
-    TORCH_CHECK(rho.dim() == 4 && rho.size(0) >= 1 && rho.size(0) <= 4,
-                "ctfp: densities must be [n_mat (1..4)][nz][ny][nx]");
+    TORCH_CHECK(rho.dim() == 4 && rho.size(0) >= 1 && rho.size(0) <= CTFP_MAX_MATERIALS,
+                "ctfp: densities must be [n_mat (1..", CTFP_MAX_MATERIALS, ")][nz][ny][nx]");
     TORCH_CHECK(rho.size(1) == sc.s.p_num_z && rho.size(2) == sc.s.p_num_y &&
                     rho.size(3) == sc.s.p_num_x,
                 "ctfp: density volume ", rho.sizes(), " does not match the scan's phantom (",
@@ -64,7 +64,7 @@ void check_densities(const torch::Tensor &rho, const Scan &sc)
 void load(ctfp_context *c, const torch::Tensor &rho, int64_t storage_dtype, void *stream)
 {
     const int64_t nvox = rho.size(1) * rho.size(2) * rho.size(3);
-    const float *ptrs[4] = {nullptr, nullptr, nullptr, nullptr};
+    const float *ptrs[CTFP_MAX_MATERIALS] = {};
     for (int64_t m = 0; m < rho.size(0); m++) ptrs[m] = rho.data_ptr<float>() + m * nvox;
     check(ctfp_set_volumes_async(c, (int)rho.size(0), ptrs, (int)storage_dtype, stream),
           "ctfp_set_volumes_async");

@@ -1,7 +1,9 @@
 #include "CBCTForwardProj.h"
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <filesystem>
 #include <fstream>
 #include <iostream>
@@ -24,8 +26,17 @@ void readTable(const std::string &path, int n, float *out, const char *what)
     std::ifstream f(nativePath(path));
     if (!f.is_open()) throw Error(CTFP_ERR_IO, std::string("cannot open ") + what + " table " + path);
     int i = 0;
-    float v;
-    while (i < n && (f >> v)) out[i++] = v;
+    std::string tok;
+    while (i < n && (f >> tok)) {
+        // every value must be a finite number: a NaN or an infinity would
+        // reach every ray through the energy sums
+        char *end = nullptr;
+        const float v = std::strtof(tok.c_str(), &end);
+        if (end == tok.c_str() || *end != '\0' || !std::isfinite(v))
+            throw Error(CTFP_ERR_INVALID, std::string(what) + " table " + path + ": value " + std::to_string(i + 1) +
+                                              " (\"" + tok + "\") is not a finite number");
+        out[i++] = v;
+    }
     if (i < n)
         throw Error(CTFP_ERR_IO, std::string(what) + " table " + path + " has " + std::to_string(i) +
                                      " values, " + std::to_string(n) + " needed");
@@ -118,8 +129,9 @@ void CBCTForwardProj::allocHost(bool withGrid)
     vADet.assign(E, 0.0f);
     I = vI.data();
     IAbsorb = vIA.data();
-    h_mForwardProj.I = vI.data();
-    h_mForwardProj.IAbsorb = vIA.data();
+    // per-bin intensities (addIandIAbsorb): none, the device sums the bins
+    h_mForwardProj.I = nullptr;
+    h_mForwardProj.IAbsorb = nullptr;
     h_mForwardProj.I0 = vI0.data();
     h_mForwardProj.spectrumNormal = vSpec.data();
     h_mForwardProj.detResponse = vDet.data();
@@ -197,7 +209,19 @@ void CBCTForwardProj::saveADetResponse()
     for (int e = 0; e < mCTScanParas.specEnergyNum; e++) f << aDetResponse[e] << "\n";
 }
 
-void CBCTForwardProj::addIandIAbsorb() {}
+// CBCTForwardProj.cpp:65-76: adds one energy bin's intensities
+// (h_mForwardProj.I / IAbsorb, [projNum][dNumV][dNumU]) to the scan totals
+// I / IAbsorb.  For callers that drive the per-energy loop themselves; the
+// scans of this class sum every bin on the device and leave the per-bin
+// pointers null, and then this adds nothing.
+void CBCTForwardProj::addIandIAbsorb()
+{
+    const size_t n = vI.size();
+    if (h_mForwardProj.I && h_mForwardProj.I != I)
+        for (size_t i = 0; i < n; i++) I[i] += h_mForwardProj.I[i];
+    if (h_mForwardProj.IAbsorb && h_mForwardProj.IAbsorb != IAbsorb)
+        for (size_t i = 0; i < n; i++) IAbsorb[i] += h_mForwardProj.IAbsorb[i];
+}
 
 void CBCTForwardProj::addStandardI0(float &I0Standard)
 {
@@ -421,7 +445,13 @@ void CBCTForwardProj::runScan(bool singleMaterial, bool withGrid, bool withRespo
         check(ctfp_set_focal_spot(ctx.c, h_mForwardProj.foSpOffsetU, h_mForwardProj.foSpOffsetV));
     }
     if (verbose) std::cout << "projecting " << mCTScanParas.projNum << " views, " << E << " energy bins\n";
+    // the reference times each GPU call, copies to the host included
+    // (CBCTPolyProjGPU.cu:94-101)
+    const auto t0 = std::chrono::steady_clock::now();
     check(ctfp_forward_host(ctx.c, I, IAbsorb, h_mForwardProj.I0));
+    mCTScanSystemInfo.totalTime = std::chrono::duration<float>(std::chrono::steady_clock::now() - t0).count();
+    if (verbose) std::cout << "==>> projection time (GPU, with the copy to the host): " << mCTScanSystemInfo.totalTime
+                           << " s\n";
 
     if (withResponse) { // response at the reference's "middle" pixel per bin
         for (specIndex = 0; specIndex < (size_t)E; specIndex++) {

@@ -65,6 +65,14 @@ struct CTScanSystemInfo {
     float phantomMAtten = 0;
 };
 
+// Sample coordinates of the reference's device layer (Source/DataType.h:120-126:
+// one FP32 position per sample, K per ray).  The projector sets every ray up
+// from its geometry instead and never fills these; the struct is kept so that
+// code written against the reference compiles.
+struct Coordinate {
+    float *imgIntX = nullptr, *imgIntY = nullptr, *imgIntZ = nullptr;
+};
+
 // Host arrays of one scan (the reference's h_mForwardProj bag).
 struct PolyForwardProj {
     uint8 *phantomStructure = nullptr;

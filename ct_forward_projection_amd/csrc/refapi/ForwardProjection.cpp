@@ -106,9 +106,10 @@ void ForwardProjection::_getCTScanConfig()
     sec = "PhantomMaterial";
     F.GetEntryValue(sec, "MaterialNum", -100, _mPhantomMaterial.materialNum);
     F.GetEntryValue(sec, "StructureMarkPath", "null", _mPhantomMaterial.structureMarkPath);
-    if (_mPhantomMaterial.materialNum < 1 || _mPhantomMaterial.materialNum > 4 ||
+    if (_mPhantomMaterial.materialNum < 1 || _mPhantomMaterial.materialNum > CTFP_MAX_MATERIALS ||
         _mPhantomMaterial.structureMarkPath == "null")
-        bad("[PhantomMaterial] MaterialNum (1..4) and StructureMarkPath are required");
+        bad("[PhantomMaterial] MaterialNum (1.." + std::to_string(CTFP_MAX_MATERIALS) +
+            ") and StructureMarkPath are required");
     _mPhantomMaterial.structureMarkPath = resolve(_mPhantomMaterial.structureMarkPath);
     const int M = _mPhantomMaterial.materialNum;
     delete[] _density;

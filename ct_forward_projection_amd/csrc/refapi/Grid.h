@@ -44,6 +44,17 @@ class Grid {
     void getPeriodIncex(std::vector<size_t> &gridPeriodIndex);
     // Writes the period indices, one per line, to `path`.
     void savePeriodIndex(const std::string &path) const;
+    // The slant heights (um) of the 1-um lamella units, as FP32 raw, to
+    // GridImageData/Grid_dNums..._Hight<units>.raw under the working
+    // directory (Grid.cpp:506-523).
+    void showGridH();
+    // Retrieves the period indices of the current row and writes them to
+    // GridPeriodIndex_<dNums>.txt in the working directory (Grid.cpp:525-529).
+    void showPeriodIncex();
+    // Flat field behind the grid for a square dNums x dNums detector:
+    // row[u] * mI0Photons in every row, as FP32 raw, to
+    // AddIntensityImageData/Background(Grid)_... (Grid.cpp:553-561, 583-592).
+    void creatBackground();
 
   private:
     GridAndDetectorSystem m;

@@ -66,6 +66,18 @@ def grid_transmission(d_num: int, d_size_mm: float, interspace_width: int, strip
     return T
 
 
+def grid_unit_heights(d_num: int, d_size_mm: float, interspace_width: int, strip_width: int,
+                      height_mm: float, focal_mm: float) -> np.ndarray:
+    """Slant heights (um) of the grid's 1-um lamella units (Grid.cpp:76-87)."""
+    d_um = float(np.float32(d_size_mm) * np.float32(1000))
+    g = L.GridParams(d_num, d_um, interspace_width, strip_width, height_mm, focal_mm)
+    n = C.c_int64()
+    L.check(L.lib().ctfp_grid_unit_heights(C.byref(g), None, 0, C.byref(n)))
+    H = np.empty(n.value, dtype=np.float32)
+    L.check(L.lib().ctfp_grid_unit_heights(C.byref(g), _fptr(H), H.size, C.byref(n)))
+    return H
+
+
 def focal_spot_offsets(n_views: int, spot_mm: float, seed: int = 10) -> np.ndarray:
     """Per-view focal-spot offsets [n_views][2] in mm (CBCTForwardProj.cpp:122-148,
     with the disc test u^2 + v^2 <= r^2)."""
@@ -114,16 +126,52 @@ class Projector:
         self.nu, self.nv, self.views = scan.d_num_u, scan.d_num_v, scan.proj_num
         self.n_mat = 0
         self._ctx = C.c_void_p()
+        self._graphs = 0            # autograd graphs that still use the context
+        self._close_pending = False  # close() waits for them
         L.check(L.lib().ctfp_create(C.byref(scan), device, C.byref(self._ctx)))
         info = L.SystemInfo()
-        L.check(L.lib().ctfp_get_system_info(self._ctx, C.byref(info)))
+        L.check(L.lib().ctfp_get_system_info(self._live(), C.byref(info)))
         self.info = {name: getattr(info, name) for name, _ in L.SystemInfo._fields_}
 
     # -- lifetime
     def close(self):
-        if self._ctx:
-            L.lib().ctfp_destroy(self._ctx)
-            self._ctx = C.c_void_p()
+        """Releases the context.  While an autograd graph built by
+        autograd.project still uses it, the projector is only marked closed
+        (no new calls) and the context is released with the last graph."""
+        if not self._ctx:
+            return
+        if self._graphs > 0:
+            self._close_pending = True
+            return
+        self._destroy()
+
+    def _live(self):
+        """The context pointer; raises once the projector is closed."""
+        if not self._ctx or self._close_pending:
+            raise L.CTFPError(L.CTFP_ERR_STATE, "the projector is closed")
+        return self._ctx
+
+    def _destroy(self):
+        L.lib().ctfp_destroy(self._ctx)
+        self._ctx = C.c_void_p()
+
+    def _graph_token(self):
+        """A token that an autograd graph holds while it uses this context;
+        the context is not released before every token is gone."""
+        import weakref
+
+        class _Token:
+            pass
+
+        tok = _Token()
+        self._graphs += 1
+        weakref.finalize(tok, self._graph_released)
+        return tok
+
+    def _graph_released(self):
+        self._graphs -= 1
+        if self._graphs == 0 and self._close_pending and self._ctx:
+            self._destroy()
 
     def __del__(self):
         try:
@@ -139,34 +187,38 @@ class Projector:
 
     def set_tuning(self, march: str | None = None, chunk: int | None = None,
                    lds_bytes: int | None = None, wave_u: int | None = None,
-                   block_u: int | None = None, adjoint: str | None = None):
+                   block_u: int | None = None, adjoint: str | None = None,
+                   skip_empty: bool | None = None):
         """Launch options of the march ("global" or "lds", samples per staged
-        chunk, LDS bytes per workgroup, ray-tile shape; projections are
-        identical for all) and of the adjoint ("gather" or "scatter")."""
+        chunk, LDS bytes per workgroup, ray-tile shape, skipping of chunks
+        whose cells are all zero; projections are identical for all) and of
+        the adjoint ("gather" or "scatter")."""
         t = L.Tuning()
-        L.check(L.lib().ctfp_get_tuning(self._ctx, C.byref(t)))
+        L.check(L.lib().ctfp_get_tuning(self._live(), C.byref(t)))
         if march is not None:
             t.march = {"global": L.CTFP_MARCH_GLOBAL, "lds": L.CTFP_MARCH_LDS}[march]
         if adjoint is not None:
             t.adjoint = {"gather": L.CTFP_ADJOINT_GATHER, "scatter": L.CTFP_ADJOINT_SCATTER}[adjoint]
+        if skip_empty is not None:
+            t.skip_empty = int(bool(skip_empty))
         for name, v in (("chunk", chunk), ("lds_bytes", lds_bytes), ("wave_u", wave_u),
                         ("block_u", block_u)):
             if v is not None:
                 setattr(t, name, v)
-        L.check(L.lib().ctfp_set_tuning(self._ctx, C.byref(t)))
+        L.check(L.lib().ctfp_set_tuning(self._live(), C.byref(t)))
 
     def autotune(self, max_lds_bytes: int = 0) -> dict:
         """Time launch options on a sample of this scan's views, keep the
         fastest (results do not depend on it) and return it.  max_lds_bytes
         caps the LDS budget (for scans that share a batched launch)."""
         t = L.Tuning()
-        L.check(L.lib().ctfp_autotune(self._ctx, int(max_lds_bytes), C.byref(t)))
+        L.check(L.lib().ctfp_autotune(self._live(), int(max_lds_bytes), C.byref(t)))
         return self.get_tuning()
 
     @property
     def handle(self) -> int:
         """The ctfp_context pointer as an integer (for the PyTorch extension)."""
-        if not self._ctx:
+        if not self._ctx or self._close_pending:
             raise L.CTFPError(L.CTFP_ERR_STATE, "the projector is closed")
         return int(self._ctx.value)
 
@@ -174,33 +226,42 @@ class Projector:
     def device_bytes(self) -> int:
         """Device memory held by the projector itself (volume layout, tables)."""
         b = C.c_int64()
-        L.check(L.lib().ctfp_device_bytes(self._ctx, C.byref(b)))
+        L.check(L.lib().ctfp_device_bytes(self._live(), C.byref(b)))
         return int(b.value)
 
     def get_tuning(self) -> dict:
         t = L.Tuning()
-        L.check(L.lib().ctfp_get_tuning(self._ctx, C.byref(t)))
+        L.check(L.lib().ctfp_get_tuning(self._live(), C.byref(t)))
         return {"march": "lds" if t.march == L.CTFP_MARCH_LDS else "global", "chunk": t.chunk,
                 "lds_bytes": t.lds_bytes, "wave_u": t.wave_u, "block_u": t.block_u,
-                "adjoint": "scatter" if t.adjoint == L.CTFP_ADJOINT_SCATTER else "gather"}
+                "adjoint": "scatter" if t.adjoint == L.CTFP_ADJOINT_SCATTER else "gather",
+                "skip_empty": bool(t.skip_empty)}
 
     def set_focal_spot(self, offsets=None):
         """Per-view focal-spot source offsets [proj_num][2] (mm, along the
         detector's u and v), or None for a point source."""
         if offsets is None:
-            L.check(L.lib().ctfp_set_focal_spot(self._ctx, None, None))
+            L.check(L.lib().ctfp_set_focal_spot(self._live(), None, None))
             return
         o = np.ascontiguousarray(offsets, dtype=np.float32)
         if o.shape != (self.views, 2):
             raise ValueError(f"focal offsets must be [{self.views}][2]")
         ou, ov = np.ascontiguousarray(o[:, 0]), np.ascontiguousarray(o[:, 1])
-        L.check(L.lib().ctfp_set_focal_spot(self._ctx, _fptr(ou), _fptr(ov)))
+        L.check(L.lib().ctfp_set_focal_spot(self._live(), _fptr(ou), _fptr(ov)))
 
     def marched_samples(self, view_begin: int = 0, view_count: int | None = None) -> int:
         """Samples the march evaluates (rays' windows clipped to the volume)."""
         vb, vc = self._views(view_begin, view_count)
         n = C.c_int64()
-        L.check(L.lib().ctfp_count_samples(self._ctx, vb, vc, C.byref(n)))
+        L.check(L.lib().ctfp_count_samples(self._live(), vb, vc, C.byref(n)))
+        return int(n.value)
+
+    def evaluated_samples(self, view_begin: int = 0, view_count: int | None = None) -> int:
+        """Samples the LDS march evaluates: the clipped windows less the
+        chunks skipped as empty (with the current tuning)."""
+        vb, vc = self._views(view_begin, view_count)
+        n = C.c_int64()
+        L.check(L.lib().ctfp_count_marched(self._live(), vb, vc, C.byref(n)))
         return int(n.value)
 
     @property
@@ -227,7 +288,7 @@ class Projector:
             if tuple(v.shape) != expect:
                 raise ValueError(f"volume {i} has shape {tuple(v.shape)}, expected {expect}")
             keep.append(v)
-        L.check(L.lib().ctfp_set_volumes(self._ctx, len(vols), ptrs, int(on_device), storage_dtype))
+        L.check(L.lib().ctfp_set_volumes(self._live(), len(vols), ptrs, int(on_device), storage_dtype))
         self.n_mat = len(vols)
 
     def set_labels(self, labels: np.ndarray, density, single_material: bool,
@@ -235,7 +296,7 @@ class Projector:
         lab = np.ascontiguousarray(labels, dtype=np.uint8)
         rho = np.ascontiguousarray(np.atleast_1d(density), dtype=np.float32)
         n_mat = rho.size
-        L.check(L.lib().ctfp_set_labels(self._ctx, lab.ctypes.data, 0, n_mat, _fptr(rho),
+        L.check(L.lib().ctfp_set_labels(self._live(), lab.ctypes.data, 0, n_mat, _fptr(rho),
                                         int(single_material), storage_dtype))
         self.n_mat = 1 if single_material else n_mat
 
@@ -269,7 +330,7 @@ class Projector:
             keep += [m, t]
             sp.scint_mu = _fptr(m)
             sp.scint_thickness = _fptr(t)
-        L.check(L.lib().ctfp_set_spectral(self._ctx, C.byref(sp)))
+        L.check(L.lib().ctfp_set_spectral(self._live(), C.byref(sp)))
 
     # -- compute (device)
     def _views(self, view_begin, view_count):
@@ -296,7 +357,7 @@ class Projector:
             raise ValueError("output size does not match the view range")
         if L_out is not None and L_out.numel() != vc * self.n_mat * self.nv * self.nu:
             raise ValueError("L size does not match the view range")
-        L.check(L.lib().ctfp_forward(self._ctx, vb, vc, I.data_ptr(), IA.data_ptr(),
+        L.check(L.lib().ctfp_forward(self._live(), vb, vc, I.data_ptr(), IA.data_ptr(),
                                      L_out.data_ptr() if L_out is not None else None,
                                      _stream_handle(stream)))
         return (I, IA, L_out) if want_L or L_out is not None else (I, IA)
@@ -310,7 +371,7 @@ class Projector:
                               device=torch.device("cuda", self.device))
         if out.numel() != vc * self.n_mat * self.nv * self.nu or not out.is_contiguous():
             raise ValueError("L output has the wrong size or layout")
-        L.check(L.lib().ctfp_line_integrals(self._ctx, vb, vc, out.data_ptr(),
+        L.check(L.lib().ctfp_line_integrals(self._live(), vb, vc, out.data_ptr(),
                                             _stream_handle(stream)))
         return out
 
@@ -337,7 +398,7 @@ class Projector:
                 or not grad.is_contiguous() or grad.dtype != torch.float32:
             raise ValueError("grad must be a contiguous float32 [n_mat][nz][ny][nx] tensor")
         ptr = lambda t: None if t is None else t.data_ptr()
-        L.check(L.lib().ctfp_backward(self._ctx, vb, vc, ptr(gI), ptr(gIA), ptr(L_in), grad.data_ptr(),
+        L.check(L.lib().ctfp_backward(self._live(), vb, vc, ptr(gI), ptr(gIA), ptr(L_in), grad.data_ptr(),
                                       _stream_handle(stream)))
         return grad
 
@@ -345,7 +406,7 @@ class Projector:
         torch = _torch()
         out = torch.empty((self.nv, self.nu), dtype=torch.float32,
                           device=torch.device("cuda", self.device))
-        L.check(L.lib().ctfp_flat_field(self._ctx, out.data_ptr(), _stream_handle(stream)))
+        L.check(L.lib().ctfp_flat_field(self._live(), out.data_ptr(), _stream_handle(stream)))
         return out
 
     # -- compute (host buffers)
@@ -354,7 +415,7 @@ class Projector:
         I = np.empty((self.views, self.nv, self.nu), dtype=np.float32)
         IA = np.empty_like(I)
         I0 = np.empty((self.nv, self.nu), dtype=np.float32)
-        L.check(L.lib().ctfp_forward_host(self._ctx, I.ctypes.data, IA.ctypes.data, I0.ctypes.data))
+        L.check(L.lib().ctfp_forward_host(self._live(), I.ctypes.data, IA.ctypes.data, I0.ctypes.data))
         return I, IA, I0
 
 
@@ -375,7 +436,7 @@ class Batch:
             for t in (I, IA):
                 if tuple(t.shape) != shape or t.dtype != torch.float32 or not t.is_contiguous():
                     raise ValueError(f"batch item {i}: outputs must be contiguous float32 {shape}")
-            arr[i] = L.BatchItem(proj._ctx.value, vb, vc, I.data_ptr(), IA.data_ptr())
+            arr[i] = L.BatchItem(proj._live().value, vb, vc, I.data_ptr(), IA.data_ptr())
             self._keep.append((proj, I, IA))
         self._b = C.c_void_p()
         L.check(L.lib().ctfp_batch_create(arr, len(items), C.byref(self._b)))

@@ -291,3 +291,37 @@ def config4() -> ScanSetup:
     s.det, s.d_size, s.views = (512, 512), 0.8, 180
     return s
 
+
+
+def config5_specs(seed: int = 2024) -> list:
+    """The eight uneven scans of the batched workload (seeded): cubic phantoms
+    of {128, 192, 256, 384, 512} voxels at 0.5 mm with 1-3 materials,
+    detectors from 256x256 to 1024x768, 60-360 views, 80-140 kVp spectra
+    (61-121 active 1-keV bins from 20 keV), grid on or off."""
+    rng = np.random.default_rng(seed)
+    dets = [(256, 256), (512, 384), (512, 512), (768, 576), (1024, 768)]
+    out = []
+    for k in range(8):
+        n = int(rng.choice([128, 192, 256, 384, 512]))
+        out.append(dict(n=n, n_mat=int(rng.integers(1, 4)), det=dets[int(rng.integers(0, len(dets)))],
+                        views=int(rng.integers(60, 361)), kvp=float(rng.choice(np.arange(80, 141, 10))),
+                        grid=bool(rng.random() < 0.5), seed=100 + k))
+    return out
+
+
+def config5_setup(spec: dict) -> tuple:
+    """(ScanSetup, volumes) of one scan of config5_specs: the config-2 scanner
+    and tables with the scan's phantom size, materials, detector, views,
+    tube voltage and grid; the detector spans the phantom's diagonal at 1.5x
+    magnification."""
+    s = config2()
+    n = spec["n"]
+    s.name = f"batch{spec['seed']}"
+    s.n_vox, s.length_mm = (n, n, n), (n * 0.5,) * 3
+    s.det, s.views, s.kvp = tuple(spec["det"]), spec["views"], spec["kvp"]
+    s.d_size = float(n * 0.5 * 1.45 * 1.5 / min(spec["det"]))
+    s.materials = s.materials[:spec["n_mat"]]
+    if not spec["grid"]:
+        s.grid = None
+    vols = list(three_material_phantom(n, seed=spec["seed"]))[:spec["n_mat"]]
+    return s, vols

@@ -42,7 +42,18 @@ extern "C" {
 #define CTFP_ERR_CONFIG -6      /* configuration entry missing or invalid */
 #define CTFP_ERR_UNSUPPORTED -7 /* valid request this build does not handle */
 
-/* Storage precision of the resident material volumes. */
+/* Materials per scan (the reference reads any MaterialNum,
+ * Source/ForwardProjection.cpp:290-316).  Up to four are marched in one pass;
+ * more run in groups of four, their line integrals joined by a separate
+ * spectrum pass. */
+#define CTFP_MAX_MATERIALS 16
+
+/* Storage precision of the resident material volumes.  FP16 rounds each
+ * density to nearest.  BF16 (8 significant bits) rounds each density to one
+ * of its two neighbouring BF16 values with a fixed pseudo-random threshold
+ * stratified over 2x2x2 voxel blocks (an unbiased dither), so that line
+ * integrals through uniform regions keep their mean; a single voxel may then
+ * be off by up to one BF16 step (2^-7 relative). */
 #define CTFP_F32 0
 #define CTFP_F16 1
 #define CTFP_BF16 2
@@ -116,6 +127,9 @@ typedef struct ctfp_tuning {
     int32_t wave_u;    /* ray tile: pixels along u per wavefront, 4/8/16/32 (64 / wave_u along v) */
     int32_t block_u;   /* ray tile: wavefronts along u per workgroup, 1/2/4 (4 / block_u along v) */
     int32_t adjoint;   /* ctfp_backward: CTFP_ADJOINT_GATHER (default) or CTFP_ADJOINT_SCATTER */
+    int32_t skip_empty; /* 1 (default): a staged chunk whose cell box is zero in every channel
+                           (checked on an occupancy grid of 8^3-cell bricks built with the
+                           volumes) is skipped; it would add exact zeros.  0: every chunk marched */
 } ctfp_tuning;
 #define CTFP_ADJOINT_GATHER 0  /* ray pass to dl/dL, then a voxel-driven gather: no atomics, deterministic */
 #define CTFP_ADJOINT_SCATTER 1 /* ray-driven scatter through LDS boxes and float atomics */
@@ -221,6 +235,7 @@ int ctfp_line_integrals(ctfp_context *ctx, int64_t view_begin, int64_t view_coun
 int ctfp_backward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, const float *gI,
                   const float *gIAbsorb, const float *L, float *grad, void *stream);
 
+
 /* Flat field I0 = sum_E I0Val*S_E*T_E*eta_E in bin order.  Replaces addI0 /
  * addI0NoResponse (CBCTPolyForwardProjGrid.cpp:330-350,
  * CBCTPolyForwardProjNoGrid.cpp:290-310).  I0: device [d_num_v][d_num_u]. */
@@ -237,6 +252,12 @@ int ctfp_forward_host(ctfp_context *ctx, float *I, float *IAbsorb, float *I0);
  * K per ray of the reference's lattice (the rest are exact zeros).
  * Synchronous. */
 int ctfp_count_samples(ctfp_context *ctx, int64_t view_begin, int64_t view_count, int64_t *marched);
+
+/* Samples the LDS march evaluates with the context's tuning: those of
+ * ctfp_count_samples less the samples of staged chunks skipped as empty
+ * (tuning.skip_empty), summed over the material groups (each group of a scan
+ * of more than four materials is marched on its own).  Synchronous. */
+int ctfp_count_marched(ctfp_context *ctx, int64_t view_begin, int64_t view_count, int64_t *marched);
 
 
 /* Times candidate launch options (ray-tile shape, samples per staged chunk,
@@ -292,6 +313,13 @@ int ctfp_det_response(int64_t npix, float factor, float mu_c, const float *thick
  * mu_strip, mu_inter: [n_energy] in 1/mm; T: [n_energy][d_num]. */
 int ctfp_grid_transmission(const ctfp_grid_params *grid, int n_energy, const float *mu_strip,
                            const float *mu_inter, float *T);
+
+/* Slant heights (um) of the grid's 1-um lamella units, h sqrt(FD^2 + d_i^2) / FD
+ * for unit centres d_i = i + 0.5 - G/2, G = d_num * (whole-um pitch).
+ * Replaces Grid::computeunitH (Grid.cpp:76-87), whose table Grid::showGridH
+ * writes (Grid.cpp:506-523).  H holds up to `capacity` entries; *count
+ * receives G. */
+int ctfp_grid_unit_heights(const ctfp_grid_params *grid, float *H, int64_t capacity, int64_t *count);
 
 /* Grid period indices: 0, every strict local minimum of T, d_num-1.
  * Replaces Grid::RetrievePeriodIndex (Grid.cpp:605-620).  `out` holds up to

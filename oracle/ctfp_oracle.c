@@ -714,17 +714,19 @@ int orc_num_threads(void)
  * ray (view, v, u) adds dx * gL * w_corner to the 8 corners of its cell
  * (zero-padded border: corners outside the volume are dropped).
  * gL: [n_views][n_mat][nv][nu]; grad: [n_mat][nz][ny][nx], accumulated.
- * Serial (the order of additions is fixed). */
+ * OpenMP over detector rows with atomic adds: the order in which threads
+ * meet at a voxel varies, which moves only the last bits of its FP64 sum. */
 void orc_backproject_focal_f64(const orc_geom *g, const orc_paras *s, int n_mat, int64_t view0,
                                int64_t n_views, const double *focal, const double *gL, double *grad)
 {
     const int64_t nvox = g->nx * g->ny * g->nz;
     const int64_t npix = g->nu * g->nv;
     const double t0 = (double)g->sod - (double)s->fov_r + (double)s->dx / 2;
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
     for (int64_t iv = 0; iv < n_views; iv++) {
-        double th = (double)orc_view_angle(g, s, view0 + iv);
-        double cth = cos(th), sth = sin(th);
         for (int64_t v = 0; v < g->nv; v++) {
+            const double th = (double)orc_view_angle(g, s, view0 + iv);
+            const double cth = cos(th), sth = sin(th);
             for (int64_t u = 0; u < g->nu; u++) {
                 line_f64 d = ray_line_f64(g, s, u, v, focal ? focal + 2 * (view0 + iv) : NULL);
                 double gm[16];
@@ -755,7 +757,10 @@ void orc_backproject_focal_f64(const orc_geom *g, const orc_paras *s, int n_mat,
                                     continue;
                                 double wc = w[dx_][0] * w[dy][1] * w[dz][2];
                                 int64_t q = (z * g->ny + y) * g->nx + x;
-                                for (int m = 0; m < n_mat; m++) grad[m * nvox + q] += gm[m] * wc;
+                                for (int m = 0; m < n_mat; m++) {
+#pragma omp atomic
+                                    grad[m * nvox + q] += gm[m] * wc;
+                                }
                             }
                 }
             }

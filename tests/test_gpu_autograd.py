@@ -120,3 +120,107 @@ def test_wls_directional_finite_differences(require_gpu):
                 fd = (4 * q(0.5) - q(1.0)) / 3
             an = float((g * d32.double()).sum())
             assert abs(fd - an) <= 1e-3 * abs(fd), (trial, fd, an)
+
+
+def test_projector_closed_or_dropped_before_backward(require_gpu):
+    """A graph keeps its projector: backward after close(), and after the
+    last Python reference to the projector is gone, gives the gradient of
+    an untouched projector; the context is released with the graph."""
+    import gc
+    import torch
+    s, rho_np = small_case()
+    kw = P.spectral_inputs(s, *s.det)
+    dev = torch.device("cuda")
+    with P.Projector(P.scan_params(**s.scan_kwargs())) as ref_proj:
+        ref_proj.set_spectral(**kw)
+        rho = torch.from_numpy(rho_np).to(dev).requires_grad_(True)
+        I, _ = A.project(rho, ref_proj)
+        I.sum().backward()
+        ref = rho.grad.clone()
+    # closed before backward
+    proj = P.Projector(P.scan_params(**s.scan_kwargs()))
+    proj.set_spectral(**kw)
+    rho.grad = None
+    I, IA = A.project(rho, proj)
+    proj.close()
+    with pytest.raises(P.CTFPError):
+        A.project(rho, proj)
+    I.sum().backward()
+    torch.cuda.synchronize()
+    assert torch.equal(rho.grad, ref)
+    assert proj._ctx  # still held by the graph
+    del I, IA  # the last references to the graph
+    gc.collect()
+    assert not proj._ctx
+    # dropped before backward
+    rho.grad = None
+
+    def make():
+        p = P.Projector(P.scan_params(**s.scan_kwargs()))
+        p.set_spectral(**kw)
+        return p
+
+    I, _ = A.project(rho, make())
+    gc.collect()
+    I.sum().backward()
+    torch.cuda.synchronize()
+    assert torch.equal(rho.grad, ref)
+
+
+def test_batched_module_directional_finite_differences(require_gpu):
+    """PolyProjection over two phantoms with their own projectors (different
+    detectors and view counts) and one shared projector: each output equals
+    its own projection, and the gradient of a WLS loss summed over the batch
+    agrees with central differences along a random direction to 1e-3."""
+    import torch
+    dev = torch.device("cuda")
+    cases = [small_case(n=32, views=4, det=(56, 44), seed=5), small_case(n=32, views=5, det=(48, 40), seed=6)]
+    projs, targets = [], []
+    for s, rho_np in cases:
+        pr = P.Projector(P.scan_params(**s.scan_kwargs()))
+        pr.set_spectral(**P.spectral_inputs(s, *s.det))
+        projs.append(pr)
+        I0 = pr.flat_field().double()
+        with torch.no_grad():
+            It, _ = A.project(torch.from_numpy(rho_np).to(dev), pr)
+        counts = A.poisson_target(It, seed=4).double()
+        targets.append((I0, A.log_projection(counts, I0), counts))
+    module = A.PolyProjection(projs)
+    truth = [torch.from_numpy(r).to(dev) for _, r in cases]
+    rng = np.random.default_rng(21)
+    est = [(t * torch.from_numpy(rng.uniform(0.85, 1.15, size=t.shape).astype(np.float32)).to(dev))
+           .requires_grad_(True) for t in truth]
+    I, IA = module(est)
+    assert isinstance(I, list) and len(I) == 2
+    for b in range(2):
+        ref_I, ref_IA = A.project(est[b].detach(), projs[b])
+        assert torch.equal(I[b], ref_I) and torch.equal(IA[b], ref_IA)
+
+    def loss(rs):
+        Ib, _ = module(rs)
+        return sum(A.wls_loss(A.log_projection(x.double(), t[0]), t[1], t[2]) for x, t in zip(Ib, targets))
+
+    loss(est).backward()
+    kw = P.spectral_inputs(cases[0][0], *cases[0][0].det)
+    atten = np.asarray(kw["atten"], np.float64)[np.asarray(kw["spectrum"]) > 0].max(axis=0)
+    deltas = []
+    for t in truth:
+        d = torch.from_numpy(rng.normal(size=t.shape)).to(dev) * torch.from_numpy(0.02 / atten).to(dev)[:, None, None, None]
+        deltas.append(d.float())
+    an = sum(float((e.grad.double() * d.double()).sum()) for e, d in zip(est, deltas))
+    with torch.no_grad():
+        def q(h):
+            lp = loss([e.detach() + h * d for e, d in zip(est, deltas)]).item()
+            lm = loss([e.detach() - h * d for e, d in zip(est, deltas)]).item()
+            return (lp - lm) / (2 * h)
+        fd = (4 * q(0.5) - q(1.0)) / 3
+    assert abs(fd - an) <= 1e-3 * abs(fd), (fd, an)
+    # one projector shared by a stacked batch of equal shapes
+    shared = A.PolyProjection(projs[0])
+    rb = torch.stack([truth[0], 0.9 * truth[0]])
+    Is, IAs = shared(rb)
+    assert Is.shape == (2, 4, 44, 56)
+    r0, _ = A.project(rb[1], projs[0])
+    assert torch.equal(Is[1], r0)
+    for pr in projs:
+        pr.close()

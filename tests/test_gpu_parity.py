@@ -450,11 +450,15 @@ def test_config2_full_scan_properties(require_gpu, cfg2_volumes):
 
 # ------------------------------------------------------- mixed precision
 
-# FP16 volumes meet the 1e-3 target.  BF16 keeps 8 mantissa bits: a bone or
-# iodine insert's density is quantised by up to 2^-9 (0.2%) as one systematic
-# offset per insert, which reaches ~1e-3 of p and ~4e-3 of L on the small
-# phantoms here; it is held to 5e-3 and reported, not claimed at 1e-3.
-MIXED_TOL = {"f16": 1e-3, "bf16": 5e-3}
+# FP16 volumes meet the 1e-3 target against FP64 (values rounded to nearest,
+# 2^-11, and trilinear weights rounded to nearest, 2^-11: at most ~1e-3 of a
+# line integral, and the projections stay well inside).  BF16 keeps 8
+# significant bits.  Its layout dithers each voxel to one of its two
+# neighbouring values (error below one step, 2^-7 relative) so that long
+# paths through uniform inserts keep their mean (mean error ~1e-5), but a
+# ray through a few voxels keeps up to a step of error: BF16 is held to 1e-2
+# and reported, not claimed at 1e-3.
+MIXED_TOL = {"f16": 1e-3, "bf16": 1e-2}
 
 
 def check_mixed(got, ref64, name, kind):
@@ -505,11 +509,14 @@ def test_mixed_layout_edges(require_gpu):
         vols.append(v)
     kw = P.spectral_inputs(s, *s.det)
     L32 = run_projector(s, vols, spec_kw=kw)[2]
-    # values and trilinear weights are each rounded once to 16 bits (unit
-    # roundoff u: 2^-11 for FP16, 2^-8 for BF16), so |dL| <= (2u + u^2) L
-    for dt, u in ((P.CTFP_F16, 2.0 ** -11), (P.CTFP_BF16, 2.0 ** -8)):
+    # unit roundoff u: 2^-11 for FP16, 2^-8 for BF16.  Trilinear weights are
+    # rounded to nearest (u); FP16 values to nearest (u), so
+    # |dL| <= (2u + u^2) L; BF16 values are dithered to a neighbouring value
+    # (below one step, 2u), so |dL| <= (3u + 2u^2) L
+    for dt, u, bound in ((P.CTFP_F16, 2.0 ** -11, 2 * 2.0 ** -11 + 2.0 ** -22),
+                         (P.CTFP_BF16, 2.0 ** -8, 3 * 2.0 ** -8 + 2 * 2.0 ** -16)):
         L16 = run_projector(s, vols, spec_kw=kw, dtype=dt)[2]
-        assert rel_err(L16, L32) < 2 * u + u * u + 1e-5, dt
+        assert rel_err(L16, L32) < bound + 1e-5, dt
 
 
 # -------------------------------------------------------- launch tuning
@@ -624,3 +631,50 @@ def test_focal_spot_parity(require_gpu):
     assert rel_err(np.log(I0[None] / I), np.log(I064[None] / I64), 1e-2) < TOL
     # the offsets matter: the point-source projection is far from this one
     assert rel_err(to_np(Lnom), L64) > 100 * TOL
+
+
+@pytest.mark.parametrize("dtype", [P.CTFP_F32, P.CTFP_F16, P.CTFP_BF16])
+def test_skip_empty_chunks_is_exact(require_gpu, dtype):
+    """Chunks whose staged cells are all zero are skipped: the outputs are
+    bit-identical with skipping off, for several chunk lengths and ray-tile
+    shapes, and fewer samples are evaluated than the windows hold.  An empty
+    volume evaluates nothing and gives the flat field."""
+    import torch
+    s, vols = small_three_material(n=96, views=8, det=(160, 120), d_size=0.8)
+    kw = P.spectral_inputs(s, *s.det)
+    with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
+        proj.set_volumes(vols, storage_dtype=dtype)
+        proj.set_spectral(**kw)
+        window = proj.marched_samples()
+        for chunk, wave_u, block_u in [(16, 16, 1), (32, 8, 2), (48, 32, 1)]:
+            proj.set_tuning(chunk=chunk, wave_u=wave_u, block_u=block_u, skip_empty=False)
+            ref = proj.forward(want_L=True)
+            assert proj.evaluated_samples() == window
+            proj.set_tuning(skip_empty=True)
+            got = proj.forward(want_L=True)
+            evaluated = proj.evaluated_samples()
+            torch.cuda.synchronize()
+            for a, b in zip(got, ref):
+                assert torch.equal(a, b), (chunk, wave_u, block_u)
+            assert 0 < evaluated < window, (evaluated, window)
+        # a small ball in a large empty volume: most chunks are skipped
+        n = vols[0].shape[0]
+        zz, yy, xx = np.meshgrid(*(np.arange(n) - (n - 1) / 2,) * 3, indexing="ij")
+        ball = [np.where(zz ** 2 + yy ** 2 + xx ** 2 <= 100.0, 1.0, 0.0).astype(np.float32) * f
+                for f in (1.0, 0.5, 0.01)]
+        proj.set_volumes(ball, storage_dtype=dtype)
+        proj.set_tuning(chunk=16, wave_u=16, block_u=1, skip_empty=False)
+        ref = proj.forward(want_L=True)
+        proj.set_tuning(skip_empty=True)
+        got = proj.forward(want_L=True)
+        evaluated = proj.evaluated_samples()
+        torch.cuda.synchronize()
+        for a, b in zip(got, ref):
+            assert torch.equal(a, b)
+        assert 0 < evaluated < 0.3 * window, (evaluated, window)
+        proj.set_volumes([np.zeros_like(v) for v in vols], storage_dtype=dtype)
+        I, IA = proj.forward()
+        I0 = proj.flat_field()
+        torch.cuda.synchronize()
+        assert proj.evaluated_samples() == 0
+        assert torch.equal(I, I0[None].expand_as(I)) and not IA.any()

@@ -77,6 +77,16 @@ def test_grid_matches_oracle_bitwise(nu, d_mm, w, sw):
         np.testing.assert_array_equal(T[e], ref)
 
 
+@pytest.mark.parametrize("nu,d_mm", [(64, 0.4), (63, 0.4), (61, 0.125)])
+def test_grid_unit_heights_match_oracle(nu, d_mm):
+    """The table Grid::showGridH writes (Grid.cpp:76-87, 506-523): G = nu *
+    (whole-um pitch) slant heights, bit-equal to the oracle's."""
+    H = P.grid_unit_heights(nu, d_mm, 214, 36, 2.14, 1500.0)
+    d_um = int(np.float32(d_mm) * np.float32(1000))
+    assert H.size == nu * d_um
+    np.testing.assert_array_equal(H, O.grid_unit_heights(nu * d_um, 2.14, 1500.0))
+
+
 def test_grid_rejects_degenerate():
     with pytest.raises(L.CTFPError):
         P.grid_transmission(2, 0.1, 214, 36, 2.0, 1500.0, [1.0], [0.0])

@@ -142,7 +142,7 @@ def test_testI0_matches_oracle(tmp_path):
     (("ScanAngleNum=4", "ScanAngleNum=-4"), "ScanAngleNum"),
     (("PNumY=24", "PNumY=abc"), "PNumX/Y/Z"),
     (("Density2=1.92", "Density2x=1.92"), "Density2"),
-    (("MaterialNum=2", "MaterialNum=9"), "MaterialNum"),
+    (("MaterialNum=2", "MaterialNum=17"), "MaterialNum"),
     (("Thickness=0.4", "Thickness=-0.4"), "Scintillator"),
     (("FD=600", "FD=0"), "PostGrid"),
 ])
