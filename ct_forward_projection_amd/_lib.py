@@ -36,13 +36,15 @@ EXPORTED = (
     "ctfp_get_scan", "ctfp_get_device", "ctfp_device_bytes", "ctfp_autotune",
     "ctfp_batch_create", "ctfp_batch_run", "ctfp_batch_samples", "ctfp_batch_destroy",
     "ctfp_count_samples", "ctfp_set_focal_spot", "ctfp_focal_spot_offsets",
-    "ctfp_grid_unit_heights", "ctfp_count_marched",
+    "ctfp_backward_noalloc", "ctfp_grid_unit_heights", "ctfp_count_marched",
 )
 
 CTFP_MARCH_GLOBAL = 0
 CTFP_MARCH_LDS = 1
 CTFP_ADJOINT_GATHER = 0
 CTFP_ADJOINT_SCATTER = 1
+# bytes of the static dl/dL workspace of ctfp_backward_noalloc (per GPU)
+CTFP_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 class ScanParams(C.Structure):
@@ -146,6 +148,7 @@ def _declare(lib):
         "ctfp_bench_copy": (i32, [vp, vp, i64, vp]),
         "ctfp_set_tuning": (i32, [vp, P(Tuning)]),
         "ctfp_backward": (i32, [vp, i64, i64, vp, vp, vp, vp, vp]),
+        "ctfp_backward_noalloc": (i32, [vp, i64, i64, vp, vp, vp, vp, vp]),
         "ctfp_get_tuning": (i32, [vp, P(Tuning)]),
         "ctfp_set_volumes_async": (i32, [vp, i32, P(vp), i32, vp]),
         "ctfp_get_scan": (i32, [vp, P(ScanParams)]),

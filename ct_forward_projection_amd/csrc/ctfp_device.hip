@@ -33,6 +33,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -108,6 +109,14 @@ struct ctfp_context {
     // bumped by every change of volumes or spectral tables (batches check it)
     uint64_t generation = 0;
 };
+
+// The workspace of ctfp_backward_noalloc: the gather adjoint's dl/dL of one
+// group of views, in a static device array (one copy per GPU, held until the
+// process ends) so that the call allocates nothing.  Host code never passes
+// `scratch` itself to a kernel (on the host it names a stand-in, not device
+// memory): static_scratch() resolves its device address per GPU.
+constexpr size_t kStaticScratchBytes = (size_t)CTFP_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) float scratch[kStaticScratchBytes / sizeof(float)];
 
 namespace {
 
@@ -2277,6 +2286,30 @@ int check_ctx(const ctfp_context *ctx)
     return CTFP_OK;
 }
 
+// Device address of `scratch` on `device`, which must be current: looked up
+// with hipGetSymbolAddress on the first call per device and kept.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[kMaxDevices] = {};
+
+int static_scratch(int device, float **out)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return set_error(CTFP_ERR_UNSUPPORTED, "device %d: the static workspace covers devices 0..%d", device,
+                         kMaxDevices - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(CTFP_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                             hipGetErrorString(e));
+        g_scratch_ptr[device] = (float *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return CTFP_OK;
+}
+
 // Channel groups of a scan of n_mat materials: fours, then the rest.
 void plan_groups(int n_mat, int &n_groups, int (&mats)[kMaxGroups], int (&moff)[kMaxGroups])
 {
@@ -3183,6 +3216,18 @@ int ctfp_backward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, con
 {
     Range range_("ctfp_backward");
     return backward_impl(ctx, view_begin, view_count, gI, gIAbsorb, L, grad, stream, nullptr, 0);
+}
+
+int ctfp_backward_noalloc(ctfp_context *ctx, int64_t view_begin, int64_t view_count, const float *gI,
+                          const float *gIAbsorb, const float *L, float *grad, void *stream)
+{
+    Range range_("ctfp_backward_noalloc");
+    if (check_ctx(ctx)) return CTFP_ERR_INVALID;
+    DeviceGuard g(ctx->device);
+    float *ws = nullptr;
+    int rc = static_scratch(ctx->device, &ws);
+    if (rc) return rc;
+    return backward_impl(ctx, view_begin, view_count, gI, gIAbsorb, L, grad, stream, ws, kStaticScratchBytes);
 }
 
 int ctfp_line_integrals(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *L,

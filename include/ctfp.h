@@ -235,6 +235,19 @@ int ctfp_line_integrals(ctfp_context *ctx, int64_t view_begin, int64_t view_coun
 int ctfp_backward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, const float *gI,
                   const float *gIAbsorb, const float *L, float *grad, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight training loops, harnesses that
+ * forbid hipMalloc): ctfp_backward with the gather adjoint's dl/dL workspace
+ * in a static device array of CTFP_NOALLOC_SCRATCH_BYTES per GPU (held until
+ * the process ends) instead of the context's allocation.  Calls no hipMalloc
+ * or hipFree.  Views run in groups whose dl/dL (n_mat * d_num_v * d_num_u * 4
+ * bytes per view) fits the array; a scan one view of which needs more is
+ * rejected with CTFP_ERR_UNSUPPORTED.  Every call on one GPU shares the
+ * array: calls that overlap on one GPU (several streams or host threads)
+ * race.  Arguments and results as ctfp_backward (the caller zeroes grad). */
+#define CTFP_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int ctfp_backward_noalloc(ctfp_context *ctx, int64_t view_begin, int64_t view_count, const float *gI,
+                          const float *gIAbsorb, const float *L, float *grad, void *stream);
 
 /* Flat field I0 = sum_E I0Val*S_E*T_E*eta_E in bin order.  Replaces addI0 /
  * addI0NoResponse (CBCTPolyForwardProjGrid.cpp:330-350,
