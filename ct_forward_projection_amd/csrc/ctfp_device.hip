@@ -2789,7 +2789,21 @@ int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
         S[i] = sp->spectrum[e] * (sp->energy_weighting ? (float)(e + 1) : 1.0f);
         for (int m = 0; m < sp->n_mat; m++) A[(size_t)i * sp->n_mat + m] = sp->atten[(size_t)e * sp->n_mat + m];
     }
-    const bool per_pixel = sp->det != nullptr || sp->scint_mu != nullptr;
+    // A response that does not vary along v (a scintillator of one thickness
+    // per detector column, or a response table whose rows repeat) keeps one
+    // row of weights per bin: the same values, read from a table that stays
+    // in the caches instead of one row per pixel row.
+    bool v_uniform = true;
+    if (sp->det) {
+        for (int i = 0; i < na && v_uniform; i++) {
+            const float *d = sp->det + (size_t)act[i] * npix;
+            for (int64_t q = nu; q < npix && v_uniform; q++) v_uniform = d[q] == d[q % nu];
+        }
+    } else if (sp->scint_mu) {
+        for (int64_t q = nu; q < npix && v_uniform; q++)
+            v_uniform = sp->scint_thickness[q] == sp->scint_thickness[q % nu];
+    }
+    const bool per_pixel = (sp->det != nullptr || sp->scint_mu != nullptr) && !v_uniform;
     const int64_t rows = per_pixel ? nv : 1;
     ctx->generation++; // the tables are replaced from here on
     dfree(ctx->A);
@@ -2844,19 +2858,22 @@ int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
         e = hipMalloc(&dT, T.size() * sizeof(float));
         if (e == hipSuccess) e = hipMemcpy(dT, T.data(), T.size() * sizeof(float), hipMemcpyHostToDevice);
     }
+    // rows * nu response entries per bin: every pixel, or row 0 when the
+    // response does not vary along v
+    const int64_t per = rows * nu;
     if (e == hipSuccess && sp->det && na > 0) {
-        e = hipMalloc(&dEta, (size_t)na * npix * sizeof(float));
+        e = hipMalloc(&dEta, (size_t)na * per * sizeof(float));
         for (int i = 0; i < na && e == hipSuccess; i++)
-            e = hipMemcpy(dEta + (size_t)i * npix, sp->det + (size_t)act[i] * npix,
-                          (size_t)npix * sizeof(float), hipMemcpyHostToDevice);
+            e = hipMemcpy(dEta + (size_t)i * per, sp->det + (size_t)act[i] * npix, (size_t)per * sizeof(float),
+                          hipMemcpyHostToDevice);
     } else if (e == hipSuccess && sp->scint_mu && na > 0) {
         mu.resize(na);
         for (int i = 0; i < na; i++) mu[i] = sp->scint_mu[act[i]];
         e = hipMalloc(&dMu, mu.size() * sizeof(float));
         if (e == hipSuccess) e = hipMemcpy(dMu, mu.data(), mu.size() * sizeof(float), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMalloc(&dThick, (size_t)npix * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&dThick, (size_t)per * sizeof(float));
         if (e == hipSuccess)
-            e = hipMemcpy(dThick, sp->scint_thickness, (size_t)npix * sizeof(float), hipMemcpyHostToDevice);
+            e = hipMemcpy(dThick, sp->scint_thickness, (size_t)per * sizeof(float), hipMemcpyHostToDevice);
     }
     if (e == hipSuccess && na > 0) {
         WeightArgs wa{};

@@ -716,10 +716,25 @@ int orc_num_threads(void)
  * gL: [n_views][n_mat][nv][nu]; grad: [n_mat][nz][ny][nx], accumulated.
  * OpenMP over detector rows with atomic adds: the order in which threads
  * meet at a voxel varies, which moves only the last bits of its FP64 sum. */
+/* The same accumulated into the voxels of the box [box0, box0 + boxn)
+ * (z, y, x) only: grad [n_mat][boxn_z][boxn_y][boxn_x].  For volumes whose
+ * full gradient would not fit in memory. */
+void orc_backproject_box_f64(const orc_geom *g, const orc_paras *s, int n_mat, int64_t view0,
+                             int64_t n_views, const double *focal, const double *gL, const int64_t *box0,
+                             const int64_t *boxn, double *grad);
+
 void orc_backproject_focal_f64(const orc_geom *g, const orc_paras *s, int n_mat, int64_t view0,
                                int64_t n_views, const double *focal, const double *gL, double *grad)
 {
-    const int64_t nvox = g->nx * g->ny * g->nz;
+    const int64_t box0[3] = {0, 0, 0}, boxn[3] = {g->nz, g->ny, g->nx};
+    orc_backproject_box_f64(g, s, n_mat, view0, n_views, focal, gL, box0, boxn, grad);
+}
+
+void orc_backproject_box_f64(const orc_geom *g, const orc_paras *s, int n_mat, int64_t view0,
+                             int64_t n_views, const double *focal, const double *gL, const int64_t *box0,
+                             const int64_t *boxn, double *grad)
+{
+    const int64_t nvox = boxn[0] * boxn[1] * boxn[2];
     const int64_t npix = g->nu * g->nv;
     const double t0 = (double)g->sod - (double)s->fov_r + (double)s->dx / 2;
 #pragma omp parallel for collapse(2) schedule(dynamic, 1)
@@ -755,8 +770,13 @@ void orc_backproject_focal_f64(const orc_geom *g, const orc_paras *s, int n_mat,
                                 int64_t x = ix + dx_, y = iy + dy, z = iz + dz;
                                 if (x < 0 || y < 0 || z < 0 || x >= g->nx || y >= g->ny || z >= g->nz)
                                     continue;
+                                x -= box0[2];
+                                y -= box0[1];
+                                z -= box0[0];
+                                if (x < 0 || y < 0 || z < 0 || x >= boxn[2] || y >= boxn[1] || z >= boxn[0])
+                                    continue;
                                 double wc = w[dx_][0] * w[dy][1] * w[dz][2];
-                                int64_t q = (z * g->ny + y) * g->nx + x;
+                                int64_t q = (z * boxn[1] + y) * boxn[2] + x;
                                 for (int m = 0; m < n_mat; m++) {
 #pragma omp atomic
                                     grad[m * nvox + q] += gm[m] * wc;

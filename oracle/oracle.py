@@ -70,6 +70,8 @@ def lib():
             "orc_backproject_f64": (None, [G, P, i32, i64, i64, _D, _D]),
             "orc_march_focal_f64": (None, [G, P, i32, _F, i64, i64, _D, _D, i32]),
             "orc_backproject_focal_f64": (None, [G, P, i32, i64, i64, _D, _D, _D]),
+            "orc_backproject_box_f64": (None, [G, P, i32, i64, i64, _D, _D, C.POINTER(i64),
+                                               C.POINTER(i64), _D]),
             "orc_focal_offsets": (None, [i64, C.c_float, C.c_uint32, _F, _F]),
             "orc_spectrum_grad_f64": (None, [i64, i64, i64, i32, i32, _D, _F, _F, C.c_double, _F, _F,
                                              _D, _D, _D]),
@@ -290,6 +292,18 @@ def backproject_f64(g, p, gL, view0=0, focal=None):
         fo = np.ascontiguousarray(focal, dtype=np.float64)
         lib().orc_backproject_focal_f64(C.byref(g), C.byref(p), n_mat, view0, n_views, _d(fo), _d(gL),
                                         _d(grad))
+    return grad
+
+
+def backproject_box_f64(g, p, gL, box0, boxn, view0=0):
+    """backproject_f64 into the voxel box [box0, box0 + boxn) ((z, y, x)
+    origin and extent) only: grad [M][boxn_z][boxn_y][boxn_x]."""
+    gL = np.ascontiguousarray(gL, dtype=np.float64)
+    n_views, n_mat = gL.shape[:2]
+    b0 = (C.c_int64 * 3)(*[int(v) for v in box0])
+    bn = (C.c_int64 * 3)(*[int(v) for v in boxn])
+    grad = np.zeros((n_mat,) + tuple(int(v) for v in boxn), dtype=np.float64)
+    lib().orc_backproject_box_f64(C.byref(g), C.byref(p), n_mat, view0, n_views, None, _d(gL), b0, bn, _d(grad))
     return grad
 
 

@@ -675,6 +675,86 @@ def test_skip_empty_chunks_is_exact(require_gpu, dtype):
         proj.set_volumes([np.zeros_like(v) for v in vols], storage_dtype=dtype)
         I, IA = proj.forward()
         I0 = proj.flat_field()
+        assert proj.evaluated_samples() == 0  # counted under the current tuning
+        proj.set_tuning(skip_empty=False)
+        Ir, IAr = proj.forward()
         torch.cuda.synchronize()
-        assert proj.evaluated_samples() == 0
-        assert torch.equal(I, I0[None].expand_as(I)) and not IA.any()
+        assert torch.equal(I, Ir) and torch.equal(IA, IAr) and not IA.any()
+        # the 16-bit spectrum sums bins in groups of four, the flat field bin
+        # by bin: equal up to FP32 rounding of the sum
+        assert ((I.double() - I0.double()[None]).abs() / I0.double()[None]).max().item() < 1e-6
+
+
+@pytest.fixture(scope="module")
+def wide_three_material():
+    """Three channels of 1000 x 1000 x 708 voxels: the padded FP32 layout
+    (1004 x 1004 x 712 x 12 B = 8.61 GB) spans more than 4 GiB and planes
+    z >= 352 (of 708, centre 353.5) lie past it.  Zero except seeded blocks
+    around the centre that straddle that plane."""
+    rng = np.random.default_rng(13)
+    vols = []
+    for m in range(3):
+        v = np.zeros((708, 1000, 1000), dtype=np.float32)
+        v[324:384, 470:530, 470:530] = rng.uniform(0.05, 1.5, size=(60, 60, 60)).astype(np.float32)
+        vols.append(v)
+    return vols
+
+
+BOX = (310, 460, 460, 90, 80, 80)  # z, y, x origin and extent of the checked gradient box
+
+
+def test_volume_past_4gib_three_materials(require_gpu, wide_three_material):
+    """The wide (64-bit) march with 16-byte three-material cells, inside the
+    forward and inside the adjoint's per-ray recomputation of the line
+    integrals: LDS-staged equals the global gather bit for bit in FP32 and
+    BF16, the forward meets the FP64 bar, and the gradient meets the FP64 CPU
+    adjoint on the block."""
+    import torch
+    vols = wide_three_material
+    nz, ny, nx = vols[0].shape
+    assert (nx + 4) * (ny + 4) * (nz + 4) * 12 > 2 ** 32
+    s = S.config2()
+    s.n_vox = (nx, ny, nz)
+    s.length_mm = (nx * 0.1, ny * 0.1, nz * 0.1)
+    s.det, s.d_size, s.views = (32, 48), 0.075, 10
+    kw = P.spectral_inputs(s, *s.det)
+    rng = np.random.default_rng(14)
+    gI = rng.normal(size=(2, s.det[1], s.det[0])).astype(np.float32)
+    gIA = rng.normal(size=gI.shape).astype(np.float32)
+    dev = torch.device("cuda")
+    got = {}
+    with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
+        for dtype in (P.CTFP_F32, P.CTFP_BF16):
+            print(f"past 4 GiB, three materials: dtype {dtype}", flush=True)
+            proj.set_volumes(vols, storage_dtype=dtype)
+            proj.set_spectral(**kw)
+            outs = {}
+            for march in ("lds", "global"):
+                proj.set_tuning(march=march)
+                I, IA, L = proj.forward(0, 2, want_L=True)
+                grad = proj.backward(torch.from_numpy(gI).to(dev), torch.from_numpy(gIA).to(dev),
+                                     view_begin=0, view_count=2)
+                torch.cuda.synchronize()
+                outs[march] = (I, IA, L, grad)
+            for a, b in zip(outs["lds"], outs["global"]):
+                assert torch.equal(a, b), dtype
+            I, IA, L, grad = outs["lds"]
+            got[dtype] = (to_np(I), to_np(IA), to_np(L), to_np(proj.flat_field()),
+                          grad[:, BOX[0]:BOX[0] + BOX[3], BOX[1]:BOX[1] + BOX[4], BOX[2]:BOX[2] + BOX[5]]
+                          .cpu().numpy())
+            del outs, I, IA, L, grad
+    ref32, ref64 = oracle_expect(s, vols, kw, 0, 2)
+    assert float(np.abs(ref64[2]).min()) > 0.0  # every ray crosses the blocks
+    check_all(got[P.CTFP_F32][:4], (ref32, ref64), name="volume_past_4gib_three_materials")
+    check_mixed(got[P.CTFP_BF16][:4], ref64, "volume_past_4gib_three_materials_bf16", "bf16")
+    g = O.geom_from_setup(s)
+    p = O.paras(g)
+    det = np.stack([O.det_response(kw["det_factor"], float(m), kw["scint_thickness"]) for m in kw["scint_mu"]])
+    gL = O.spectrum_grad_f64(ref64[2], kw["atten"], kw["spectrum"], kw["i0_val"], gI, gIA, kw["grid"], det)
+    # the FP64 adjoint and the backprojection of |dl/dL| on the box around
+    # the blocks (z from 310, y and x from 460; 90 x 80 x 80 voxels)
+    both = O.backproject_box_f64(g, p, np.concatenate([gL, np.abs(gL)], axis=1), BOX[:3], BOX[3:])
+    ref, scale = both[:3], both[3:]
+    assert float(np.abs(ref).max()) > 0.0
+    err = np.abs(got[P.CTFP_F32][4] - ref) / np.maximum(scale, 1e-3 * scale.max())
+    assert err.max() < 1e-4, err.max()

@@ -379,6 +379,21 @@ def test_backprojection_is_the_adjoint():
     assert lhs == pytest.approx(rhs, rel=1e-12)
 
 
+def test_backprojection_box_is_the_full_one_restricted():
+    """backproject_box_f64 gives the full backprojection's values on its box
+    (the boxes cross the volume's faces), and nothing else."""
+    g = small_geom(n=20, nu=18, nv=14, views=3, lx=20.0, dsize=1.6, sod=300.0, sdd=450.0)
+    p = O.paras(g)
+    rng = np.random.default_rng(23)
+    y = rng.normal(size=(3, 2, 14, 18))
+    full = O.backproject_f64(g, p, y)
+    for b0, bn in [((0, 0, 0), (20, 20, 20)), ((3, 5, 7), (6, 4, 9)), ((15, 0, 12), (5, 20, 8))]:
+        box = O.backproject_box_f64(g, p, y, b0, bn)
+        sub = full[:, b0[0]:b0[0] + bn[0], b0[1]:b0[1] + bn[1], b0[2]:b0[2] + bn[2]]
+        assert box.shape == (2,) + tuple(bn)
+        np.testing.assert_allclose(box, sub, rtol=1e-12, atol=1e-12 * np.abs(full).max())
+
+
 def test_spectrum_grad_matches_central_differences():
     s = S.config2()
     rng = np.random.default_rng(22)
