@@ -33,7 +33,7 @@ EXPORTED = (
     "ctfp_postprocess", "ctfp_synchronize", "ctfp_scint_thickness", "ctfp_det_response",
     "ctfp_grid_transmission", "ctfp_grid_period_index", "ctfp_bench_copy",
     "ctfp_set_tuning", "ctfp_get_tuning", "ctfp_backward", "ctfp_set_volumes_async",
-    "ctfp_get_scan", "ctfp_get_device", "ctfp_device_bytes", "ctfp_autotune",
+    "ctfp_get_scan", "ctfp_get_device", "ctfp_get_volumes", "ctfp_device_bytes", "ctfp_autotune",
     "ctfp_batch_create", "ctfp_batch_run", "ctfp_batch_samples", "ctfp_batch_destroy",
     "ctfp_count_samples", "ctfp_set_focal_spot", "ctfp_focal_spot_offsets",
     "ctfp_backward_noalloc", "ctfp_grid_unit_heights", "ctfp_count_marched",
@@ -153,6 +153,7 @@ def _declare(lib):
         "ctfp_set_volumes_async": (i32, [vp, i32, P(vp), i32, vp]),
         "ctfp_get_scan": (i32, [vp, P(ScanParams)]),
         "ctfp_get_device": (i32, [vp, P(i32)]),
+        "ctfp_get_volumes": (i32, [vp, P(i32), P(i32)]),
         "ctfp_device_bytes": (i32, [vp, P(i64)]),
         "ctfp_autotune": (i32, [vp, i32, P(Tuning)]),
         "ctfp_count_samples": (i32, [vp, i64, i64, P(i64)]),

@@ -81,6 +81,7 @@ struct ctfp_context {
     // cells, non-zero when a cell of the brick holds a non-zero channel
     uint8_t *occ = nullptr;
     int64_t ob_x = 0, ob_y = 0, ob_z = 0; // bricks per axis
+    int32_t *obox = nullptr; // [n_groups][6] bounds of the occupied bricks
     // line-integral workspace of grouped forwards and adjoints
     float *l_ws = nullptr;
     size_t l_ws_bytes = 0;
@@ -265,6 +266,7 @@ __global__ void layout_pairs_16(VolPtrs src, uint32_t *__restrict__ dst, int n_m
 // kOccBrick^3 padded cells, 1 when any dword of any cell in it is non-zero
 // (a 16-bit cell holds v(x) and v(x+1)).  One workgroup per brick.
 constexpr int kOccBrick = 8;
+constexpr int kBlock256Waves = 4; // wavefronts of a 256-thread workgroup
 __global__ __launch_bounds__(256) void occupancy_kernel(const uint32_t *__restrict__ vol, int words, int64_t px,
                                                         int64_t py, int64_t pz, int64_t obx, int64_t oby,
                                                         uint8_t *__restrict__ occ)
@@ -281,6 +283,50 @@ __global__ __launch_bounds__(256) void occupancy_kernel(const uint32_t *__restri
     }
     any = __syncthreads_or(any);
     if (threadIdx.x == 0) occ[b] = any ? 1 : 0;
+}
+
+// Bounds of the occupied bricks of every group: box[6 g + (0..5)] = min / max
+// brick x, y, z (INT_MAX / INT_MIN when no brick is occupied).
+__global__ void occupied_box_init(int32_t *box, int n)
+{
+    if ((int)threadIdx.x < n) box[threadIdx.x] = (threadIdx.x & 1) ? INT_MIN : INT_MAX;
+}
+
+__global__ __launch_bounds__(256) void occupied_box_kernel(const uint8_t *__restrict__ occ, int64_t obx,
+                                                           int64_t oby, int64_t obn, int32_t *box)
+{
+    __shared__ int32_t part[kBlock256Waves][6];
+    int32_t v[6] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+    for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < obn; b += (int64_t)gridDim.x * blockDim.x) {
+        if (!occ[b]) continue;
+        const int32_t c[3] = {(int32_t)(b % obx), (int32_t)((b / obx) % oby), (int32_t)(b / (obx * oby))};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            v[2 * a] = min(v[2 * a], c[a]);
+            v[2 * a + 1] = max(v[2 * a + 1], c[a]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 6; a++)
+#pragma unroll
+        for (int sh = 32; sh >= 1; sh >>= 1) {
+            const int32_t o = __shfl_xor(v[a], sh);
+            v[a] = (a & 1) ? max(v[a], o) : min(v[a], o);
+        }
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int a = 0; a < 6; a++) part[threadIdx.x >> 6][a] = v[a];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        int32_t r = part[0][a];
+        for (int w = 1; w < kBlock256Waves; w++) r = (a & 1) ? max(r, part[w][a]) : min(r, part[w][a]);
+        if (a & 1) {
+            if (r != INT_MIN) atomicMax(box + a, r);
+        } else if (r != INT_MAX) {
+            atomicMin(box + a, r);
+        }
+    }
 }
 
 // Density channels [m][z][y][x] from a label volume.
@@ -425,6 +471,9 @@ struct MarchArgs {
     const uint8_t *occ;
     int32_t ob_x, ob_y;
     int32_t pcx, pcy, pcz;
+    // bounds of the occupied bricks (x lo, x hi, y lo, y hi, z lo, z hi, in
+    // bricks), or null: the LDS march clips each ray's window to them
+    const int32_t *obox;
     // ray tile: wavefronts of (1 << wu_log2) x (64 >> wu_log2) pixels,
     // (1 << bu_log2) x (4 >> bu_log2) wavefronts per workgroup
     int32_t wu_log2, bu_log2;
@@ -458,17 +507,19 @@ __device__ __forceinline__ void load_vox(__amdgpu_buffer_rsrc_t r, uint32_t voff
     }
 }
 
-// Support interval of index s + k*d inside (-1, n): intersect into [lo, hi].
-__device__ __forceinline__ void clip_axis(double s, double d, double n, double &lo, double &hi)
+// Support interval of index s + k*d inside (a, b): intersect into [lo, hi].
+// (-1, n) is the support of a volume of n voxels: a sample outside it has no
+// corner inside the volume.
+__device__ __forceinline__ void clip_axis(double s, double d, double a, double b, double &lo, double &hi)
 {
     // no fused multiply-adds beyond the explicit ones: every kernel that
     // inlines this evaluates the same operations
 #pragma clang fp contract(off)
     if (fabs(d) < 1e-300) {
-        if (!(s > -1.0 && s < n)) { lo = 1.0; hi = 0.0; }
+        if (!(s > a && s < b)) { lo = 1.0; hi = 0.0; }
         return;
     }
-    double k1 = (-1.0 - s) / d, k2 = (n - s) / d;
+    double k1 = (a - s) / d, k2 = (b - s) / d;
     lo = fmax(lo, fmin(k1, k2));
     hi = fmin(hi, fmax(k1, k2));
 }
@@ -480,10 +531,11 @@ __device__ __forceinline__ void clip_axis(double s, double d, double n, double &
 // reaches an anchor, by whatever steps, gets the same value, and the march
 // needs no FP64 after the set-up.
 struct RaySetup {
-    int64_t p0[3]; // position of sample k_lo
+    int64_t p0[3]; // position of sample k_ref
     int64_t d[3];  // step per sample
     float df[3];   // step per sample in FP32 (samples after an anchor)
-    int k_lo, n;
+    int k_lo, n;   // the samples marched, [k_lo, k_lo + n)
+    int k_ref;     // the start of the volume's support window (k_ref <= k_lo)
 };
 
 constexpr double kFix = 4294967296.0; // 2^32
@@ -569,6 +621,12 @@ __device__ __forceinline__ void project_point(const MarchArgs &P, int vw, double
     v = (P.half_v - P.dsize * 0.5 - detV) / P.dsize;
 }
 
+// kBox: also clip the window to P.obox, the occupied bricks, when set.  The
+// positions stay anchored at the volume window's start, so a clipped ray
+// samples the same positions and only leaves out samples whose corners are
+// all zero: the line integrals keep every bit.  (Only the forward quantities
+// may be clipped; the scatter adjoint needs every sample in the volume.)
+template <bool kBox = false>
 __device__ __forceinline__ RaySetup setup_ray(const MarchArgs &P, int u, int v, int vw)
 {
 #pragma clang fp contract(off)
@@ -576,9 +634,9 @@ __device__ __forceinline__ RaySetup setup_ray(const MarchArgs &P, int u, int v, 
     double sa = Ln.sa + Ln.da * P.t0, sb2 = Ln.sb + Ln.db * P.t0, sc = Ln.sc + Ln.dc * P.t0; // k = 0
     double da = Ln.da * P.dx, db = Ln.db * P.dx, dc = Ln.dc * P.dx;
     double lo = 0.0, hi = (double)P.K - 1.0;
-    clip_axis(sa, da, (double)P.nx, lo, hi);
-    clip_axis(sb2, db, (double)P.ny, lo, hi);
-    clip_axis(sc, dc, (double)P.nz, lo, hi);
+    clip_axis(sa, da, -1.0, (double)P.nx, lo, hi);
+    clip_axis(sb2, db, -1.0, (double)P.ny, lo, hi);
+    clip_axis(sc, dc, -1.0, (double)P.nz, lo, hi);
     RaySetup R;
     int k_lo = 0, k_hi = 0;
     if (hi >= lo) {
@@ -586,11 +644,33 @@ __device__ __forceinline__ RaySetup setup_ray(const MarchArgs &P, int u, int v, 
         k_lo = max(0, (int)floor(lo) - 1);
         k_hi = min(P.K, (int)ceil(hi) + 2);
     }
+    const int k_ref = k_lo;
+    if (kBox && P.obox && k_hi > k_lo) {
+        // Voxels outside [8 lo - 2, 8 hi + 5] on an axis are zero: a brick
+        // covers padded cells 8 b .. 8 b + 7, padded cell c holds voxel c - 2
+        // (and, in 16-bit storage, c - 1 as the pair's second half), and
+        // every cell outside the occupied bricks is zero.
+        const int32_t *ob = P.obox;
+        if (ob[0] > ob[1]) {
+            k_hi = k_lo; // nothing occupied
+        } else {
+            clip_axis(sa, da, 8.0 * ob[0] - 3.0, 8.0 * ob[1] + 6.0, lo, hi);
+            clip_axis(sb2, db, 8.0 * ob[2] - 3.0, 8.0 * ob[3] + 6.0, lo, hi);
+            clip_axis(sc, dc, 8.0 * ob[4] - 3.0, 8.0 * ob[5] + 6.0, lo, hi);
+            if (hi >= lo) {
+                k_lo = max(k_lo, (int)floor(lo) - 1);
+                k_hi = min(k_hi, (int)ceil(hi) + 2);
+            } else {
+                k_hi = k_lo;
+            }
+        }
+    }
     R.k_lo = k_lo;
     R.n = max(0, k_hi - k_lo);
-    R.p0[0] = to_fix(sa + da * k_lo);
-    R.p0[1] = to_fix(sb2 + db * k_lo);
-    R.p0[2] = to_fix(sc + dc * k_lo);
+    R.k_ref = k_ref;
+    R.p0[0] = to_fix(sa + da * k_ref);
+    R.p0[1] = to_fix(sb2 + db * k_ref);
+    R.p0[2] = to_fix(sc + dc * k_ref);
     R.d[0] = to_fix(da);
     R.d[1] = to_fix(db);
     R.d[2] = to_fix(dc);
@@ -612,6 +692,7 @@ __device__ __forceinline__ RaySetup empty_ray()
     }
     R.k_lo = 0;
     R.n = 0;
+    R.k_ref = 0;
     return R;
 }
 
@@ -711,7 +792,7 @@ struct FixPos {
 // Position of global sample k: one 64-bit product, for loop starts.
 __device__ __forceinline__ FixPos fix_at(const RaySetup &R, int k)
 {
-    const int64_t j = (int64_t)(k - R.k_lo);
+    const int64_t j = (int64_t)(k - R.k_ref);
     FixPos q;
 #pragma unroll
     for (int a = 0; a < 3; a++) q.p[a] = R.p0[a] + j * R.d[a];
@@ -1571,9 +1652,11 @@ __device__ __forceinline__ void march_chunks(const MarchArgs &P, const RaySetup 
     }
 }
 
+// kBox: clip to the occupied bricks (the LDS march and what mirrors it).
+template <bool kBox = false>
 __device__ __forceinline__ RaySetup lane_ray(const MarchArgs &P, const TileLane &t)
 {
-    return t.active ? setup_ray(P, t.u, t.v, (int)(P.view_begin + t.vw_local)) : empty_ray();
+    return t.active ? setup_ray<kBox>(P, t.u, t.v, (int)(P.view_begin + t.vw_local)) : empty_ray();
 }
 
 // One ray tile (logical block `logical` of P) through the LDS-staged march.
@@ -1584,7 +1667,7 @@ template <int M, int ST, bool kSpectrum, bool kWide = true>
 __device__ __forceinline__ void march_tile(const MarchArgs &P, uint32_t logical, char *lds_raw)
 {
     const TileLane t = tile_lane(P, logical);
-    const RaySetup R = lane_ray(P, t);
+    const RaySetup R = lane_ray<true>(P, t);
     __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(P.vol), 0, (int)P.vol_bytes, 0x00020000);
     float acc[M];
@@ -1637,7 +1720,7 @@ __global__ __launch_bounds__(kBlock) void count_marched_kernel(MarchArgs P, unsi
     const uint32_t logical = xcd_logical(P);
     if (logical >= P.n_blocks) return;
     const TileLane t = tile_lane(P, logical);
-    const RaySetup R = lane_ray(P, t);
+    const RaySetup R = lane_ray<true>(P, t);
     __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(P.vol), 0, (int)P.vol_bytes, 0x00020000);
     float acc[1] = {0.0f};
@@ -1900,7 +1983,7 @@ __global__ __launch_bounds__(kBlock) void ray_grad_kernel(MarchArgs P, AdjointAr
     const uint32_t logical = xcd_logical(P);
     if (logical >= P.n_blocks) return;
     const TileLane t = tile_lane(P, logical);
-    const RaySetup R = lane_ray(P, t);
+    const RaySetup R = lane_ray<true>(P, t);
     float g[M];
     ray_gradient<M, ST>(P, G, t, R, lds_raw, g);
     if (t.active) {
@@ -2351,9 +2434,12 @@ int alloc_volume(ctfp_context *ctx, int n_mat)
     ctx->vol_bytes = bytes;
     const int64_t obx = (px + kOccBrick - 1) / kOccBrick, oby = (py + kOccBrick - 1) / kOccBrick;
     const int64_t obz = (pz + kOccBrick - 1) / kOccBrick;
-    if (ctx->occ && (ctx->ob_x != obx || ctx->ob_y != oby || ctx->ob_z != obz || ctx->n_groups != n_groups))
+    if (ctx->occ && (ctx->ob_x != obx || ctx->ob_y != oby || ctx->ob_z != obz || ctx->n_groups != n_groups)) {
         dfree(ctx->occ);
+        dfree(ctx->obox);
+    }
     if (!ctx->occ) HIP_TRY(hipMalloc(&ctx->occ, (size_t)(obx * oby * obz) * n_groups));
+    if (!ctx->obox) HIP_TRY(hipMalloc(&ctx->obox, (size_t)n_groups * 6 * sizeof(int32_t)));
     ctx->ob_x = obx;
     ctx->ob_y = oby;
     ctx->ob_z = obz;
@@ -2416,6 +2502,7 @@ int fill_march_args(ctfp_context *ctx, int64_t view_begin, int64_t view_count, M
     P.l_mats = ctx->n_mat;
     P.l_moff = ctx->group_moff[group];
     P.occ = ctx->tune.skip_empty && ctx->occ ? ctx->occ + (size_t)group * ctx->ob_x * ctx->ob_y * ctx->ob_z : nullptr;
+    P.obox = P.occ ? ctx->obox + 6 * group : nullptr;
     P.ob_x = (int32_t)ctx->ob_x;
     P.ob_y = (int32_t)ctx->ob_y;
     P.pcx = (int32_t)ctx->px;
@@ -2512,6 +2599,7 @@ int ctfp_destroy(ctfp_context *ctx)
     // caller must not free a context that other threads still launch on
     dfree(ctx->vol);
     dfree(ctx->occ);
+    dfree(ctx->obox);
     dfree(ctx->d_flag);
     dfree(ctx->l_ws);
     dfree(ctx->adj_ws);
@@ -2568,6 +2656,13 @@ static int write_layout(ctfp_context *ctx, const float *const *chan, int n_mat, 
         const int64_t obn = ctx->ob_x * ctx->ob_y * ctx->ob_z;
         occupancy_kernel<<<(unsigned)obn, 256, 0, stream>>>((const uint32_t *)dst, mg, ctx->px, ctx->py, ctx->pz,
                                                             ctx->ob_x, ctx->ob_y, ctx->occ + g * obn);
+        HIP_TRY(hipGetLastError());
+        if (g == 0) {
+            occupied_box_init<<<1, 64, 0, stream>>>(ctx->obox, 6 * ctx->n_groups);
+            HIP_TRY(hipGetLastError());
+        }
+        occupied_box_kernel<<<(unsigned)std::min<int64_t>(1024, (obn + 255) / 256), 256, 0, stream>>>(
+            ctx->occ + g * obn, ctx->ob_x, ctx->ob_y, obn, ctx->obox + 6 * g);
         HIP_TRY(hipGetLastError());
     }
     ctx->dtype = dtype;
@@ -2693,6 +2788,15 @@ int ctfp_get_device(const ctfp_context *ctx, int *device)
 {
     if (check_ctx(ctx) || !device) return set_error(CTFP_ERR_INVALID, "null argument");
     *device = ctx->device;
+    return CTFP_OK;
+}
+
+int ctfp_get_volumes(const ctfp_context *ctx, int *n_mat, int *storage_dtype)
+{
+    if (check_ctx(ctx) || !n_mat || !storage_dtype) return set_error(CTFP_ERR_INVALID, "null argument");
+    if (!ctx->vol) return set_error(CTFP_ERR_STATE, "no volumes uploaded");
+    *n_mat = ctx->n_mat;
+    *storage_dtype = ctx->dtype;
     return CTFP_OK;
 }
 

@@ -1,8 +1,10 @@
 // Differentiable polyenergetic projection for PyTorch: a torch::autograd::Function
 // over the C ABI of libctfp.  Forward lays the density tensor out on the
-// current HIP stream and projects it (ctfp_forward); backward lays the saved
-// densities out again and runs the hand-written adjoint (ctfp_backward), which
-// recomputes every ray's line integrals instead of keeping per-sample state.
+// current HIP stream and projects it (ctfp_forward), keeping every ray's line
+// integrals L (n_mat floats per ray) unless asked not to; backward runs the
+// hand-written adjoint (ctfp_backward) from those L, so it neither marches
+// the volume again nor needs the densities laid out.  Without kept L,
+// backward lays the saved densities out again and the adjoint re-marches.
 //
 // The reference has no gradient; forward is its per-view projection call
 // (CBCTSinMatPolyForwardProjGrid::computePolyForwProj,
@@ -80,7 +82,8 @@ const float *grad_ptr(torch::Tensor &g, int64_t vc, const ctfp_scan_params &s)
 
 struct Project : public torch::autograd::Function<Project> {
     static variable_list forward(AutogradContext *actx, torch::Tensor rho, int64_t handle,
-                                 int64_t view_begin, int64_t view_count, int64_t storage_dtype)
+                                 int64_t view_begin, int64_t view_count, int64_t storage_dtype,
+                                 bool keep_line_integrals)
     {
         ctfp_context *c = as_context(handle);
         const Scan sc = scan_of(c);
@@ -93,11 +96,15 @@ struct Project : public torch::autograd::Function<Project> {
         load(c, rho, storage_dtype, stream);
         auto I = torch::empty({view_count, sc.s.d_num_v, sc.s.d_num_u}, rho.options());
         auto IA = torch::empty_like(I);
+        // [view][m][v][u], as ctfp_backward reads them
+        torch::Tensor L;
+        if (keep_line_integrals)
+            L = torch::empty({view_count, rho.size(0), sc.s.d_num_v, sc.s.d_num_u}, rho.options());
         check(ctfp_forward(c, view_begin, view_count, I.data_ptr<float>(), IA.data_ptr<float>(),
-                           nullptr, stream),
+                           L.defined() ? L.data_ptr<float>() : nullptr, stream),
               "ctfp_forward");
         actx->set_materialize_grads(false); // an unused output passes no gradient
-        actx->save_for_backward({rho});
+        actx->save_for_backward({rho, L});
         actx->saved_data["handle"] = handle;
         actx->saved_data["view_begin"] = view_begin;
         actx->saved_data["view_count"] = view_count;
@@ -107,7 +114,8 @@ struct Project : public torch::autograd::Function<Project> {
 
     static variable_list backward(AutogradContext *actx, variable_list grads)
     {
-        const torch::Tensor rho = actx->get_saved_variables()[0];
+        const variable_list saved = actx->get_saved_variables();
+        const torch::Tensor rho = saved[0], L = saved[1];
         const int64_t handle = actx->saved_data["handle"].toInt();
         const int64_t vb = actx->saved_data["view_begin"].toInt();
         const int64_t vc = actx->saved_data["view_count"].toInt();
@@ -119,19 +127,26 @@ struct Project : public torch::autograd::Function<Project> {
         auto grad = torch::zeros_like(rho);
         if (pI || pIA) {
             void *stream = c10::hip::getCurrentHIPStream(rho.get_device()).stream();
-            // the context may hold other densities since forward ran
-            load(c, rho, dtype, stream);
-            check(ctfp_backward(c, vb, vc, pI, pIA, nullptr, grad.data_ptr<float>(), stream),
+            // The context may hold other densities since forward ran.  With
+            // kept L only the material count (the layout of L and of the
+            // adjoint's channel groups) has to match; without, the adjoint
+            // re-marches and needs these densities.
+            int held_mat = 0, held_dtype = 0;
+            if (!L.defined() || ctfp_get_volumes(c, &held_mat, &held_dtype) != CTFP_OK ||
+                held_mat != rho.size(0))
+                load(c, rho, dtype, stream);
+            check(ctfp_backward(c, vb, vc, pI, pIA, L.defined() ? L.data_ptr<float>() : nullptr,
+                                grad.data_ptr<float>(), stream),
                   "ctfp_backward");
         }
-        return {grad, torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor()};
+        return {grad, torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor()};
     }
 };
 
 std::vector<torch::Tensor> project(torch::Tensor rho, int64_t handle, int64_t view_begin,
-                                   int64_t view_count, int64_t storage_dtype)
+                                   int64_t view_count, int64_t storage_dtype, bool keep_line_integrals)
 {
-    variable_list out = Project::apply(rho, handle, view_begin, view_count, storage_dtype);
+    variable_list out = Project::apply(rho, handle, view_begin, view_count, storage_dtype, keep_line_integrals);
     return {out[0], out[1]};
 }
 
@@ -140,7 +155,9 @@ std::vector<torch::Tensor> project(torch::Tensor rho, int64_t handle, int64_t vi
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
 {
     m.doc() = "Differentiable polyenergetic cone-beam projection (libctfp adjoint)";
-    m.def("project", &project, "I, IAbsorb = project(rho, context, view_begin, view_count, storage_dtype)",
+    m.def("project", &project,
+          "I, IAbsorb = project(rho, context, view_begin, view_count, storage_dtype, keep_line_integrals)",
           pybind11::arg("rho"), pybind11::arg("context"), pybind11::arg("view_begin") = 0,
-          pybind11::arg("view_count") = -1, pybind11::arg("storage_dtype") = CTFP_F32);
+          pybind11::arg("view_count") = -1, pybind11::arg("storage_dtype") = CTFP_F32,
+          pybind11::arg("keep_line_integrals") = true);
 }

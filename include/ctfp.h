@@ -187,6 +187,11 @@ int ctfp_set_volumes_async(ctfp_context *ctx, int n_mat, const float *const *vol
 /* The scan parameters and device of a context (accessors for bindings). */
 int ctfp_get_scan(const ctfp_context *ctx, ctfp_scan_params *out);
 int ctfp_get_device(const ctfp_context *ctx, int *device);
+/* The material count and storage dtype of the volumes the context holds
+ * (CTFP_ERR_STATE before the first upload).  An accessor for bindings that
+ * keep a scan's line integrals for ctfp_backward and so need not upload the
+ * same densities again. */
+int ctfp_get_volumes(const ctfp_context *ctx, int *n_mat, int *storage_dtype);
 /* Device memory the context holds (volume layout, spectral tables, staging). */
 int ctfp_device_bytes(const ctfp_context *ctx, int64_t *bytes);
 

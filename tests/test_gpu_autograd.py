@@ -1,5 +1,6 @@
 """The torch::autograd::Function over the adjoint (`_ctfp_torch`): forward
-identical to the projector's, gradients identical to ctfp_backward, a
+identical to the projector's, gradients identical to ctfp_backward (from the
+kept line integrals and from a re-march, bit for bit equal), a
 weighted least-squares loss on log projections checked by central
 differences along random directions (every voxel at once), and a backward
 that runs after the context has been handed other densities."""
@@ -53,6 +54,16 @@ def test_autograd_forward_and_gradient_match_c_abi(require_gpu):
         proj.set_volumes(list(rho_np))
         g3 = proj.backward(c[1:3].contiguous(), None, view_begin=1, view_count=2)
         assert rel_err(rho.grad.cpu().numpy(), g3.cpu().numpy()) < 1e-4
+        # the backward from the kept line integrals and the one that marches
+        # the volume again read the same L: equal bit for bit
+        grads = []
+        for keep in (True, False):
+            rho.grad = None
+            I, IA = A.project(rho, proj, keep_line_integrals=keep)
+            (I * c + IA * d).sum().backward()
+            grads.append(rho.grad.clone())
+        torch.cuda.synchronize()
+        assert torch.equal(grads[0], grads[1])
 
 
 def test_backward_after_other_densities(require_gpu):
@@ -74,6 +85,14 @@ def test_backward_after_other_densities(require_gpu):
         A.project(other * 1.5, proj)
         I.sum().backward()
         assert rel_err(rho.grad.cpu().numpy(), ref.cpu().numpy()) < 1e-4
+        # volumes of another material count uploaded in between: backward
+        # lays its own out again
+        for keep in (True, False):
+            rho.grad = None
+            I, _ = A.project(rho, proj, keep_line_integrals=keep)
+            proj.set_volumes(list(other[:2] * 1.5))
+            I.sum().backward()
+            assert rel_err(rho.grad.cpu().numpy(), ref.cpu().numpy()) < 1e-4, keep
 
 
 def test_wls_directional_finite_differences(require_gpu):
