@@ -1079,6 +1079,11 @@ __device__ __forceinline__ void spectrum_mfma(const MarchArgs &P, const TileLane
     float sI[2] = {0.0f, 0.0f}, sA[2] = {0.0f, 0.0f};
     const float16_t zero = {};
     const int n_act = P.n_act;
+    // exp2 argument and the series test straight from the scaled sum d =
+    // 2^s tau: d * (-2^-s log2 e) and |d| < 0.35 * 2^s round and compare
+    // exactly as -tau * log2 e and |tau| < 0.35 (power-of-two scalings)
+    const float arg_scale = -P.aq_inv_scale * 1.44269504088896341f;
+    const float small_d = 0.35f / P.aq_inv_scale;
 #pragma unroll 1
     for (int c = 0; c < 2; c++) {
         // ray c * 32 + (lane % 32): this lane's own ray when c == h
@@ -1108,15 +1113,21 @@ __device__ __forceinline__ void spectrum_mfma(const MarchArgs &P, const TileLane
         }
         const half8_t b = __builtin_bit_cast(half8_t, bw);
         const int pix = own ? pix_own : pix_x;
-        const float *wp = P.W + (pix < 0 ? 0 : pix);
+        // The weight of bin e = 32 rb + 8 bb + 4 h + q for this lane's ray
+        // sits at W + e * stride + pix.  Buffer loads from a descriptor at
+        // row 32 rb take the uniform (8 bb + q) * stride in a scalar offset
+        // and this lane's (pix + 4 h stride) as its vector offset, so no
+        // weight load needs address arithmetic (both offsets stay below
+        // 32 * stride * 4 bytes, which the host keeps under 4 GiB).
+        const uint32_t woff = ((uint32_t)(pix < 0 ? 0 : pix) + (uint32_t)(4 * h) * (uint32_t)P.w_stride_e) * 4u;
+        const uint32_t wrow = (uint32_t)P.w_stride_e * 4u;
         KSum si = ksum_zero(), sa = ksum_zero();
 #pragma unroll 1
         for (int rb = 0; rb * 32 < n_act; rb++) {
             const half8_t a = *(const half8_t *)(P.Aq + ((rb * 32 + (lane & 31)) * 16 + 8 * h));
             const float16_t d = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, zero, 0, 0, 0);
-            // the weight of bin e sits at wp + e * stride: one pointer steps
-            // through the bins (no 64-bit multiply per bin)
-            const float *we = wp + (int64_t)(rb * 32 + 4 * h) * P.w_stride_e;
+            const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<float *>(P.W + (int64_t)rb * 32 * P.w_stride_e), 0, (int)0xFFFFFFFFu, 0x00020000);
 #pragma unroll
             for (int bb = 0; bb < 4; bb++) {
                 if (rb * 32 + 8 * bb >= n_act) break; // uniform: the rest of the block is past the bins
@@ -1126,22 +1137,23 @@ __device__ __forceinline__ void spectrum_mfma(const MarchArgs &P, const TileLane
                 // the last active one read the table's zero rows and add 0
                 float wv[4];
 #pragma unroll
-                for (int q = 0; q < 4; q++) wv[q] = we[q * P.w_stride_e];
-                float tau[4], ex[4], ab[4];
+                for (int q = 0; q < 4; q++)
+                    wv[q] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(wr, woff, (uint32_t)(8 * bb + q) * wrow, 0));
+                float ex[4], ab[4];
                 bool small = false;
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    tau[q] = d[4 * bb + q] * P.aq_inv_scale;
-                    ex[q] = __builtin_amdgcn_exp2f(-tau[q] * 1.44269504088896341f);
+                    ex[q] = __builtin_amdgcn_exp2f(d[4 * bb + q] * arg_scale);
                     ab[q] = 1.0f - ex[q];
-                    small |= fabsf(tau[q]) < 0.35f;
+                    small |= fabsf(d[4 * bb + q]) < small_d;
                 }
                 // the series only when a lane of the wavefront needs it
                 // (a uniform branch; the same values as attenuate)
                 if (__builtin_amdgcn_ballot_w64(small) != 0) {
 #pragma unroll
                     for (int q = 0; q < 4; q++)
-                        if (fabsf(tau[q]) < 0.35f) ab[q] = absorb_series(tau[q]);
+                        if (fabsf(d[4 * bb + q]) < small_d)
+                            ab[q] = absorb_series(d[4 * bb + q] * P.aq_inv_scale);
                 }
                 // the four bins summed in FP32, then one compensated add each
                 // (a few ulps of the total, as for bin-by-bin compensation)
@@ -1153,7 +1165,6 @@ __device__ __forceinline__ void spectrum_mfma(const MarchArgs &P, const TileLane
                 }
                 ksum_add(si, gI);
                 ksum_add(sa, gA);
-                we += 8 * P.w_stride_e;
             }
         }
         if (c == 0) {
@@ -2027,8 +2038,8 @@ struct BrickView {
 constexpr int kGatherViews = 64;
 
 // Staged dwords per shadow pixel, one record per pixel: the support bounds
-// at the brick centre and the inverse step (3 axes each), k* split in two
-// (read for every pixel of a voxel's window), then the closest-approach
+// at the brick centre in j = k - floor(k*) and the inverse step (3 axes
+// each), k* split in two (read for every pixel of a voxel's window), then the closest-approach
 // offset and the step (3 axes each) and dl/dL_m * dx (read for the pixels
 // that touch the voxel).  An odd record length keeps the lanes' reads of
 // different records on different LDS banks.
@@ -2130,22 +2141,26 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
                         const float e[3] = {(float)(qa + ks * dxa), (float)(qb + ks * dxb), (float)(qc + ks * dxc)};
                         const float d[3] = {(float)dxa, (float)dxb, (float)dxc};
                         float *rec = s_px + i * PW;
+                        const float kr = (float)(ks - kf);
 #pragma unroll
                         for (int a = 0; a < 3; a++) {
                             // a zero step leaves that axis unconstrained (or
-                            // empty) through +-huge bounds
-                            const float id = 1.0f / (fabsf(d[a]) > 1e-30f ? d[a] : 1e-30f);
+                            // empty) through +-huge bounds.  1/d to an ulp:
+                            // a bound off by that moves a sample whose hat is
+                            // ~1e-6 across the support's edge
+                            const float id = __builtin_amdgcn_rcpf(fabsf(d[a]) > 1e-30f ? d[a] : 1e-30f);
                             // samples s = k - k* with |e - o + s d| < 1 lie
-                            // between (+-1 - e) / d + o / d: the bounds at o = 0
+                            // between (+-1 - e) / d + o / d: the bounds at o = 0,
+                            // stored as k - kf = s + kr
                             const float l = (-1.0f - e[a]) * id, h = (1.0f - e[a]) * id;
-                            rec[kGLo + a] = fminf(l, h);
-                            rec[kGHi + a] = fmaxf(l, h);
+                            rec[kGLo + a] = fminf(l, h) + kr;
+                            rec[kGHi + a] = fmaxf(l, h) + kr;
                             rec[kGId + a] = id;
                             rec[kGE + a] = e[a];
                             rec[kGD + a] = d[a];
                         }
                         rec[kGKf] = __int_as_float((int)kf);
-                        rec[kGKr] = (float)(ks - kf);
+                        rec[kGKr] = kr;
                         const int64_t pix = (int64_t)pv * P.nu + pu;
 #pragma unroll
                         for (int m = 0; m < M; m++)
@@ -2158,20 +2173,20 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
                     for (int pv = q0; pv <= q1; pv++) {
                         const float *rec = s_px + ((pv - b0) * rw + (w0 - c0)) * PW;
                         for (int pu = w0; pu <= w1; pu++, rec += PW) {
-                            // the voxel's support in s: the centre bounds
-                            // shifted by o / d on each axis
+                            // the voxel's support in j = k - kf: the centre
+                            // bounds shifted by o / d on each axis (empty
+                            // when it misses the voxel: then j0 > j1)
                             const float ia = rec[kGId], ib = rec[kGId + 1], ic = rec[kGId + 2];
-                            const float slo = fmaxf(fmaxf(fmaf(ox, ia, rec[kGLo]), fmaf(oy, ib, rec[kGLo + 1])),
+                            const float jlo = fmaxf(fmaxf(fmaf(ox, ia, rec[kGLo]), fmaf(oy, ib, rec[kGLo + 1])),
                                                     fmaf(oz, ic, rec[kGLo + 2]));
-                            const float shi = fminf(fminf(fmaf(ox, ia, rec[kGHi]), fmaf(oy, ib, rec[kGHi + 1])),
+                            const float jhi = fminf(fminf(fmaf(ox, ia, rec[kGHi]), fmaf(oy, ib, rec[kGHi + 1])),
                                                     fmaf(oz, ic, rec[kGHi + 2]));
-                            if (!(shi > slo)) continue;
                             const int kf = __float_as_int(rec[kGKf]);
-                            const float kr = rec[kGKr];
-                            // j = k - kf over the open interval, and 0 <= k < K
-                            const int j0 = max((int)ceilf(kr + slo), -kf);
-                            const int j1 = min((int)floorf(kr + shi), P.K - 1 - kf);
+                            // j over the open interval, and 0 <= k < K
+                            const int j0 = max((int)ceilf(jlo), -kf);
+                            const int j1 = min((int)floorf(jhi), P.K - 1 - kf);
                             if (j0 > j1) continue;
+                            const float kr = rec[kGKr];
                             const float fa = rec[kGE] - ox, fb = rec[kGE + 1] - oy, fc = rec[kGE + 2] - oz;
                             const float da = rec[kGD], db = rec[kGD + 1], dc = rec[kGD + 2];
                             float wsum = 0.0f, sj = (float)j0 - kr;
@@ -3013,6 +3028,11 @@ static int spectral_args(ctfp_context *ctx, MarchArgs &P)
     if (ctx->spec_n_mat != ctx->n_mat)
         return set_error(CTFP_ERR_STATE, "spectral tables are for %d materials, volumes have %d", ctx->spec_n_mat,
                          ctx->n_mat);
+    // the matrix-core spectrum of 16-bit volumes addresses 32 weight rows
+    // from one buffer descriptor with 32-bit offsets
+    if (ctx->dtype != CTFP_F32 && ctx->w_stride_e * 4 * 32 >= (int64_t)(1ll << 32))
+        return set_error(CTFP_ERR_UNSUPPORTED, "16-bit volumes: a weight row of %lld entries exceeds the %lld the "
+                         "matrix-core spectrum addresses", (long long)ctx->w_stride_e, (long long)((1ll << 32) / 128));
     P.n_act = ctx->n_act;
     P.A = ctx->A;
     P.Aq = ctx->Aq;
