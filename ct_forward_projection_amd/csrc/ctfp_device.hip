@@ -103,7 +103,7 @@ struct ctfp_context {
     float *dI = nullptr, *dIA = nullptr;
     int64_t staged_views = 0;
     // launch tuning
-    ctfp_tuning tune{CTFP_MARCH_LDS, 16, 40 * 1024, 16, 1, CTFP_ADJOINT_GATHER, 1};
+    ctfp_tuning tune{CTFP_MARCH_LDS, 16, 40 * 1024, 8, 2, CTFP_ADJOINT_GATHER, 1};
     // dl/dL workspace of the gather adjoint
     float *adj_ws = nullptr;
     size_t adj_ws_bytes = 0;
@@ -524,6 +524,15 @@ __device__ __forceinline__ void clip_axis(double s, double d, double a, double b
     hi = fmin(hi, fmax(k1, k2));
 }
 
+// Sample positions.  Every kAnchor-th sample of the ray (global sample index
+// k a multiple of kAnchor) is an anchor whose position is exact in 32.32
+// fixed point and split into an integer cell and an FP32 offset; the
+// kAnchor samples from it on are offset + i * step in FP32 (|error| <
+// ~3e-6 * max(1, |step|) voxel, against ~1e-4 for the reference's FP32
+// t * d - SOD).  The anchor grid is fixed, so every launch option sees the
+// same positions.
+constexpr int kAnchor = 16;
+
 // Ray (u, v) of view `vw`: FP64 set-up of the index-space start (at k_lo) and
 // step, and the sample range [k_lo, k_hi).
 // A ray's samples in index space.  Positions are 32.32 fixed point (units of
@@ -664,6 +673,16 @@ __device__ __forceinline__ RaySetup setup_ray(const MarchArgs &P, int u, int v, 
                 k_hi = k_lo;
             }
         }
+        if (k_hi > k_lo) {
+            // Whole anchor segments: the window grows to the anchor grid
+            // (up to kAnchor - 1 samples each side, outside the window
+            // above, so exact zeros; cells past the padded volume stage as
+            // zeros), and every segment of the ray but one at the lattice's
+            // end takes the unrolled path of the march instead of the
+            // sample-by-sample loop.
+            k_lo &= ~(kAnchor - 1);
+            k_hi = min(P.K, (k_hi + kAnchor - 1) & ~(kAnchor - 1));
+        }
     }
     R.k_lo = k_lo;
     R.n = max(0, k_hi - k_lo);
@@ -768,15 +787,6 @@ template <int ST> __device__ __forceinline__ uint32_t x_pair(float w, float ax)
 {
     return pack_w<ST>(fmaf(-w, ax, w), w * ax);
 }
-
-// Sample positions.  Every kAnchor-th sample of the ray (global sample index
-// k a multiple of kAnchor) is an anchor whose position is exact in 32.32
-// fixed point and split into an integer cell and an FP32 offset; the
-// kAnchor samples from it on are offset + i * step in FP32 (|error| <
-// ~3e-6 * max(1, |step|) voxel, against ~1e-4 for the reference's FP32
-// t * d - SOD).  The anchor grid is fixed, so every launch option sees the
-// same positions.
-constexpr int kAnchor = 16;
 
 struct Anchor {
     int ia, ib, ic;       // cell of the anchor
@@ -1566,8 +1576,12 @@ __device__ __forceinline__ bool chunk_empty(const MarchArgs &P, const int (&lo)[
     const int lane = threadIdx.x & 63;
     const int jx = lane & 3, jy = (lane >> 2) & 3, jz = lane >> 4;
     bool any = false;
+    // 24-bit products: a padded volume has fewer than 2^24 bricks per axis
+    // and fewer than 2^32 in all (2^41 cells)
     if (jx < nbx && jy < nby && jz < nbz)
-        any = P.occ[((int64_t)(bz0 + jz) * P.ob_y + (by0 + jy)) * P.ob_x + (bx0 + jx)] != 0;
+        any = P.occ[__umul24(__umul24((uint32_t)(bz0 + jz), (uint32_t)P.ob_y) + (uint32_t)(by0 + jy),
+                             (uint32_t)P.ob_x) +
+                    (uint32_t)(bx0 + jx)] != 0;
     return __builtin_amdgcn_ballot_w64(any) == 0;
 }
 
@@ -2449,6 +2463,10 @@ int alloc_volume(ctfp_context *ctx, int n_mat)
     ctx->vol_bytes = bytes;
     const int64_t obx = (px + kOccBrick - 1) / kOccBrick, oby = (py + kOccBrick - 1) / kOccBrick;
     const int64_t obz = (pz + kOccBrick - 1) / kOccBrick;
+    // the march indexes the occupancy grid with 24-bit products
+    if (obz * oby >= (1ll << 24) || obx >= (1ll << 24))
+        return set_error(CTFP_ERR_UNSUPPORTED, "volume of %lld x %lld padded planes exceeds the occupancy index",
+                         (long long)pz, (long long)py);
     if (ctx->occ && (ctx->ob_x != obx || ctx->ob_y != oby || ctx->ob_z != obz || ctx->n_groups != n_groups)) {
         dfree(ctx->occ);
         dfree(ctx->obox);
@@ -3516,11 +3534,17 @@ int ctfp_autotune(ctfp_context *ctx, int max_lds_bytes, ctfp_tuning *best)
     const ctfp_scan_params &s = ctx->scan;
     const int64_t npix = s.d_num_u * s.d_num_v;
     // time enough views to fill the GPU several times (>= 8192 tiles of
-    // 256 rays), in two groups a quarter turn apart
+    // 256 rays), in groups at 0, 1/8, 1/4 and 3/8 of the scan's turn: rays
+    // along the volume's axes and across its diagonals stage chunk boxes of
+    // very different sizes, so a sample of one kind alone misjudges the
+    // staging step and the LDS budget
+    constexpr int kGroups = 4;
     const int64_t tiles_per_view = (npix + 255) / 256;
     const int64_t want = std::max<int64_t>(1, (8192 + tiles_per_view - 1) / tiles_per_view);
-    const int64_t per_group = std::min<int64_t>(want, std::max<int64_t>(1, s.proj_num / 2));
-    const int64_t starts[2] = {0, std::min<int64_t>(s.proj_num / 4, s.proj_num - per_group)};
+    const int64_t per_group =
+        std::max<int64_t>(1, std::min<int64_t>((want + kGroups - 1) / kGroups, s.proj_num / kGroups));
+    int64_t starts[kGroups];
+    for (int gi = 0; gi < kGroups; gi++) starts[gi] = std::min<int64_t>(gi * s.proj_num / 8, s.proj_num - per_group);
     DeviceGuard g(ctx->device);
     float *I = nullptr, *IA = nullptr;
     HIP_TRY(hipMalloc(&I, (size_t)per_group * npix * sizeof(float)));
@@ -3540,12 +3564,12 @@ int ctfp_autotune(ctfp_context *ctx, int max_lds_bytes, ctfp_tuning *best)
         cleanup();
         return set_error(CTFP_ERR_DEVICE, "autotune set-up: %s", hipGetErrorString(e));
     }
-    // time one candidate (ms over both groups; +inf if it cannot launch)
+    // time one candidate (ms over all groups)
     auto time_it = [&](const ctfp_tuning &t, float &ms) -> int {
         ctx->tune = t;
         for (int rep = 0; rep < 2; rep++) { // rep 0 warms caches and code
             hipError_t r = hipEventRecord(ev0, nullptr);
-            for (int gi = 0; gi < 2 && r == hipSuccess; gi++) {
+            for (int gi = 0; gi < kGroups && r == hipSuccess; gi++) {
                 int rc = launch_march(ctx, starts[gi], per_group, I, IA, nullptr, nullptr, true);
                 if (rc) return rc;
             }

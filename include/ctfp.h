@@ -122,10 +122,10 @@ typedef struct ctfp_grid_params {
 #define CTFP_MARCH_LDS 1    /* per-chunk volume boxes staged in LDS (default) */
 typedef struct ctfp_tuning {
     int32_t march;     /* CTFP_MARCH_GLOBAL or CTFP_MARCH_LDS */
-    int32_t chunk;     /* samples per staged chunk (LDS mode): a multiple of 16 in [16, 4096] */
-    int32_t lds_bytes; /* LDS per workgroup (LDS mode), 1 KiB..64 KiB */
-    int32_t wave_u;    /* ray tile: pixels along u per wavefront, 4/8/16/32 (64 / wave_u along v) */
-    int32_t block_u;   /* ray tile: wavefronts along u per workgroup, 1/2/4 (4 / block_u along v) */
+    int32_t chunk;     /* samples per staged chunk (LDS mode): a multiple of 16 in [16, 4096]; default 16 */
+    int32_t lds_bytes; /* LDS per workgroup (LDS mode), 1 KiB..64 KiB; default 40 KiB */
+    int32_t wave_u;    /* ray tile: pixels along u per wavefront, 4/8/16/32 (64 / wave_u along v); default 8 */
+    int32_t block_u;   /* ray tile: wavefronts along u per workgroup, 1/2/4 (4 / block_u along v); default 2 */
     int32_t adjoint;   /* ctfp_backward: CTFP_ADJOINT_GATHER (default) or CTFP_ADJOINT_SCATTER */
     int32_t skip_empty; /* 1 (default): a staged chunk whose cell box is zero in every channel
                            (checked on an occupancy grid of 8^3-cell bricks built with the
@@ -279,7 +279,8 @@ int ctfp_count_marched(ctfp_context *ctx, int64_t view_begin, int64_t view_count
 
 
 /* Times candidate launch options (ray-tile shape, samples per staged chunk,
- * LDS budget) on two groups of the scan's views a quarter turn apart, keeps
+ * LDS budget) on four groups of the scan's views at 0, 1/8, 1/4 and 3/8 of
+ * its turn (rays along the volume's axes and across its diagonals), keeps
  * the fastest in the context and returns it in `best` (may be NULL).
  * max_lds_bytes > 0 caps the LDS budget (0: no cap).
  * Needs volumes and spectral tables; results are bit-identical for every
