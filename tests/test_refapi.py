@@ -4,6 +4,7 @@ paths, case-insensitive OutPutFolder.ini), configuration errors, the host-only
 TestI0 mode and (on a GPU) full scans written to the reference's file formats,
 checked against the oracle."""
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -133,7 +134,15 @@ def test_testI0_matches_oracle(tmp_path):
         eta = O.det_response(1.0, float(mu[k]), t)
         acc = acc + (np.float32(case["photons"]) * spec[k]) * eta
     np.testing.assert_array_equal(I0.ravel(), acc)
-    assert "I0 = " in r.stdout
+    # addStandardI0 (CBCTForwardProj.cpp:92-95): the same sum for the nominal
+    # thickness without jitter, printed with 6 significant digits
+    m = re.search(r"gives I0 = ([0-9.eE+-]+) photons", r.stdout)
+    assert m, r.stdout
+    t0 = np.full(1, 0.4, np.float32)
+    std = np.float32(0)
+    for k in range(case["kvp"]):
+        std = std + (np.float32(case["photons"]) * spec[k]) * O.det_response(1.0, float(mu[k]), t0)[0]
+    assert float(m.group(1)) == pytest.approx(float(std), rel=1e-5)
 
 
 @pytest.mark.parametrize("edit,msg", [
