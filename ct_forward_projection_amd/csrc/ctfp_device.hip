@@ -2040,15 +2040,21 @@ __global__ __launch_bounds__(kBlock) void ray_grad_kernel(MarchArgs P, AdjointAr
     }
 }
 
-// Voxel pass of the gather adjoint.  Each thread owns one voxel of an
-// 8 x 8 x 4 brick and sums, over the views, every sample of every ray whose
-// trilinear support (the open cube of side 2 around the voxel) holds it:
+// Voxel pass of the gather adjoint.  Each thread owns a 2 x 2 quad of voxels
+// (in x and y) of a 16 x 16 x 4 brick and sums, over the views, every sample
+// of every ray whose trilinear support (the open cube of side 2 around a
+// voxel) holds it:
 //   grad_m(c) += sum_rays gL_m(ray) sum_k prod_axes max(0, 1 - |p_k - c|).
 // Per view the block stages, for each detector pixel of the brick's shadow,
 // the ray's closest approach to the brick centre (FP64-derived offset e and
 // sample index k*), its per-sample step and the ray's gL, so that the voxel
 // loop runs in FP32 on small numbers (positions good to ~1e-6 voxel).  The
-// voxel writes its own gradient: no atomics, and the result is deterministic.
+// quad is walked as two pairs of neighbours along x or along y, whichever
+// moves the brick's shadow less in that view (the axis closer to the rays):
+// the two voxels of a pair see nearly the same pixels, so they share the
+// pixel's bounds, the hats of the other two axes and the sample loop.  Each
+// voxel writes its own gradient: no atomics, and the result is
+// deterministic.
 struct GatherArgs {
     MarchArgs P;       // geometry of the scan (views [view_begin, +n_views))
     const float *gL;   // [n_views][gl_mats][nv][nu]
@@ -2060,14 +2066,15 @@ struct GatherArgs {
     int32_t cap;       // staged pixels that fit the LDS budget
 };
 
-constexpr int kBrickX = 8, kBrickY = 8, kBrickZ = 4;
+constexpr int kBrickX = 16, kBrickY = 16, kBrickZ = 4;
 
 // A brick's shadow in one view: the projection of its centre, the Jacobian
 // (pixels per voxel step), the half-extent of one voxel's support in pixels,
-// and the shadow's pixel rectangle.
+// the shadow's pixel rectangle, and the axis of the view's voxel pairs.
 struct BrickView {
     float u0, v0, jua, jub, juc, jva, jvb, jvc, hu, hv;
     int ru0, ru1, rv0, rv1;
+    int pair_y; // 1: pairs along y (a y step moves the shadow less than an x step)
 };
 // Views whose brick shadows a workgroup computes at once (one lane each).
 constexpr int kGatherViews = 64;
@@ -2080,6 +2087,58 @@ constexpr int kGatherViews = 64;
 // different records on different LDS banks.
 __host__ __device__ constexpr int gather_pixel_words(int m) { return (17 + m) | 1; }
 constexpr int kGLo = 0, kGHi = 3, kGId = 6, kGKf = 9, kGKr = 10, kGE = 11, kGD = 14, kGG = 17;
+
+// hat(r) = max(0, 1 - |r|), one clamped subtraction
+__device__ __forceinline__ float hat(float r) { return __builtin_amdgcn_fmed3f(1.0f - fabsf(r), 0.0f, 1.0f); }
+
+// One pair of neighbouring voxels along axis AX (0: x, 1: y) at offsets oP
+// and oP + 1 from the brick centre on that axis, oQ on the other in-plane
+// axis and oz on z: for each staged pixel of rows [q0, q1] x columns
+// [w0, w1] (records from `rec0`, row stride rs records), the samples inside
+// either voxel's support, weighted into wA (the voxel at oP) and wB (at
+// oP + 1) and then times the pixel's dl/dL into accA / accB.  A sample
+// outside one voxel's support has a zero hat there, so the union of the two
+// intervals is walked once.
+template <int M, int AX>
+__device__ __forceinline__ void pair_pixels(const float *rec0, int rs, int q0, int q1, int w0, int w1, float oP,
+                                            float oQ, float oz, int K, float (&accA)[M], float (&accB)[M])
+{
+    constexpr int PW = gather_pixel_words(M);
+    constexpr int IP = AX, IQ = 1 - AX;
+    for (int pv = q0; pv <= q1; pv++) {
+        const float *rec = rec0 + (pv - q0) * rs * PW;
+        for (int pu = w0; pu <= w1; pu++, rec += PW) {
+            const float iP = rec[kGId + IP], iQ = rec[kGId + IQ], iZ = rec[kGId + 2];
+            // support in j = k - kf: the centre bounds shifted by o / d on each
+            // axis; on the pair's axis the union of the shifts by oP and oP + 1
+            const float pl = fmaf(oP, iP, rec[kGLo + IP]), ph = fmaf(oP, iP, rec[kGHi + IP]);
+            const float jlo = fmaxf(fmaxf(fmaf(oQ, iQ, rec[kGLo + IQ]), fmaf(oz, iZ, rec[kGLo + 2])),
+                                    fminf(pl, pl + iP));
+            const float jhi = fminf(fminf(fmaf(oQ, iQ, rec[kGHi + IQ]), fmaf(oz, iZ, rec[kGHi + 2])),
+                                    fmaxf(ph, ph + iP));
+            const int kf = __float_as_int(rec[kGKf]);
+            // j over the open interval, and 0 <= k < K
+            const int j0 = max((int)ceilf(jlo), -kf);
+            const int j1 = min((int)floorf(jhi), K - 1 - kf);
+            if (j0 > j1) continue;
+            const float kr = rec[kGKr];
+            const float fP = rec[kGE + IP] - oP, fQ = rec[kGE + IQ] - oQ, fZ = rec[kGE + 2] - oz;
+            const float dP = rec[kGD + IP], dQ = rec[kGD + IQ], dZ = rec[kGD + 2];
+            float wA = 0.0f, wB = 0.0f, sj = (float)j0 - kr;
+            for (int j = j0; j <= j1; j++, sj += 1.0f) {
+                const float rqz = hat(fmaf(sj, dQ, fQ)) * hat(fmaf(sj, dZ, fZ));
+                const float t = fmaf(sj, dP, fP);
+                wA = fmaf(rqz, hat(t), wA);
+                wB = fmaf(rqz, hat(t - 1.0f), wB);
+            }
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                accA[m] = fmaf(wA, rec[kGG + m], accA[m]);
+                accB[m] = fmaf(wB, rec[kGG + m], accB[m]);
+            }
+        }
+    }
+}
 
 template <int M>
 __global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
@@ -2095,19 +2154,24 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
     const int bx = (int)(brick % (uint32_t)A.bricks_x);
     const int by = (int)((brick / (uint32_t)A.bricks_x) % (uint32_t)A.bricks_y);
     const int bz = (int)(brick / ((uint32_t)A.bricks_x * (uint32_t)A.bricks_y));
-    const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
-    const int x = bx * kBrickX + lx, y = by * kBrickY + ly, z = bz * kBrickZ + lz;
+    // the thread's quad: voxels (x0 + {0, 1}, y0 + {0, 1}, z)
+    const int lx = 2 * (threadIdx.x & 7), ly = 2 * ((threadIdx.x >> 3) & 7), lz = threadIdx.x >> 6;
+    const int x0 = bx * kBrickX + lx, y0 = by * kBrickY + ly, z = bz * kBrickZ + lz;
     const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
-    const bool inside = x < nx && y < ny && z < nz;
-    // brick centre (index space) and the voxel's offset from it
+    const bool inside = x0 < nx && y0 < ny && z < nz; // the quad's first voxel
+    // brick centre (index space) and the quad's first voxel's offset from it
     const double ca = bx * kBrickX + 0.5 * (kBrickX - 1), cb = by * kBrickY + 0.5 * (kBrickY - 1);
     const double cc = bz * kBrickZ + 0.5 * (kBrickZ - 1);
     const float ox = (float)(lx - 0.5 * (kBrickX - 1)), oy = (float)(ly - 0.5 * (kBrickY - 1));
     const float oz = (float)(lz - 0.5 * (kBrickZ - 1));
     const int64_t npix = (int64_t)P.nu * P.nv;
-    float acc[M];
+    float acc[2][2][M]; // [y - y0][x - x0][m]
 #pragma unroll
-    for (int m = 0; m < M; m++) acc[m] = 0.0f;
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int m = 0; m < M; m++) acc[i][j][m] = 0.0f;
     for (int vt = 0; vt < P.n_views; vt += kGatherViews) {
         const int nvt = min(kGatherViews, P.n_views - vt);
         __syncthreads(); // the previous views' table is consumed
@@ -2141,6 +2205,7 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
             bv.ru1 = min(P.nu - 1, (int)floor(u0 + eu));
             bv.rv0 = max(0, (int)ceil(v0 - ev));
             bv.rv1 = min(P.nv - 1, (int)floor(v0 + ev));
+            bv.pair_y = fabsf(bv.jub) < fabsf(bv.jua) ? 1 : 0;
             s_view[threadIdx.x] = bv;
         }
         __syncthreads();
@@ -2148,11 +2213,24 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
             const BrickView bv = s_view[vw - vt];
             const int ru0 = bv.ru0, ru1 = bv.ru1, rv0 = bv.rv0, rv1 = bv.rv1;
             if (ru0 > ru1 || rv0 > rv1) continue; // block-uniform
-            // the voxel's own pixel window (inside the brick's)
-            const float cu = bv.u0 + bv.jua * ox + bv.jub * oy + bv.juc * oz;
-            const float cv = bv.v0 + bv.jva * ox + bv.jvb * oy + bv.jvc * oz;
-            const int pu0 = max(ru0, (int)ceilf(cu - bv.hu)), pu1 = min(ru1, (int)floorf(cu + bv.hu));
-            const int pv0 = max(rv0, (int)ceilf(cv - bv.hv)), pv1 = min(rv1, (int)floorf(cv + bv.hv));
+            // the two pairs of the quad: along x (pair p at y0 + p) or along y
+            // (pair p at x0 + p); each pair's pixel window is its two voxels'
+            // windows joined (inside the brick's)
+            const bool py = bv.pair_y != 0; // block-uniform
+            const float jP_u = py ? bv.jub : bv.jua, jP_v = py ? bv.jvb : bv.jva;
+            const float jQ_u = py ? bv.jua : bv.jub, jQ_v = py ? bv.jva : bv.jvb;
+            const float oP = py ? oy : ox, oQ0 = py ? ox : oy;
+            const float hu = bv.hu + 0.5f * fabsf(jP_u), hv = bv.hv + 0.5f * fabsf(jP_v);
+            int pu0[2], pu1[2], pv0[2], pv1[2];
+#pragma unroll
+            for (int pp = 0; pp < 2; pp++) {
+                const float cu = bv.u0 + jP_u * (oP + 0.5f) + jQ_u * (oQ0 + (float)pp) + bv.juc * oz;
+                const float cv = bv.v0 + jP_v * (oP + 0.5f) + jQ_v * (oQ0 + (float)pp) + bv.jvc * oz;
+                pu0[pp] = max(ru0, (int)ceilf(cu - hu));
+                pu1[pp] = min(ru1, (int)floorf(cu + hu));
+                pv0[pp] = max(rv0, (int)ceilf(cv - hv));
+                pv1[pp] = min(rv1, (int)floorf(cv + hv));
+            }
             // the brick's shadow in pieces of at most `cap` pixels: column
             // segments no wider than cap, each in bands of whole rows
             for (int c0 = ru0; c0 <= ru1; c0 += cap) {
@@ -2203,49 +2281,34 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
                     }
                     __syncthreads();
                     if (!inside) continue;
-                    const int q0 = max(pv0, b0), q1 = min(pv1, b1);
-                    const int w0 = max(pu0, c0), w1 = min(pu1, c1);
-                    for (int pv = q0; pv <= q1; pv++) {
-                        const float *rec = s_px + ((pv - b0) * rw + (w0 - c0)) * PW;
-                        for (int pu = w0; pu <= w1; pu++, rec += PW) {
-                            // the voxel's support in j = k - kf: the centre
-                            // bounds shifted by o / d on each axis (empty
-                            // when it misses the voxel: then j0 > j1)
-                            const float ia = rec[kGId], ib = rec[kGId + 1], ic = rec[kGId + 2];
-                            const float jlo = fmaxf(fmaxf(fmaf(ox, ia, rec[kGLo]), fmaf(oy, ib, rec[kGLo + 1])),
-                                                    fmaf(oz, ic, rec[kGLo + 2]));
-                            const float jhi = fminf(fminf(fmaf(ox, ia, rec[kGHi]), fmaf(oy, ib, rec[kGHi + 1])),
-                                                    fmaf(oz, ic, rec[kGHi + 2]));
-                            const int kf = __float_as_int(rec[kGKf]);
-                            // j over the open interval, and 0 <= k < K
-                            const int j0 = max((int)ceilf(jlo), -kf);
-                            const int j1 = min((int)floorf(jhi), P.K - 1 - kf);
-                            if (j0 > j1) continue;
-                            const float kr = rec[kGKr];
-                            const float fa = rec[kGE] - ox, fb = rec[kGE + 1] - oy, fc = rec[kGE + 2] - oz;
-                            const float da = rec[kGD], db = rec[kGD + 1], dc = rec[kGD + 2];
-                            float wsum = 0.0f, sj = (float)j0 - kr;
-                            for (int j = j0; j <= j1; j++, sj += 1.0f) {
-                                // hat(r) = max(0, 1 - |r|), as one clamp
-                                const float ra = __builtin_amdgcn_fmed3f(1.0f - fabsf(fmaf(sj, da, fa)), 0.0f, 1.0f);
-                                const float rb = __builtin_amdgcn_fmed3f(1.0f - fabsf(fmaf(sj, db, fb)), 0.0f, 1.0f);
-                                const float rc = __builtin_amdgcn_fmed3f(1.0f - fabsf(fmaf(sj, dc, fc)), 0.0f, 1.0f);
-                                wsum = fmaf(ra * rb, rc, wsum);
-                            }
 #pragma unroll
-                            for (int m = 0; m < M; m++) acc[m] = fmaf(wsum, rec[kGG + m], acc[m]);
-                        }
+                    for (int pp = 0; pp < 2; pp++) {
+                        const int q0 = max(pv0[pp], b0), q1 = min(pv1[pp], b1);
+                        const int w0 = max(pu0[pp], c0), w1 = min(pu1[pp], c1);
+                        if (q0 > q1 || w0 > w1) continue;
+                        const float *rec0 = s_px + ((q0 - b0) * rw + (w0 - c0)) * PW;
+                        if (py)
+                            pair_pixels<M, 1>(rec0, rw, q0, q1, w0, w1, oy, ox + (float)pp, oz, P.K, acc[0][pp],
+                                              acc[1][pp]);
+                        else
+                            pair_pixels<M, 0>(rec0, rw, q0, q1, w0, w1, ox, oy + (float)pp, oz, P.K, acc[pp][0],
+                                              acc[pp][1]);
                     }
                 }
             }
         }
     }
-    if (inside) {
-        const int64_t nvox = (int64_t)nx * ny * nz;
-        const int64_t q = ((int64_t)z * ny + y) * nx + x;
+    const int64_t nvox = (int64_t)nx * ny * nz;
 #pragma unroll
-        for (int m = 0; m < M; m++) A.grad[(int64_t)(A.mat_off + m) * nvox + q] += acc[m];
-    }
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int x = x0 + j, y = y0 + i;
+            if (x >= nx || y >= ny || z >= nz) continue;
+            const int64_t q = ((int64_t)z * ny + y) * nx + x;
+#pragma unroll
+            for (int m = 0; m < M; m++) A.grad[(int64_t)(A.mat_off + m) * nvox + q] += acc[i][j][m];
+        }
 }
 
 template <int M, int ST>
