@@ -264,25 +264,30 @@ __global__ void layout_pairs_16(VolPtrs src, uint32_t *__restrict__ dst, int n_m
 
 // Occupancy of a group's resident layout: one byte per brick of
 // kOccBrick^3 padded cells, 1 when any dword of any cell in it is non-zero
-// (a 16-bit cell holds v(x) and v(x+1)).  One workgroup per brick.
+// (a 16-bit cell holds v(x) and v(x+1)).  One workgroup per brick at a
+// time, bricks strided by the grid (the grid stays below the launch limit for
+// any volume).
 constexpr int kOccBrick = 8;
 constexpr int kBlock256Waves = 4; // wavefronts of a 256-thread workgroup
+constexpr int64_t kOccGrid = 1 << 20;
 __global__ __launch_bounds__(256) void occupancy_kernel(const uint32_t *__restrict__ vol, int words, int64_t px,
                                                         int64_t py, int64_t pz, int64_t obx, int64_t oby,
-                                                        uint8_t *__restrict__ occ)
+                                                        int64_t obn, uint8_t *__restrict__ occ)
 {
-    const int64_t b = blockIdx.x;
-    const int64_t bx = b % obx, by = (b / obx) % oby, bz = b / (obx * oby);
-    int any = 0;
-    for (int c = threadIdx.x; c < kOccBrick * kOccBrick * kOccBrick; c += blockDim.x) {
-        const int64_t x = bx * kOccBrick + (c % kOccBrick), y = by * kOccBrick + (c / kOccBrick) % kOccBrick;
-        const int64_t z = bz * kOccBrick + c / (kOccBrick * kOccBrick);
-        if (x >= px || y >= py || z >= pz) continue;
-        const uint32_t *cell = vol + ((z * py + y) * px + x) * words;
-        for (int w = 0; w < words; w++) any |= cell[w] != 0u;
+    for (int64_t b = blockIdx.x; b < obn; b += gridDim.x) { // block-uniform
+        const int64_t bx = b % obx, by = (b / obx) % oby, bz = b / (obx * oby);
+        int any = 0;
+        for (int c = threadIdx.x; c < kOccBrick * kOccBrick * kOccBrick; c += blockDim.x) {
+            const int64_t x = bx * kOccBrick + (c % kOccBrick), y = by * kOccBrick + (c / kOccBrick) % kOccBrick;
+            const int64_t z = bz * kOccBrick + c / (kOccBrick * kOccBrick);
+            if (x >= px || y >= py || z >= pz) continue;
+            const uint32_t *cell = vol + ((z * py + y) * px + x) * words;
+            for (int w = 0; w < words; w++) any |= cell[w] != 0u;
+        }
+        any = __syncthreads_or(any);
+        if (threadIdx.x == 0) occ[b] = any ? 1 : 0;
+        __syncthreads(); // the reduction's scratch is free for the next brick
     }
-    any = __syncthreads_or(any);
-    if (threadIdx.x == 0) occ[b] = any ? 1 : 0;
 }
 
 // Bounds of the occupied bricks of every group: box[6 g + (0..5)] = min / max
@@ -2524,6 +2529,33 @@ int check_n_mat(int n_mat)
     return CTFP_OK;
 }
 
+// The context holds no volume (after an upload that failed part way: its
+// layout, occupancy grid and material count no longer agree).
+void drop_volume(ctfp_context *ctx)
+{
+    dfree(ctx->vol);
+    dfree(ctx->occ);
+    dfree(ctx->obox);
+    ctx->vol_bytes = 0;
+    ctx->n_mat = 0;
+    ctx->n_groups = 0;
+}
+
+static int alloc_volume_buffers(ctfp_context *ctx, size_t bytes, int64_t obx, int64_t oby, int64_t obz,
+                                int n_groups)
+{
+    if (ctx->vol && (ctx->vol_bytes != bytes)) dfree(ctx->vol);
+    if (!ctx->vol) HIP_TRY(hipMalloc(&ctx->vol, bytes));
+    ctx->vol_bytes = bytes;
+    if (ctx->occ && (ctx->ob_x != obx || ctx->ob_y != oby || ctx->ob_z != obz || ctx->n_groups != n_groups)) {
+        dfree(ctx->occ);
+        dfree(ctx->obox);
+    }
+    if (!ctx->occ) HIP_TRY(hipMalloc(&ctx->occ, (size_t)(obx * oby * obz) * n_groups));
+    if (!ctx->obox) HIP_TRY(hipMalloc(&ctx->obox, (size_t)n_groups * 6 * sizeof(int32_t)));
+    return CTFP_OK;
+}
+
 int alloc_volume(ctfp_context *ctx, int n_mat)
 {
     if (check_n_mat(n_mat)) return CTFP_ERR_UNSUPPORTED;
@@ -2542,21 +2574,16 @@ int alloc_volume(ctfp_context *ctx, int n_mat)
         gbytes[g] = (size_t)(px * py * pz) * (size_t)mats[g] * sizeof(float);
         bytes += (gbytes[g] + 255) / 256 * 256;
     }
-    if (ctx->vol && (ctx->vol_bytes != bytes)) dfree(ctx->vol);
-    if (!ctx->vol) HIP_TRY(hipMalloc(&ctx->vol, bytes));
-    ctx->vol_bytes = bytes;
     const int64_t obx = (px + kOccBrick - 1) / kOccBrick, oby = (py + kOccBrick - 1) / kOccBrick;
     const int64_t obz = (pz + kOccBrick - 1) / kOccBrick;
     // the march indexes the occupancy grid with 24-bit products
     if (obz * oby >= (1ll << 24) || obx >= (1ll << 24))
         return set_error(CTFP_ERR_UNSUPPORTED, "volume of %lld x %lld padded planes exceeds the occupancy index",
                          (long long)pz, (long long)py);
-    if (ctx->occ && (ctx->ob_x != obx || ctx->ob_y != oby || ctx->ob_z != obz || ctx->n_groups != n_groups)) {
-        dfree(ctx->occ);
-        dfree(ctx->obox);
+    if (int rc = alloc_volume_buffers(ctx, bytes, obx, oby, obz, n_groups)) {
+        drop_volume(ctx);
+        return rc;
     }
-    if (!ctx->occ) HIP_TRY(hipMalloc(&ctx->occ, (size_t)(obx * oby * obz) * n_groups));
-    if (!ctx->obox) HIP_TRY(hipMalloc(&ctx->obox, (size_t)n_groups * 6 * sizeof(int32_t)));
     ctx->ob_x = obx;
     ctx->ob_y = oby;
     ctx->ob_z = obz;
@@ -2743,8 +2770,8 @@ int ctfp_get_system_info(const ctfp_context *ctx, ctfp_system_info *out)
 // (chan[m], [z][y][x]) on `stream`, one interleaved volume per channel group.
 // Unless `async` it waits for the layout (on `stream` only) and rejects
 // volumes holding a NaN or an infinity (the context then holds no volume).
-static int write_layout(ctfp_context *ctx, const float *const *chan, int n_mat, int dtype,
-                        hipStream_t stream = nullptr, bool async = false)
+static int write_layout_impl(ctfp_context *ctx, const float *const *chan, int n_mat, int dtype, hipStream_t stream,
+                             bool async)
 {
     int rc = alloc_volume(ctx, n_mat);
     if (rc) return rc;
@@ -2771,8 +2798,8 @@ static int write_layout(ctfp_context *ctx, const float *const *chan, int n_mat, 
                                                                        ctx->pz, bad);
         HIP_TRY(hipGetLastError());
         const int64_t obn = ctx->ob_x * ctx->ob_y * ctx->ob_z;
-        occupancy_kernel<<<(unsigned)obn, 256, 0, stream>>>((const uint32_t *)dst, mg, ctx->px, ctx->py, ctx->pz,
-                                                            ctx->ob_x, ctx->ob_y, ctx->occ + g * obn);
+        occupancy_kernel<<<(unsigned)std::min<int64_t>(obn, kOccGrid), 256, 0, stream>>>(
+            (const uint32_t *)dst, mg, ctx->px, ctx->py, ctx->pz, ctx->ob_x, ctx->ob_y, obn, ctx->occ + g * obn);
         HIP_TRY(hipGetLastError());
         if (g == 0) {
             occupied_box_init<<<1, 64, 0, stream>>>(ctx->obox, 6 * ctx->n_groups);
@@ -2788,15 +2815,19 @@ static int write_layout(ctfp_context *ctx, const float *const *chan, int n_mat, 
         int h = 0;
         HIP_TRY(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
-        if (h) {
-            dfree(ctx->vol);
-            ctx->vol_bytes = 0;
-            ctx->n_mat = 0;
-            ctx->n_groups = 0;
-            return set_error(CTFP_ERR_INVALID, "volumes: a density is NaN or infinite");
-        }
+        if (h) return set_error(CTFP_ERR_INVALID, "volumes: a density is NaN or infinite");
     }
     return CTFP_OK;
+}
+
+// Any failure leaves the context with no volume (a forward then reports
+// CTFP_ERR_STATE instead of reading a layout that does not match).
+static int write_layout(ctfp_context *ctx, const float *const *chan, int n_mat, int dtype,
+                        hipStream_t stream = nullptr, bool async = false)
+{
+    const int rc = write_layout_impl(ctx, chan, n_mat, dtype, stream, async);
+    if (rc) drop_volume(ctx);
+    return rc;
 }
 
 static int check_dtype(int dtype)
@@ -3215,6 +3246,10 @@ static int64_t l_views(const ctfp_context *ctx, size_t bytes, int64_t view_count
 }
 
 constexpr size_t kLWorkspaceBytes = 512ull << 20;
+// The gather's dl/dL workspace per context: 1 GiB holds 300 views of a
+// 3-material 512 x 512 detector (3 MiB each), so a 180-view scan of that
+// size runs as one group (every brick then walks all views in one pass).
+constexpr size_t kAdjWorkspaceBytes = 1ull << 30;
 
 static int grouped_forward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, float *I, float *IA, float *L,
                            void *stream, float *lws, size_t lws_bytes);
@@ -3319,8 +3354,8 @@ int ctfp_forward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, floa
 
 // The adjoint of both backward entry points.  ws == nullptr: the gather's
 // dl/dL workspace (and, for more than four materials without L, the line
-// integrals) are the context's, allocated on first use for up to 512 MiB of
-// views per group; otherwise `ws` of ws_bytes holds them and nothing is
+// integrals) are the context's, allocated on first use for up to
+// kAdjWorkspaceBytes of views per group; otherwise `ws` of ws_bytes holds them and nothing is
 // allocated.
 static int backward_impl(ctfp_context *ctx, int64_t view_begin, int64_t view_count, const float *gI,
                          const float *gIAbsorb, const float *L, float *grad, void *stream, float *ws,
@@ -3381,7 +3416,8 @@ static int backward_impl(ctfp_context *ctx, int64_t view_begin, int64_t view_cou
                              (long long)per_view_ws, (long long)ws_bytes);
         if (need_l) lbuf = ws + group * (per_view / (int64_t)sizeof(float));
     } else {
-        group = std::max<int64_t>(1, std::min<int64_t>({view_count, (512ll << 20) / per_view, views_per_launch(ctx)}));
+        group = std::max<int64_t>(
+            1, std::min<int64_t>({view_count, (int64_t)kAdjWorkspaceBytes / per_view, views_per_launch(ctx)}));
         const size_t need = (size_t)(group * per_view);
         if (ctx->adj_ws_bytes < need) {
             dfree(ctx->adj_ws);

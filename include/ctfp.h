@@ -234,7 +234,7 @@ int ctfp_line_integrals(ctfp_context *ctx, int64_t view_begin, int64_t view_coun
  * the caller zeroes it).  L (device [view_count][n_mat][d_num_v][d_num_u],
  * from ctfp_forward) may be NULL, then the line integrals are recomputed per
  * ray.  With CTFP_ADJOINT_GATHER (the default) the result is deterministic;
- * the call holds a workspace of dl/dL for up to 512 MiB of views, kept by the
+ * the call holds a workspace of dl/dL for up to 1 GiB of views, kept by the
  * context.  With CTFP_ADJOINT_SCATTER accumulation uses FP32 atomics, so the
  * last bits can differ between runs. */
 int ctfp_backward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, const float *gI,
