@@ -82,6 +82,10 @@ struct ctfp_context {
     uint8_t *occ = nullptr;
     int64_t ob_x = 0, ob_y = 0, ob_z = 0; // bricks per axis
     int32_t *obox = nullptr; // [n_groups][6] bounds of the occupied bricks
+    // per group, the Chebyshev distance in bricks from each brick to the
+    // nearest occupied one (capped at kDistCap), then one scratch grid:
+    // [n_groups + 1][bricks]
+    uint8_t *odist = nullptr;
     // line-integral workspace of grouped forwards and adjoints
     float *l_ws = nullptr;
     size_t l_ws_bytes = 0;
@@ -290,6 +294,33 @@ __global__ __launch_bounds__(256) void occupancy_kernel(const uint32_t *__restri
     }
 }
 
+// Brick distances: dist(b) = min over occupied bricks o of max_axis |b - o|,
+// capped at kDistCap.  The cap-limited distance separates into three 1-D
+// passes, each dst(i) = min over |j - i| <= kDistCap along the axis of
+// max(|i - j|, src(j)) (src of the first pass: 0 where occupied, the cap
+// elsewhere).  Bricks strided by the grid.
+constexpr int kDistCap = 15;
+__global__ __launch_bounds__(256) void brick_distance_kernel(const uint8_t *__restrict__ src,
+                                                             uint8_t *__restrict__ dst, int64_t obx, int64_t oby,
+                                                             int64_t obz, int axis, int from_occ)
+{
+    const int64_t obn = obx * oby * obz;
+    const int64_t n = axis == 0 ? obx : axis == 1 ? oby : obz;
+    const int64_t step = axis == 0 ? 1 : axis == 1 ? obx : obx * oby;
+    for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < obn; b += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = axis == 0 ? b % obx : axis == 1 ? (b / obx) % oby : b / (obx * oby);
+        const int64_t j0 = i - kDistCap < 0 ? 0 : i - kDistCap, j1 = i + kDistCap >= n ? n - 1 : i + kDistCap;
+        int best = kDistCap;
+        for (int64_t j = j0; j <= j1; j++) {
+            const uint8_t v = src[b + (j - i) * step];
+            const int f = from_occ ? (v ? 0 : kDistCap) : (int)v;
+            const int d = (int)(j > i ? j - i : i - j);
+            best = min(best, max(d, f));
+        }
+        dst[b] = (uint8_t)best;
+    }
+}
+
 // Bounds of the occupied bricks of every group: box[6 g + (0..5)] = min / max
 // brick x, y, z (INT_MAX / INT_MIN when no brick is occupied).
 __global__ void occupied_box_init(int32_t *box, int n)
@@ -479,6 +510,10 @@ struct MarchArgs {
     // bounds of the occupied bricks (x lo, x hi, y lo, y hi, z lo, z hi, in
     // bricks), or null: the LDS march clips each ray's window to them
     const int32_t *obox;
+    // Chebyshev distance (in bricks, capped) from each brick to the nearest
+    // occupied one, or null: the LDS march also steps each ray's window ends
+    // through it
+    const uint8_t *odist;
     // ray tile: wavefronts of (1 << wu_log2) x (64 >> wu_log2) pixels,
     // (1 << bu_log2) x (4 >> bu_log2) wavefronts per workgroup
     int32_t wu_log2, bu_log2;
@@ -635,6 +670,57 @@ __device__ __forceinline__ void project_point(const MarchArgs &P, int vw, double
     v = (P.half_v - P.dsize * 0.5 - detV) / P.dsize;
 }
 
+// Steps both ends of the window [k_lo, k_hi) of the ray s + k d (index
+// space, FP64) inward past samples whose cells are all zero, through the
+// brick distances: at sample k, whose cells (floor(p) + pad and the next on
+// each axis) lie in brick B (the brick of floor(p) + pad), every brick within
+// dist(B) - 1 of B is unoccupied, so every cell within 8 (dist(B) - 1) - 1 of
+// floor(p) + pad on every axis is zero.  A sample at most 8 (dist(B) - 1) - 2
+// voxels away on every axis (the march's own positions are within 1e-5
+// voxel of these) reads only such cells: the step is that many samples of
+// the largest axis component, less one voxel for rounding.  A sample outside
+// the padded volume, or a brick within one of an occupied brick, ends the
+// stepping.  Every sample stepped over adds exact zeros.
+constexpr int kTraceSteps = 64;
+__device__ __forceinline__ void trace_window_ends(const MarchArgs &P, double sa, double sb, double sc, double da,
+                                                  double db, double dc, int &k_lo, int &k_hi)
+{
+#pragma clang fp contract(off)
+    const double amax = fmax(fabs(da), fmax(fabs(db), fabs(dc)));
+    if (!(amax > 0.0)) return;
+    const double inv = 1.0 / amax;
+    auto dist_at = [&](int k) -> int {
+        const double kk = (double)k;
+        const double x = floor(sa + da * kk) + kVoxPadLo, y = floor(sb + db * kk) + kVoxPadLo;
+        const double z = floor(sc + dc * kk) + kVoxPadLo;
+        if (!(x >= 0.0 && y >= 0.0 && z >= 0.0 && x < (double)P.pcx && y < (double)P.pcy && z < (double)P.pcz))
+            return 0;
+        const int64_t bx = (int64_t)x / kOccBrick, by = (int64_t)y / kOccBrick, bz = (int64_t)z / kOccBrick;
+        return P.odist[(bz * P.ob_y + by) * P.ob_x + bx];
+    };
+    auto step_of = [&](int d) { return (int)floor((8.0 * (d - 1) - 3.0) * inv); };
+    int k = k_lo;
+    for (int it = 0; it < kTraceSteps && k < k_hi; it++) {
+        const int d = dist_at(k);
+        const int n = d >= 2 ? step_of(d) : 0;
+        if (n < 1) break;
+        k += n; // samples k .. k + n read only zero cells
+    }
+    if (k >= k_hi) {
+        k_hi = k_lo; // the whole window reads zeros
+        return;
+    }
+    k_lo = max(k_lo, k - 1);
+    int e = k_hi - 1;
+    for (int it = 0; it < kTraceSteps && e > k_lo; it++) {
+        const int d = dist_at(e);
+        const int n = d >= 2 ? step_of(d) : 0;
+        if (n < 1) break;
+        e -= n;
+    }
+    k_hi = min(k_hi, e + 2);
+}
+
 // kBox: also clip the window to P.obox, the occupied bricks, when set.  The
 // positions stay anchored at the volume window's start, so a clipped ray
 // samples the same positions and only leaves out samples whose corners are
@@ -678,6 +764,7 @@ __device__ __forceinline__ RaySetup setup_ray(const MarchArgs &P, int u, int v, 
                 k_hi = k_lo;
             }
         }
+        if (P.odist && k_hi > k_lo) trace_window_ends(P, sa, sb2, sc, da, db, dc, k_lo, k_hi);
         if (k_hi > k_lo) {
             // Whole anchor segments: the window grows to the anchor grid
             // (up to kAnchor - 1 samples each side, outside the window
@@ -2536,6 +2623,7 @@ void drop_volume(ctfp_context *ctx)
     dfree(ctx->vol);
     dfree(ctx->occ);
     dfree(ctx->obox);
+    dfree(ctx->odist);
     ctx->vol_bytes = 0;
     ctx->n_mat = 0;
     ctx->n_groups = 0;
@@ -2550,9 +2638,11 @@ static int alloc_volume_buffers(ctfp_context *ctx, size_t bytes, int64_t obx, in
     if (ctx->occ && (ctx->ob_x != obx || ctx->ob_y != oby || ctx->ob_z != obz || ctx->n_groups != n_groups)) {
         dfree(ctx->occ);
         dfree(ctx->obox);
+        dfree(ctx->odist);
     }
     if (!ctx->occ) HIP_TRY(hipMalloc(&ctx->occ, (size_t)(obx * oby * obz) * n_groups));
     if (!ctx->obox) HIP_TRY(hipMalloc(&ctx->obox, (size_t)n_groups * 6 * sizeof(int32_t)));
+    if (!ctx->odist) HIP_TRY(hipMalloc(&ctx->odist, (size_t)(obx * oby * obz) * (n_groups + 1)));
     return CTFP_OK;
 }
 
@@ -2647,6 +2737,7 @@ int fill_march_args(ctfp_context *ctx, int64_t view_begin, int64_t view_count, M
     P.l_moff = ctx->group_moff[group];
     P.occ = ctx->tune.skip_empty && ctx->occ ? ctx->occ + (size_t)group * ctx->ob_x * ctx->ob_y * ctx->ob_z : nullptr;
     P.obox = P.occ ? ctx->obox + 6 * group : nullptr;
+    P.odist = P.occ ? ctx->odist + (size_t)group * ctx->ob_x * ctx->ob_y * ctx->ob_z : nullptr;
     P.ob_x = (int32_t)ctx->ob_x;
     P.ob_y = (int32_t)ctx->ob_y;
     P.pcx = (int32_t)ctx->px;
@@ -2744,6 +2835,7 @@ int ctfp_destroy(ctfp_context *ctx)
     dfree(ctx->vol);
     dfree(ctx->occ);
     dfree(ctx->obox);
+    dfree(ctx->odist);
     dfree(ctx->d_flag);
     dfree(ctx->l_ws);
     dfree(ctx->adj_ws);
@@ -2807,6 +2899,14 @@ static int write_layout_impl(ctfp_context *ctx, const float *const *chan, int n_
         }
         occupied_box_kernel<<<(unsigned)std::min<int64_t>(1024, (obn + 255) / 256), 256, 0, stream>>>(
             ctx->occ + g * obn, ctx->ob_x, ctx->ob_y, obn, ctx->obox + 6 * g);
+        HIP_TRY(hipGetLastError());
+        // brick distances: x from the occupancy into the group's grid, y into
+        // the scratch grid, z back into the group's grid
+        uint8_t *dg = ctx->odist + g * obn, *tmp = ctx->odist + ctx->n_groups * obn;
+        const unsigned dgrid = (unsigned)std::min<int64_t>(kOccGrid, (obn + 255) / 256);
+        brick_distance_kernel<<<dgrid, 256, 0, stream>>>(ctx->occ + g * obn, dg, ctx->ob_x, ctx->ob_y, ctx->ob_z, 0, 1);
+        brick_distance_kernel<<<dgrid, 256, 0, stream>>>(dg, tmp, ctx->ob_x, ctx->ob_y, ctx->ob_z, 1, 0);
+        brick_distance_kernel<<<dgrid, 256, 0, stream>>>(tmp, dg, ctx->ob_x, ctx->ob_y, ctx->ob_z, 2, 0);
         HIP_TRY(hipGetLastError());
     }
     ctx->dtype = dtype;

@@ -690,6 +690,48 @@ def test_skip_empty_chunks_is_exact(require_gpu, dtype):
         assert ((I.double() - I0.double()[None]).abs() / I0.double()[None]).max().item() < 1e-6
 
 
+@pytest.mark.parametrize("dtype", [P.CTFP_F32, P.CTFP_F16, P.CTFP_BF16])
+def test_window_ends_stepped_through_brick_distances(require_gpu, dtype):
+    """Isolated voxels in empty space, several on the first or last voxel of
+    an 8-cell brick (padded cell = voxel + 2), plus two balls far apart: each
+    ray's window ends are stepped inward through the brick distances, and
+    every output is bit-identical with empty-space skipping off.  The stepped
+    windows evaluate a small part of the lattice."""
+    import torch
+    s, vols = small_three_material(n=96, views=8, det=(160, 120), d_size=0.8)
+    kw = P.spectral_inputs(s, *s.det)
+    n = vols[0].shape[0]
+    rng = np.random.default_rng(21)
+    pts = [(5, 5, 5), (6, 13, 22), (45, 46, 47), (94, 1, 50), (0, 95, 95), (93, 94, 6), (13, 14, 21)]
+    pts += [tuple(int(c) for c in rng.integers(0, n, 3)) for _ in range(12)]
+    sparse = [np.zeros((n, n, n), np.float32) for _ in range(3)]
+    for i, (z, y, x) in enumerate(pts):
+        sparse[i % 3][z, y, x] = 1.0 + 0.1 * i
+    zz, yy, xx = np.meshgrid(*(np.arange(n, dtype=np.float64),) * 3, indexing="ij")
+    balls = [np.zeros((n, n, n), np.float32) for _ in range(3)]
+    for (cz, cy, cx), r, m in [((20.0, 30.0, 25.0), 6.0, 0), ((70.0, 65.0, 72.0), 9.0, 1)]:
+        inside = (zz - cz) ** 2 + (yy - cy) ** 2 + (xx - cx) ** 2 <= r * r
+        balls[m][inside] = 1.0
+        balls[2][inside] = 0.02
+    with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
+        proj.set_spectral(**kw)
+        proj.set_volumes(sparse, storage_dtype=dtype)
+        window = proj.marched_samples()
+        for name, v in [("isolated voxels", sparse), ("two balls", balls)]:
+            proj.set_volumes(v, storage_dtype=dtype)
+            for chunk, wave_u, block_u in [(16, 8, 2), (32, 16, 1)]:
+                print(f"window ends: {name} dtype {dtype} chunk {chunk} wave_u {wave_u}", flush=True)
+                proj.set_tuning(chunk=chunk, wave_u=wave_u, block_u=block_u, skip_empty=False)
+                ref = proj.forward(want_L=True)
+                proj.set_tuning(skip_empty=True)
+                got = proj.forward(want_L=True)
+                evaluated = proj.evaluated_samples()
+                torch.cuda.synchronize()
+                for a, b in zip(got, ref):
+                    assert torch.equal(a, b), (name, chunk, wave_u)
+                assert 0 < evaluated < 0.2 * window, (name, evaluated, window)
+
+
 @pytest.fixture(scope="module")
 def wide_three_material():
     """Three channels of 1000 x 1000 x 708 voxels: the padded FP32 layout
