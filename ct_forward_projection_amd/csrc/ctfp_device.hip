@@ -2159,6 +2159,10 @@ struct GatherArgs {
 };
 
 constexpr int kBrickX = 16, kBrickY = 16, kBrickZ = 4;
+// one wavefront per z layer of the brick (8 x 8 quads)
+constexpr int kGatherBlock = 64 * kBrickZ;
+// LDS of a gather workgroup: the view table and the staged pixel records
+constexpr int kGatherLds = 40 * 1024;
 
 // A brick's shadow in one view: the projection of its centre, the Jacobian
 // (pixels per voxel step), the half-extent of one voxel's support in pixels,
@@ -2233,7 +2237,7 @@ __device__ __forceinline__ void pair_pixels(const float *rec0, int rs, int q0, i
 }
 
 template <int M>
-__global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
+__global__ __launch_bounds__(kGatherBlock) void gather_kernel(GatherArgs A)
 {
     extern __shared__ char lds_raw[];
     const MarchArgs &P = A.P;
@@ -2333,7 +2337,7 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(GatherArgs A)
                     const int b1 = min(rv1, b0 + band - 1);
                     const int count = (b1 - b0 + 1) * rw;
                     __syncthreads(); // the previous band's entries are consumed
-                    for (int i = threadIdx.x; i < count; i += kBlock) {
+                    for (int i = threadIdx.x; i < count; i += kGatherBlock) {
                         const int pv = b0 + i / rw, pu = c0 + i % rw;
                         const RayLine Ln = ray_line(P, pu, pv, (int)(P.view_begin + vw));
                         // position at sample k: q0 + k * d, relative to the brick centre
@@ -2694,6 +2698,8 @@ int alloc_volume(ctfp_context *ctx, int n_mat)
 // Workgroups of one launch: the grid's work-items (blocks x 256) must stay
 // below 2^32, and the march launches a multiple of 8 blocks.
 constexpr uint64_t kMaxLaunchBlocks = ((1ull << 32) - 1) / kBlock / 8 * 8;
+// the gather's workgroups hold kGatherBlock work-items
+constexpr uint64_t kMaxGatherBricks = ((1ull << 32) - 1) / kGatherBlock;
 
 // Views per march launch so that no launch exceeds kMaxLaunchBlocks (every
 // view has the same tiles).
@@ -3531,7 +3537,6 @@ static int backward_impl(ctfp_context *ctx, int64_t view_begin, int64_t view_cou
             lbuf = ctx->l_ws;
         }
     }
-    const int gather_lds = 40 * 1024;
     GatherArgs A;
     A.grad = grad;
     A.gL = ws;
@@ -3571,18 +3576,19 @@ static int backward_impl(ctfp_context *ctx, int64_t view_begin, int64_t view_cou
             HIP_TRY(hipGetLastError());
         }
         // the voxel pass, per channel group, in launches of at most
-        // kMaxLaunchBlocks bricks
+        // kMaxGatherBricks bricks
         for (int gi = 0; gi < ctx->n_groups; gi++) {
             GatherFn gfn = pick_gather(ctx->group_mats[gi]);
             if (!gfn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->group_mats[gi]);
             if ((rc = fill_march_args(ctx, view_begin + v0, n, A.P, gi))) return rc;
             A.mat_off = ctx->group_moff[gi];
-            A.cap = (int32_t)((gather_lds - 256 - kGatherViews * (int)sizeof(BrickView)) /
+            A.cap = (int32_t)((kGatherLds - 256 - kGatherViews * (int)sizeof(BrickView)) /
                               (gather_pixel_words(ctx->group_mats[gi]) * 4));
-            for (int64_t b0 = 0; b0 < bricks; b0 += (int64_t)kMaxLaunchBlocks) {
+            for (int64_t b0 = 0; b0 < bricks; b0 += (int64_t)kMaxGatherBricks) {
                 A.brick_base = (uint32_t)b0;
-                const int64_t nb = std::min<int64_t>((int64_t)kMaxLaunchBlocks, bricks - b0);
-                hipLaunchKernelGGL(gfn, dim3((unsigned)nb), dim3(kBlock), (size_t)gather_lds, (hipStream_t)stream, A);
+                const int64_t nb = std::min<int64_t>((int64_t)kMaxGatherBricks, bricks - b0);
+                hipLaunchKernelGGL(gfn, dim3((unsigned)nb), dim3(kGatherBlock), (size_t)kGatherLds,
+                                   (hipStream_t)stream, A);
                 HIP_TRY(hipGetLastError());
             }
         }
