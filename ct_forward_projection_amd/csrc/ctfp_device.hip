@@ -2141,12 +2141,10 @@ __global__ __launch_bounds__(kBlock) void ray_grad_kernel(MarchArgs P, AdjointAr
 // the ray's closest approach to the brick centre (FP64-derived offset e and
 // sample index k*), its per-sample step and the ray's gL, so that the voxel
 // loop runs in FP32 on small numbers (positions good to ~1e-6 voxel).  The
-// quad is walked as two pairs of neighbours along x or along y, whichever
-// moves the brick's shadow less in that view (the axis closer to the rays):
-// the two voxels of a pair see nearly the same pixels, so they share the
-// pixel's bounds, the hats of the other two axes and the sample loop.  Each
-// voxel writes its own gradient: no atomics, and the result is
-// deterministic.
+// quad's four voxels see nearly the same pixels, so they are walked together:
+// one pixel window (their four joined), one sample interval per pixel (the
+// union of theirs) and the z hat shared.  Each voxel writes its own
+// gradient: no atomics, and the result is deterministic.
 struct GatherArgs {
     MarchArgs P;       // geometry of the scan (views [view_begin, +n_views))
     const float *gL;   // [n_views][gl_mats][nv][nu]
@@ -2158,19 +2156,20 @@ struct GatherArgs {
     int32_t cap;       // staged pixels that fit the LDS budget
 };
 
-constexpr int kBrickX = 16, kBrickY = 16, kBrickZ = 4;
-// one wavefront per z layer of the brick (8 x 8 quads)
-constexpr int kGatherBlock = 64 * kBrickZ;
+// Each thread owns a cell of 2 x 2 x kCellZ voxels; a wavefront holds one z
+// layer of 8 x 8 cells, a workgroup four such layers.
+constexpr int kCellZ = 1;
+constexpr int kBrickX = 16, kBrickY = 16, kBrickZ = 4 * kCellZ;
+constexpr int kGatherBlock = 256;
 // LDS of a gather workgroup: the view table and the staged pixel records
 constexpr int kGatherLds = 40 * 1024;
 
 // A brick's shadow in one view: the projection of its centre, the Jacobian
-// (pixels per voxel step), the half-extent of one voxel's support in pixels,
-// the shadow's pixel rectangle, and the axis of the view's voxel pairs.
+// (pixels per voxel step), the half-extent of one voxel's support in pixels
+// and the shadow's pixel rectangle.
 struct BrickView {
     float u0, v0, jua, jub, juc, jva, jvb, jvc, hu, hv;
     int ru0, ru1, rv0, rv1;
-    int pair_y; // 1: pairs along y (a y step moves the shadow less than an x step)
 };
 // Views whose brick shadows a workgroup computes at once (one lane each).
 constexpr int kGatherViews = 64;
@@ -2187,50 +2186,67 @@ constexpr int kGLo = 0, kGHi = 3, kGId = 6, kGKf = 9, kGKr = 10, kGE = 11, kGD =
 // hat(r) = max(0, 1 - |r|), one clamped subtraction
 __device__ __forceinline__ float hat(float r) { return __builtin_amdgcn_fmed3f(1.0f - fabsf(r), 0.0f, 1.0f); }
 
-// One pair of neighbouring voxels along axis AX (0: x, 1: y) at offsets oP
-// and oP + 1 from the brick centre on that axis, oQ on the other in-plane
-// axis and oz on z: for each staged pixel of rows [q0, q1] x columns
-// [w0, w1] (records from `rec0`, row stride rs records), the samples inside
-// either voxel's support, weighted into wA (the voxel at oP) and wB (at
-// oP + 1) and then times the pixel's dl/dL into accA / accB.  A sample
-// outside one voxel's support has a zero hat there, so the union of the two
-// intervals is walked once.
-template <int M, int AX>
-__device__ __forceinline__ void pair_pixels(const float *rec0, int rs, int q0, int q1, int w0, int w1, float oP,
-                                            float oQ, float oz, int K, float (&accA)[M], float (&accB)[M])
+// The thread's 2 x 2 x NZ cell of voxels at offsets (ox + {0, 1},
+// oy + {0, 1}, oz + {0, NZ - 1}) from the brick centre: for each staged pixel
+// of rows [q0, q1] x columns [w0, w1] (records from `rec0`, row stride rs
+// records), the samples inside any of the voxels' supports (on each axis
+// the union of the shifted intervals), weighted into every voxel and then
+// times the pixel's dl/dL into acc[z][y][x].  A sample outside a voxel's
+// support has a zero hat there.  Per sample (NZ = 2): three positions, six
+// hats, four (y, z) products and eight accumulations for the eight voxels.
+template <int M, int NZ>
+__device__ __forceinline__ void cell_pixels(const float *rec0, int rs, int q0, int q1, int w0, int w1, float ox,
+                                            float oy, float oz, int K, float (&acc)[NZ][2][2][M])
 {
     constexpr int PW = gather_pixel_words(M);
-    constexpr int IP = AX, IQ = 1 - AX;
+    constexpr float kZ = (float)(NZ - 1);
     for (int pv = q0; pv <= q1; pv++) {
         const float *rec = rec0 + (pv - q0) * rs * PW;
         for (int pu = w0; pu <= w1; pu++, rec += PW) {
-            const float iP = rec[kGId + IP], iQ = rec[kGId + IQ], iZ = rec[kGId + 2];
+            const float iX = rec[kGId + 0], iY = rec[kGId + 1], iZ = rec[kGId + 2];
             // support in j = k - kf: the centre bounds shifted by o / d on each
-            // axis; on the pair's axis the union of the shifts by oP and oP + 1
-            const float pl = fmaf(oP, iP, rec[kGLo + IP]), ph = fmaf(oP, iP, rec[kGHi + IP]);
-            const float jlo = fmaxf(fmaxf(fmaf(oQ, iQ, rec[kGLo + IQ]), fmaf(oz, iZ, rec[kGLo + 2])),
-                                    fminf(pl, pl + iP));
-            const float jhi = fminf(fminf(fmaf(oQ, iQ, rec[kGHi + IQ]), fmaf(oz, iZ, rec[kGHi + 2])),
-                                    fmaxf(ph, ph + iP));
+            // axis, the union over the cell's shifts
+            const float xl = fmaf(ox, iX, rec[kGLo + 0]), xh = fmaf(ox, iX, rec[kGHi + 0]);
+            const float yl = fmaf(oy, iY, rec[kGLo + 1]), yh = fmaf(oy, iY, rec[kGHi + 1]);
+            const float zl = fmaf(oz, iZ, rec[kGLo + 2]), zh = fmaf(oz, iZ, rec[kGHi + 2]);
+            const float jlo = fmaxf(fmaxf(fminf(xl, xl + iX), fminf(yl, yl + iY)), fminf(zl, fmaf(kZ, iZ, zl)));
+            const float jhi = fminf(fminf(fmaxf(xh, xh + iX), fmaxf(yh, yh + iY)), fmaxf(zh, fmaf(kZ, iZ, zh)));
             const int kf = __float_as_int(rec[kGKf]);
             // j over the open interval, and 0 <= k < K
             const int j0 = max((int)ceilf(jlo), -kf);
             const int j1 = min((int)floorf(jhi), K - 1 - kf);
             if (j0 > j1) continue;
             const float kr = rec[kGKr];
-            const float fP = rec[kGE + IP] - oP, fQ = rec[kGE + IQ] - oQ, fZ = rec[kGE + 2] - oz;
-            const float dP = rec[kGD + IP], dQ = rec[kGD + IQ], dZ = rec[kGD + 2];
-            float wA = 0.0f, wB = 0.0f, sj = (float)j0 - kr;
+            const float fX = rec[kGE + 0] - ox, fY = rec[kGE + 1] - oy, fZ = rec[kGE + 2] - oz;
+            const float dX = rec[kGD + 0], dY = rec[kGD + 1], dZ = rec[kGD + 2];
+            float w[NZ][2][2];
+#pragma unroll
+            for (int c = 0; c < NZ; c++)
+                w[c][0][0] = w[c][0][1] = w[c][1][0] = w[c][1][1] = 0.0f;
+            float sj = (float)j0 - kr;
             for (int j = j0; j <= j1; j++, sj += 1.0f) {
-                const float rqz = hat(fmaf(sj, dQ, fQ)) * hat(fmaf(sj, dZ, fZ));
-                const float t = fmaf(sj, dP, fP);
-                wA = fmaf(rqz, hat(t), wA);
-                wB = fmaf(rqz, hat(t - 1.0f), wB);
+                const float tz = fmaf(sj, dZ, fZ), ty = fmaf(sj, dY, fY), tx = fmaf(sj, dX, fX);
+                const float hy[2] = {hat(ty), hat(ty - 1.0f)}, hx[2] = {hat(tx), hat(tx - 1.0f)};
+#pragma unroll
+                for (int c = 0; c < NZ; c++) {
+                    const float hz = hat(tz - (float)c);
+#pragma unroll
+                    for (int b = 0; b < 2; b++) {
+                        const float hyz = hy[b] * hz;
+                        w[c][b][0] = fmaf(hyz, hx[0], w[c][b][0]);
+                        w[c][b][1] = fmaf(hyz, hx[1], w[c][b][1]);
+                    }
+                }
             }
 #pragma unroll
             for (int m = 0; m < M; m++) {
-                accA[m] = fmaf(wA, rec[kGG + m], accA[m]);
-                accB[m] = fmaf(wB, rec[kGG + m], accB[m]);
+                const float g = rec[kGG + m];
+#pragma unroll
+                for (int c = 0; c < NZ; c++)
+#pragma unroll
+                    for (int b = 0; b < 2; b++)
+#pragma unroll
+                        for (int a = 0; a < 2; a++) acc[c][b][a][m] = fmaf(w[c][b][a], g, acc[c][b][a][m]);
             }
         }
     }
@@ -2250,24 +2266,26 @@ __global__ __launch_bounds__(kGatherBlock) void gather_kernel(GatherArgs A)
     const int bx = (int)(brick % (uint32_t)A.bricks_x);
     const int by = (int)((brick / (uint32_t)A.bricks_x) % (uint32_t)A.bricks_y);
     const int bz = (int)(brick / ((uint32_t)A.bricks_x * (uint32_t)A.bricks_y));
-    // the thread's quad: voxels (x0 + {0, 1}, y0 + {0, 1}, z)
-    const int lx = 2 * (threadIdx.x & 7), ly = 2 * ((threadIdx.x >> 3) & 7), lz = threadIdx.x >> 6;
-    const int x0 = bx * kBrickX + lx, y0 = by * kBrickY + ly, z = bz * kBrickZ + lz;
+    // the thread's cell: voxels (x0 + {0, 1}, y0 + {0, 1}, z0 + {0 .. kCellZ - 1})
+    const int lx = 2 * (threadIdx.x & 7), ly = 2 * ((threadIdx.x >> 3) & 7), lz = kCellZ * (threadIdx.x >> 6);
+    const int x0 = bx * kBrickX + lx, y0 = by * kBrickY + ly, z0 = bz * kBrickZ + lz;
     const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
-    const bool inside = x0 < nx && y0 < ny && z < nz; // the quad's first voxel
-    // brick centre (index space) and the quad's first voxel's offset from it
+    const bool inside = x0 < nx && y0 < ny && z0 < nz; // the cell's first voxel
+    // brick centre (index space) and the cell's first voxel's offset from it
     const double ca = bx * kBrickX + 0.5 * (kBrickX - 1), cb = by * kBrickY + 0.5 * (kBrickY - 1);
     const double cc = bz * kBrickZ + 0.5 * (kBrickZ - 1);
     const float ox = (float)(lx - 0.5 * (kBrickX - 1)), oy = (float)(ly - 0.5 * (kBrickY - 1));
     const float oz = (float)(lz - 0.5 * (kBrickZ - 1));
     const int64_t npix = (int64_t)P.nu * P.nv;
-    float acc[2][2][M]; // [y - y0][x - x0][m]
+    float acc[kCellZ][2][2][M]; // [z - z0][y - y0][x - x0][m]
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int c = 0; c < kCellZ; c++)
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+        for (int i = 0; i < 2; i++)
 #pragma unroll
-            for (int m = 0; m < M; m++) acc[i][j][m] = 0.0f;
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int m = 0; m < M; m++) acc[c][i][j][m] = 0.0f;
     for (int vt = 0; vt < P.n_views; vt += kGatherViews) {
         const int nvt = min(kGatherViews, P.n_views - vt);
         __syncthreads(); // the previous views' table is consumed
@@ -2301,7 +2319,6 @@ __global__ __launch_bounds__(kGatherBlock) void gather_kernel(GatherArgs A)
             bv.ru1 = min(P.nu - 1, (int)floor(u0 + eu));
             bv.rv0 = max(0, (int)ceil(v0 - ev));
             bv.rv1 = min(P.nv - 1, (int)floor(v0 + ev));
-            bv.pair_y = fabsf(bv.jub) < fabsf(bv.jua) ? 1 : 0;
             s_view[threadIdx.x] = bv;
         }
         __syncthreads();
@@ -2309,24 +2326,15 @@ __global__ __launch_bounds__(kGatherBlock) void gather_kernel(GatherArgs A)
             const BrickView bv = s_view[vw - vt];
             const int ru0 = bv.ru0, ru1 = bv.ru1, rv0 = bv.rv0, rv1 = bv.rv1;
             if (ru0 > ru1 || rv0 > rv1) continue; // block-uniform
-            // the two pairs of the quad: along x (pair p at y0 + p) or along y
-            // (pair p at x0 + p); each pair's pixel window is its two voxels'
-            // windows joined (inside the brick's)
-            const bool py = bv.pair_y != 0; // block-uniform
-            const float jP_u = py ? bv.jub : bv.jua, jP_v = py ? bv.jvb : bv.jva;
-            const float jQ_u = py ? bv.jua : bv.jub, jQ_v = py ? bv.jva : bv.jvb;
-            const float oP = py ? oy : ox, oQ0 = py ? ox : oy;
-            const float hu = bv.hu + 0.5f * fabsf(jP_u), hv = bv.hv + 0.5f * fabsf(jP_v);
-            int pu0[2], pu1[2], pv0[2], pv1[2];
-#pragma unroll
-            for (int pp = 0; pp < 2; pp++) {
-                const float cu = bv.u0 + jP_u * (oP + 0.5f) + jQ_u * (oQ0 + (float)pp) + bv.juc * oz;
-                const float cv = bv.v0 + jP_v * (oP + 0.5f) + jQ_v * (oQ0 + (float)pp) + bv.jvc * oz;
-                pu0[pp] = max(ru0, (int)ceilf(cu - hu));
-                pu1[pp] = min(ru1, (int)floorf(cu + hu));
-                pv0[pp] = max(rv0, (int)ceilf(cv - hv));
-                pv1[pp] = min(rv1, (int)floorf(cv + hv));
-            }
+            // the cell's pixel window: its voxels' windows joined (inside the
+            // brick's)
+            constexpr float hz = 0.5f * (kCellZ - 1);
+            const float hu = bv.hu + 0.5f * (fabsf(bv.jua) + fabsf(bv.jub)) + hz * fabsf(bv.juc);
+            const float hv = bv.hv + 0.5f * (fabsf(bv.jva) + fabsf(bv.jvb)) + hz * fabsf(bv.jvc);
+            const float cu = bv.u0 + bv.jua * (ox + 0.5f) + bv.jub * (oy + 0.5f) + bv.juc * (oz + hz);
+            const float cv = bv.v0 + bv.jva * (ox + 0.5f) + bv.jvb * (oy + 0.5f) + bv.jvc * (oz + hz);
+            const int pu0 = max(ru0, (int)ceilf(cu - hu)), pu1 = min(ru1, (int)floorf(cu + hu));
+            const int pv0 = max(rv0, (int)ceilf(cv - hv)), pv1 = min(rv1, (int)floorf(cv + hv));
             // the brick's shadow in pieces of at most `cap` pixels: column
             // segments no wider than cap, each in bands of whole rows
             for (int c0 = ru0; c0 <= ru1; c0 += cap) {
@@ -2377,34 +2385,28 @@ __global__ __launch_bounds__(kGatherBlock) void gather_kernel(GatherArgs A)
                     }
                     __syncthreads();
                     if (!inside) continue;
-#pragma unroll
-                    for (int pp = 0; pp < 2; pp++) {
-                        const int q0 = max(pv0[pp], b0), q1 = min(pv1[pp], b1);
-                        const int w0 = max(pu0[pp], c0), w1 = min(pu1[pp], c1);
-                        if (q0 > q1 || w0 > w1) continue;
-                        const float *rec0 = s_px + ((q0 - b0) * rw + (w0 - c0)) * PW;
-                        if (py)
-                            pair_pixels<M, 1>(rec0, rw, q0, q1, w0, w1, oy, ox + (float)pp, oz, P.K, acc[0][pp],
-                                              acc[1][pp]);
-                        else
-                            pair_pixels<M, 0>(rec0, rw, q0, q1, w0, w1, ox, oy + (float)pp, oz, P.K, acc[pp][0],
-                                              acc[pp][1]);
-                    }
+                    const int q0 = max(pv0, b0), q1 = min(pv1, b1);
+                    const int w0 = max(pu0, c0), w1 = min(pu1, c1);
+                    if (q0 > q1 || w0 > w1) continue;
+                    cell_pixels<M, kCellZ>(s_px + ((q0 - b0) * rw + (w0 - c0)) * PW, rw, q0, q1, w0, w1, ox, oy,
+                                           oz, P.K, acc);
                 }
             }
         }
     }
     const int64_t nvox = (int64_t)nx * ny * nz;
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int c = 0; c < kCellZ; c++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int x = x0 + j, y = y0 + i;
-            if (x >= nx || y >= ny || z >= nz) continue;
-            const int64_t q = ((int64_t)z * ny + y) * nx + x;
+        for (int i = 0; i < 2; i++)
 #pragma unroll
-            for (int m = 0; m < M; m++) A.grad[(int64_t)(A.mat_off + m) * nvox + q] += acc[i][j][m];
-        }
+            for (int j = 0; j < 2; j++) {
+                const int x = x0 + j, y = y0 + i, z = z0 + c;
+                if (x >= nx || y >= ny || z >= nz) continue;
+                const int64_t q = ((int64_t)z * ny + y) * nx + x;
+#pragma unroll
+                for (int m = 0; m < M; m++) A.grad[(int64_t)(A.mat_off + m) * nvox + q] += acc[c][i][j][m];
+            }
 }
 
 template <int M, int ST>
