@@ -698,11 +698,16 @@ __device__ __forceinline__ void trace_window_ends(const MarchArgs &P, double sa,
         const int64_t bx = (int64_t)x / kOccBrick, by = (int64_t)y / kOccBrick, bz = (int64_t)z / kOccBrick;
         return P.odist[(bz * P.ob_y + by) * P.ob_x + bx];
     };
-    auto step_of = [&](int d) { return (int)floor((8.0 * (d - 1) - 3.0) * inv); };
+    // the step for distance d, at most `limit` samples (converted only once
+    // it fits an int)
+    auto step_of = [&](int d, int limit) {
+        const double n = floor((8.0 * (d - 1) - 3.0) * inv);
+        return n >= (double)limit ? limit : (int)n;
+    };
     int k = k_lo;
     for (int it = 0; it < kTraceSteps && k < k_hi; it++) {
         const int d = dist_at(k);
-        const int n = d >= 2 ? step_of(d) : 0;
+        const int n = d >= 2 ? step_of(d, k_hi - k) : 0;
         if (n < 1) break;
         k += n; // samples k .. k + n read only zero cells
     }
@@ -714,7 +719,7 @@ __device__ __forceinline__ void trace_window_ends(const MarchArgs &P, double sa,
     int e = k_hi - 1;
     for (int it = 0; it < kTraceSteps && e > k_lo; it++) {
         const int d = dist_at(e);
-        const int n = d >= 2 ? step_of(d) : 0;
+        const int n = d >= 2 ? step_of(d, e - k_lo + 1) : 0;
         if (n < 1) break;
         e -= n;
     }
