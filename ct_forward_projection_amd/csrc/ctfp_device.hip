@@ -2213,9 +2213,13 @@ __device__ __forceinline__ void cell_pixels(const float *rec0, int rs, int q0, i
             // axis, the union over the cell's shifts
             const float xl = fmaf(ox, iX, rec[kGLo + 0]), xh = fmaf(ox, iX, rec[kGHi + 0]);
             const float yl = fmaf(oy, iY, rec[kGLo + 1]), yh = fmaf(oy, iY, rec[kGHi + 1]);
-            const float zl = fmaf(oz, iZ, rec[kGLo + 2]), zh = fmaf(oz, iZ, rec[kGHi + 2]);
-            const float jlo = fmaxf(fmaxf(fminf(xl, xl + iX), fminf(yl, yl + iY)), fminf(zl, fmaf(kZ, iZ, zl)));
-            const float jhi = fminf(fminf(fmaxf(xh, xh + iX), fmaxf(yh, yh + iY)), fmaxf(zh, fmaf(kZ, iZ, zh)));
+            float zl = fmaf(oz, iZ, rec[kGLo + 2]), zh = fmaf(oz, iZ, rec[kGHi + 2]);
+            if constexpr (NZ > 1) {
+                zl = fminf(zl, fmaf(kZ, iZ, zl));
+                zh = fmaxf(zh, fmaf(kZ, iZ, zh));
+            }
+            const float jlo = fmaxf(fmaxf(fminf(xl, xl + iX), fminf(yl, yl + iY)), zl);
+            const float jhi = fminf(fminf(fmaxf(xh, xh + iX), fmaxf(yh, yh + iY)), zh);
             const int kf = __float_as_int(rec[kGKf]);
             // j over the open interval, and 0 <= k < K
             const int j0 = max((int)ceilf(jlo), -kf);
