@@ -2101,15 +2101,29 @@ __device__ __forceinline__ void ray_gradient(const MarchArgs &P, const AdjointAr
             KSum sm[M];
 #pragma unroll
             for (int m = 0; m < M; m++) sm[m] = ksum_zero();
-            for (int e = 0; e < P.n_act; e++) {
-                float tau = 0.0f;
+            // each group of four bins summed in FP32, then one compensated
+            // add.  The weight rows past the last active bin are zeros
+            // (kWeightPadBins of them), so a group past the end adds exact
+            // zeros (its A rows are the last bin's).
+            const int last = P.n_act - 1;
+            for (int e0 = 0; e0 < P.n_act; e0 += 4) {
+                float w4[4], part[M];
 #pragma unroll
-                for (int m = 0; m < M; m++) tau = fmaf(P.A[e * M + m], Lm[m], tau);
-                float ex, ab;
-                attenuate(tau, ex, ab);
-                const float we = wp[(int64_t)e * P.w_stride_e] * ex;
+                for (int k = 0; k < 4; k++) w4[k] = wp[(int64_t)(e0 + k) * P.w_stride_e];
 #pragma unroll
-                for (int m = 0; m < M; m++) ksum_add(sm[m], P.A[e * M + m] * we);
+                for (int m = 0; m < M; m++) part[m] = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int e = min(e0 + k, last);
+                    float tau = 0.0f;
+#pragma unroll
+                    for (int m = 0; m < M; m++) tau = fmaf(P.A[e * M + m], Lm[m], tau);
+                    const float we = w4[k] * __builtin_amdgcn_exp2f(-tau * 1.44269504088896341f);
+#pragma unroll
+                    for (int m = 0; m < M; m++) part[m] = fmaf(P.A[e * M + m], we, part[m]);
+                }
+#pragma unroll
+                for (int m = 0; m < M; m++) ksum_add(sm[m], part[m]);
             }
 #pragma unroll
             for (int m = 0; m < M; m++) g[m] = c * ksum_value(sm[m]) * P.dx_f;
@@ -2126,7 +2140,8 @@ __global__ __launch_bounds__(kBlock) void ray_grad_kernel(MarchArgs P, AdjointAr
     const uint32_t logical = xcd_logical(P);
     if (logical >= P.n_blocks) return;
     const TileLane t = tile_lane(P, logical);
-    const RaySetup R = lane_ray<true>(P, t);
+    // the ray's samples only when its line integrals are recomputed
+    const RaySetup R = G.L ? empty_ray() : lane_ray<true>(P, t);
     float g[M];
     ray_gradient<M, ST>(P, G, t, R, lds_raw, g);
     if (t.active) {
