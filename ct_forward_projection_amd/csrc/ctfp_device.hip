@@ -2176,10 +2176,9 @@ struct GatherArgs {
     int32_t cap;       // staged pixels that fit the LDS budget
 };
 
-// Each thread owns a cell of 2 x 2 x kCellZ voxels; a wavefront holds one z
-// layer of 8 x 8 cells, a workgroup four such layers.
-constexpr int kCellZ = 1;
-constexpr int kBrickX = 16, kBrickY = 16, kBrickZ = 4 * kCellZ;
+// Each thread owns a quad of 2 x 2 voxels (x, y); a wavefront holds one z
+// layer of 8 x 8 quads, a workgroup the brick's four layers.
+constexpr int kBrickX = 16, kBrickY = 16, kBrickZ = 4;
 constexpr int kGatherBlock = 256;
 // LDS of a gather workgroup: the view table and the staged pixel records
 constexpr int kGatherLds = 40 * 1024;
@@ -2195,46 +2194,42 @@ struct BrickView {
 constexpr int kGatherViews = 64;
 
 // Staged dwords per shadow pixel, one record per pixel: the support bounds
-// at the brick centre in j = k - floor(k*) and the inverse step (3 axes
-// each), k* split in two (read for every pixel of a voxel's window), then the closest-approach
-// offset and the step (3 axes each) and dl/dL_m * dx (read for the pixels
-// that touch the voxel).  An odd record length keeps the lanes' reads of
-// different records on different LDS banks.
+// at the brick centre in j = k - floor(k*) (on x and y already widened to a
+// quad: the union over offsets 0 and 1) and the inverse step (3 axes each),
+// k* split in two (read for every pixel of a quad's window), then the
+// closest-approach offset and the step (3 axes each) and dl/dL_m * dx (read
+// for the pixels that touch the quad).  An odd record length keeps the
+// lanes' reads of different records on different LDS banks.
 __host__ __device__ constexpr int gather_pixel_words(int m) { return (17 + m) | 1; }
 constexpr int kGLo = 0, kGHi = 3, kGId = 6, kGKf = 9, kGKr = 10, kGE = 11, kGD = 14, kGG = 17;
 
 // hat(r) = max(0, 1 - |r|), one clamped subtraction
 __device__ __forceinline__ float hat(float r) { return __builtin_amdgcn_fmed3f(1.0f - fabsf(r), 0.0f, 1.0f); }
 
-// The thread's 2 x 2 x NZ cell of voxels at offsets (ox + {0, 1},
-// oy + {0, 1}, oz + {0, NZ - 1}) from the brick centre: for each staged pixel
-// of rows [q0, q1] x columns [w0, w1] (records from `rec0`, row stride rs
-// records), the samples inside any of the voxels' supports (on each axis
-// the union of the shifted intervals), weighted into every voxel and then
-// times the pixel's dl/dL into acc[z][y][x].  A sample outside a voxel's
-// support has a zero hat there.  Per sample (NZ = 2): three positions, six
-// hats, four (y, z) products and eight accumulations for the eight voxels.
-template <int M, int NZ>
-__device__ __forceinline__ void cell_pixels(const float *rec0, int rs, int q0, int q1, int w0, int w1, float ox,
-                                            float oy, float oz, int K, float (&acc)[NZ][2][2][M])
+// The thread's quad of voxels at offsets (ox + {0, 1}, oy + {0, 1}, oz) from
+// the brick centre: for each staged pixel of rows [q0, q1] x columns
+// [w0, w1] (records from `rec0`, row stride rs records), the samples inside
+// any of the four voxels' supports (the staged x and y bounds hold the union
+// of the two shifted intervals, so only the quad's offset is added),
+// weighted into every voxel and then times the pixel's dl/dL into
+// acc[y][x].  A sample outside a voxel's support has a zero hat there.  Per
+// sample: three positions, five hats, two (y, z) products and four
+// accumulations for the four voxels.
+template <int M>
+__device__ __forceinline__ void quad_pixels(const float *rec0, int rs, int q0, int q1, int w0, int w1, float ox,
+                                            float oy, float oz, int K, float (&acc)[2][2][M])
 {
     constexpr int PW = gather_pixel_words(M);
-    constexpr float kZ = (float)(NZ - 1);
     for (int pv = q0; pv <= q1; pv++) {
         const float *rec = rec0 + (pv - q0) * rs * PW;
         for (int pu = w0; pu <= w1; pu++, rec += PW) {
             const float iX = rec[kGId + 0], iY = rec[kGId + 1], iZ = rec[kGId + 2];
-            // support in j = k - kf: the centre bounds shifted by o / d on each
-            // axis, the union over the cell's shifts
-            const float xl = fmaf(ox, iX, rec[kGLo + 0]), xh = fmaf(ox, iX, rec[kGHi + 0]);
-            const float yl = fmaf(oy, iY, rec[kGLo + 1]), yh = fmaf(oy, iY, rec[kGHi + 1]);
-            float zl = fmaf(oz, iZ, rec[kGLo + 2]), zh = fmaf(oz, iZ, rec[kGHi + 2]);
-            if constexpr (NZ > 1) {
-                zl = fminf(zl, fmaf(kZ, iZ, zl));
-                zh = fmaxf(zh, fmaf(kZ, iZ, zh));
-            }
-            const float jlo = fmaxf(fmaxf(fminf(xl, xl + iX), fminf(yl, yl + iY)), zl);
-            const float jhi = fminf(fminf(fmaxf(xh, xh + iX), fmaxf(yh, yh + iY)), zh);
+            // support in j = k - kf: the staged bounds shifted by o / d on
+            // each axis
+            const float jlo = fmaxf(fmaxf(fmaf(ox, iX, rec[kGLo + 0]), fmaf(oy, iY, rec[kGLo + 1])),
+                                    fmaf(oz, iZ, rec[kGLo + 2]));
+            const float jhi = fminf(fminf(fmaf(ox, iX, rec[kGHi + 0]), fmaf(oy, iY, rec[kGHi + 1])),
+                                    fmaf(oz, iZ, rec[kGHi + 2]));
             const int kf = __float_as_int(rec[kGKf]);
             // j over the open interval, and 0 <= k < K
             const int j0 = max((int)ceilf(jlo), -kf);
@@ -2243,34 +2238,24 @@ __device__ __forceinline__ void cell_pixels(const float *rec0, int rs, int q0, i
             const float kr = rec[kGKr];
             const float fX = rec[kGE + 0] - ox, fY = rec[kGE + 1] - oy, fZ = rec[kGE + 2] - oz;
             const float dX = rec[kGD + 0], dY = rec[kGD + 1], dZ = rec[kGD + 2];
-            float w[NZ][2][2];
-#pragma unroll
-            for (int c = 0; c < NZ; c++)
-                w[c][0][0] = w[c][0][1] = w[c][1][0] = w[c][1][1] = 0.0f;
-            float sj = (float)j0 - kr;
+            float w00 = 0.0f, w01 = 0.0f, w10 = 0.0f, w11 = 0.0f, sj = (float)j0 - kr;
             for (int j = j0; j <= j1; j++, sj += 1.0f) {
-                const float tz = fmaf(sj, dZ, fZ), ty = fmaf(sj, dY, fY), tx = fmaf(sj, dX, fX);
-                const float hy[2] = {hat(ty), hat(ty - 1.0f)}, hx[2] = {hat(tx), hat(tx - 1.0f)};
-#pragma unroll
-                for (int c = 0; c < NZ; c++) {
-                    const float hz = hat(tz - (float)c);
-#pragma unroll
-                    for (int b = 0; b < 2; b++) {
-                        const float hyz = hy[b] * hz;
-                        w[c][b][0] = fmaf(hyz, hx[0], w[c][b][0]);
-                        w[c][b][1] = fmaf(hyz, hx[1], w[c][b][1]);
-                    }
-                }
+                const float hz = hat(fmaf(sj, dZ, fZ));
+                const float ty = fmaf(sj, dY, fY), tx = fmaf(sj, dX, fX);
+                const float hy0 = hat(ty) * hz, hy1 = hat(ty - 1.0f) * hz;
+                const float hx0 = hat(tx), hx1 = hat(tx - 1.0f);
+                w00 = fmaf(hy0, hx0, w00);
+                w01 = fmaf(hy0, hx1, w01);
+                w10 = fmaf(hy1, hx0, w10);
+                w11 = fmaf(hy1, hx1, w11);
             }
 #pragma unroll
             for (int m = 0; m < M; m++) {
                 const float g = rec[kGG + m];
-#pragma unroll
-                for (int c = 0; c < NZ; c++)
-#pragma unroll
-                    for (int b = 0; b < 2; b++)
-#pragma unroll
-                        for (int a = 0; a < 2; a++) acc[c][b][a][m] = fmaf(w[c][b][a], g, acc[c][b][a][m]);
+                acc[0][0][m] = fmaf(w00, g, acc[0][0][m]);
+                acc[0][1][m] = fmaf(w01, g, acc[0][1][m]);
+                acc[1][0][m] = fmaf(w10, g, acc[1][0][m]);
+                acc[1][1][m] = fmaf(w11, g, acc[1][1][m]);
             }
         }
     }
@@ -2290,26 +2275,24 @@ __global__ __launch_bounds__(kGatherBlock) void gather_kernel(GatherArgs A)
     const int bx = (int)(brick % (uint32_t)A.bricks_x);
     const int by = (int)((brick / (uint32_t)A.bricks_x) % (uint32_t)A.bricks_y);
     const int bz = (int)(brick / ((uint32_t)A.bricks_x * (uint32_t)A.bricks_y));
-    // the thread's cell: voxels (x0 + {0, 1}, y0 + {0, 1}, z0 + {0 .. kCellZ - 1})
-    const int lx = 2 * (threadIdx.x & 7), ly = 2 * ((threadIdx.x >> 3) & 7), lz = kCellZ * (threadIdx.x >> 6);
-    const int x0 = bx * kBrickX + lx, y0 = by * kBrickY + ly, z0 = bz * kBrickZ + lz;
+    // the thread's quad: voxels (x0 + {0, 1}, y0 + {0, 1}, z)
+    const int lx = 2 * (threadIdx.x & 7), ly = 2 * ((threadIdx.x >> 3) & 7), lz = threadIdx.x >> 6;
+    const int x0 = bx * kBrickX + lx, y0 = by * kBrickY + ly, z = bz * kBrickZ + lz;
     const int nx = (int)P.nx, ny = (int)P.ny, nz = (int)P.nz;
-    const bool inside = x0 < nx && y0 < ny && z0 < nz; // the cell's first voxel
-    // brick centre (index space) and the cell's first voxel's offset from it
+    const bool inside = x0 < nx && y0 < ny && z < nz; // the quad's first voxel
+    // brick centre (index space) and the quad's first voxel's offset from it
     const double ca = bx * kBrickX + 0.5 * (kBrickX - 1), cb = by * kBrickY + 0.5 * (kBrickY - 1);
     const double cc = bz * kBrickZ + 0.5 * (kBrickZ - 1);
     const float ox = (float)(lx - 0.5 * (kBrickX - 1)), oy = (float)(ly - 0.5 * (kBrickY - 1));
     const float oz = (float)(lz - 0.5 * (kBrickZ - 1));
     const int64_t npix = (int64_t)P.nu * P.nv;
-    float acc[kCellZ][2][2][M]; // [z - z0][y - y0][x - x0][m]
+    float acc[2][2][M]; // [y - y0][x - x0][m]
 #pragma unroll
-    for (int c = 0; c < kCellZ; c++)
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++)
 #pragma unroll
-            for (int j = 0; j < 2; j++)
-#pragma unroll
-                for (int m = 0; m < M; m++) acc[c][i][j][m] = 0.0f;
+            for (int m = 0; m < M; m++) acc[i][j][m] = 0.0f;
     for (int vt = 0; vt < P.n_views; vt += kGatherViews) {
         const int nvt = min(kGatherViews, P.n_views - vt);
         __syncthreads(); // the previous views' table is consumed
@@ -2350,13 +2333,12 @@ __global__ __launch_bounds__(kGatherBlock) void gather_kernel(GatherArgs A)
             const BrickView bv = s_view[vw - vt];
             const int ru0 = bv.ru0, ru1 = bv.ru1, rv0 = bv.rv0, rv1 = bv.rv1;
             if (ru0 > ru1 || rv0 > rv1) continue; // block-uniform
-            // the cell's pixel window: its voxels' windows joined (inside the
-            // brick's)
-            constexpr float hz = 0.5f * (kCellZ - 1);
-            const float hu = bv.hu + 0.5f * (fabsf(bv.jua) + fabsf(bv.jub)) + hz * fabsf(bv.juc);
-            const float hv = bv.hv + 0.5f * (fabsf(bv.jva) + fabsf(bv.jvb)) + hz * fabsf(bv.jvc);
-            const float cu = bv.u0 + bv.jua * (ox + 0.5f) + bv.jub * (oy + 0.5f) + bv.juc * (oz + hz);
-            const float cv = bv.v0 + bv.jva * (ox + 0.5f) + bv.jvb * (oy + 0.5f) + bv.jvc * (oz + hz);
+            // the quad's pixel window: its four voxels' windows joined
+            // (inside the brick's)
+            const float hu = bv.hu + 0.5f * (fabsf(bv.jua) + fabsf(bv.jub));
+            const float hv = bv.hv + 0.5f * (fabsf(bv.jva) + fabsf(bv.jvb));
+            const float cu = bv.u0 + bv.jua * (ox + 0.5f) + bv.jub * (oy + 0.5f) + bv.juc * oz;
+            const float cv = bv.v0 + bv.jva * (ox + 0.5f) + bv.jvb * (oy + 0.5f) + bv.jvc * oz;
             const int pu0 = max(ru0, (int)ceilf(cu - hu)), pu1 = min(ru1, (int)floorf(cu + hu));
             const int pv0 = max(rv0, (int)ceilf(cv - hv)), pv1 = min(rv1, (int)floorf(cv + hv));
             // the brick's shadow in pieces of at most `cap` pixels: column
@@ -2394,8 +2376,11 @@ __global__ __launch_bounds__(kGatherBlock) void gather_kernel(GatherArgs A)
                             // between (+-1 - e) / d + o / d: the bounds at o = 0,
                             // stored as k - kf = s + kr
                             const float l = (-1.0f - e[a]) * id, h = (1.0f - e[a]) * id;
-                            rec[kGLo + a] = fminf(l, h) + kr;
-                            rec[kGHi + a] = fmaxf(l, h) + kr;
+                            // on x and y, the union with the interval one
+                            // voxel further (a quad's second column / row)
+                            const float wl = a < 2 ? fminf(id, 0.0f) : 0.0f, wh = a < 2 ? fmaxf(id, 0.0f) : 0.0f;
+                            rec[kGLo + a] = fminf(l, h) + kr + wl;
+                            rec[kGHi + a] = fmaxf(l, h) + kr + wh;
                             rec[kGId + a] = id;
                             rec[kGE + a] = e[a];
                             rec[kGD + a] = d[a];
@@ -2412,25 +2397,23 @@ __global__ __launch_bounds__(kGatherBlock) void gather_kernel(GatherArgs A)
                     const int q0 = max(pv0, b0), q1 = min(pv1, b1);
                     const int w0 = max(pu0, c0), w1 = min(pu1, c1);
                     if (q0 > q1 || w0 > w1) continue;
-                    cell_pixels<M, kCellZ>(s_px + ((q0 - b0) * rw + (w0 - c0)) * PW, rw, q0, q1, w0, w1, ox, oy,
-                                           oz, P.K, acc);
+                    quad_pixels<M>(s_px + ((q0 - b0) * rw + (w0 - c0)) * PW, rw, q0, q1, w0, w1, ox, oy, oz, P.K,
+                                   acc);
                 }
             }
         }
     }
     const int64_t nvox = (int64_t)nx * ny * nz;
 #pragma unroll
-    for (int c = 0; c < kCellZ; c++)
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) {
+            const int x = x0 + j, y = y0 + i;
+            if (x >= nx || y >= ny || z >= nz) continue;
+            const int64_t q = ((int64_t)z * ny + y) * nx + x;
 #pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const int x = x0 + j, y = y0 + i, z = z0 + c;
-                if (x >= nx || y >= ny || z >= nz) continue;
-                const int64_t q = ((int64_t)z * ny + y) * nx + x;
-#pragma unroll
-                for (int m = 0; m < M; m++) A.grad[(int64_t)(A.mat_off + m) * nvox + q] += acc[c][i][j][m];
-            }
+            for (int m = 0; m < M; m++) A.grad[(int64_t)(A.mat_off + m) * nvox + q] += acc[i][j][m];
+        }
 }
 
 template <int M, int ST>
