@@ -2196,12 +2196,13 @@ constexpr int kGatherViews = 64;
 // Staged dwords per shadow pixel, one record per pixel: the support bounds
 // at the brick centre in j = k - floor(k*) (on x and y already widened to a
 // quad: the union over offsets 0 and 1) and the inverse step (3 axes each),
-// k* split in two (read for every pixel of a quad's window), then the
+// the lattice's range in j (-floor(k*) and K - 1 - floor(k*), as ints; read
+// for every pixel of a quad's window), then k* - floor(k*), the
 // closest-approach offset and the step (3 axes each) and dl/dL_m * dx (read
 // for the pixels that touch the quad).  An odd record length keeps the
 // lanes' reads of different records on different LDS banks.
-__host__ __device__ constexpr int gather_pixel_words(int m) { return (17 + m) | 1; }
-constexpr int kGLo = 0, kGHi = 3, kGId = 6, kGKf = 9, kGKr = 10, kGE = 11, kGD = 14, kGG = 17;
+__host__ __device__ constexpr int gather_pixel_words(int m) { return (18 + m) | 1; }
+constexpr int kGLo = 0, kGHi = 3, kGId = 6, kGJ0 = 9, kGJ1 = 10, kGKr = 11, kGE = 12, kGD = 15, kGG = 18;
 
 // hat(r) = max(0, 1 - |r|), one clamped subtraction
 __device__ __forceinline__ float hat(float r) { return __builtin_amdgcn_fmed3f(1.0f - fabsf(r), 0.0f, 1.0f); }
@@ -2217,7 +2218,7 @@ __device__ __forceinline__ float hat(float r) { return __builtin_amdgcn_fmed3f(1
 // accumulations for the four voxels.
 template <int M>
 __device__ __forceinline__ void quad_pixels(const float *rec0, int rs, int q0, int q1, int w0, int w1, float ox,
-                                            float oy, float oz, int K, float (&acc)[2][2][M])
+                                            float oy, float oz, float (&acc)[2][2][M])
 {
     constexpr int PW = gather_pixel_words(M);
     for (int pv = q0; pv <= q1; pv++) {
@@ -2230,10 +2231,9 @@ __device__ __forceinline__ void quad_pixels(const float *rec0, int rs, int q0, i
                                     fmaf(oz, iZ, rec[kGLo + 2]));
             const float jhi = fminf(fminf(fmaf(ox, iX, rec[kGHi + 0]), fmaf(oy, iY, rec[kGHi + 1])),
                                     fmaf(oz, iZ, rec[kGHi + 2]));
-            const int kf = __float_as_int(rec[kGKf]);
             // j over the open interval, and 0 <= k < K
-            const int j0 = max((int)ceilf(jlo), -kf);
-            const int j1 = min((int)floorf(jhi), K - 1 - kf);
+            const int j0 = max((int)ceilf(jlo), __float_as_int(rec[kGJ0]));
+            const int j1 = min((int)floorf(jhi), __float_as_int(rec[kGJ1]));
             if (j0 > j1) continue;
             const float kr = rec[kGKr];
             const float fX = rec[kGE + 0] - ox, fY = rec[kGE + 1] - oy, fZ = rec[kGE + 2] - oz;
@@ -2385,7 +2385,8 @@ __global__ __launch_bounds__(kGatherBlock) void gather_kernel(GatherArgs A)
                             rec[kGE + a] = e[a];
                             rec[kGD + a] = d[a];
                         }
-                        rec[kGKf] = __int_as_float((int)kf);
+                        rec[kGJ0] = __int_as_float(-(int)kf);
+                        rec[kGJ1] = __int_as_float(P.K - 1 - (int)kf);
                         rec[kGKr] = kr;
                         const int64_t pix = (int64_t)pv * P.nu + pu;
 #pragma unroll
@@ -2397,8 +2398,7 @@ __global__ __launch_bounds__(kGatherBlock) void gather_kernel(GatherArgs A)
                     const int q0 = max(pv0, b0), q1 = min(pv1, b1);
                     const int w0 = max(pu0, c0), w1 = min(pu1, c1);
                     if (q0 > q1 || w0 > w1) continue;
-                    quad_pixels<M>(s_px + ((q0 - b0) * rw + (w0 - c0)) * PW, rw, q0, q1, w0, w1, ox, oy, oz, P.K,
-                                   acc);
+                    quad_pixels<M>(s_px + ((q0 - b0) * rw + (w0 - c0)) * PW, rw, q0, q1, w0, w1, ox, oy, oz, acc);
                 }
             }
         }
