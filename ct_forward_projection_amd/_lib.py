@@ -36,7 +36,7 @@ EXPORTED = (
     "ctfp_get_scan", "ctfp_get_device", "ctfp_get_volumes", "ctfp_device_bytes", "ctfp_autotune",
     "ctfp_batch_create", "ctfp_batch_run", "ctfp_batch_samples", "ctfp_batch_destroy",
     "ctfp_count_samples", "ctfp_set_focal_spot", "ctfp_focal_spot_offsets",
-    "ctfp_backward_noalloc", "ctfp_grid_unit_heights", "ctfp_count_marched",
+    "ctfp_backward_noalloc", "ctfp_grid_unit_heights", "ctfp_count_marched", "ctfp_debug_fail_allocations",
 )
 
 CTFP_MARCH_GLOBAL = 0
@@ -158,6 +158,7 @@ def _declare(lib):
         "ctfp_autotune": (i32, [vp, i32, P(Tuning)]),
         "ctfp_count_samples": (i32, [vp, i64, i64, P(i64)]),
         "ctfp_count_marched": (i32, [vp, i64, i64, P(i64)]),
+        "ctfp_debug_fail_allocations": (i32, [C.c_longlong]),
         "ctfp_set_focal_spot": (i32, [vp, _FP, _FP]),
         "ctfp_focal_spot_offsets": (i32, [i64, C.c_float, C.c_uint32, _FP, _FP]),
         "ctfp_batch_create": (i32, [P(BatchItem), i32, P(vp)]),

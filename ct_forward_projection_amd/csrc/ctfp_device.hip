@@ -33,6 +33,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -150,6 +151,27 @@ template <typename T> void dfree(T *&p)
     if (p) (void)hipFree(p);
     p = nullptr;
 }
+
+// Device allocations that still succeed before every allocation fails as out
+// of memory (ctfp_debug_fail_allocations; -1: no failures).  Process-wide.
+std::atomic<long long> g_alloc_budget{-1};
+
+// hipMalloc through the failure hook.
+template <typename T> hipError_t dev_malloc(T **p, size_t bytes)
+{
+    long long b = g_alloc_budget.load();
+    while (b >= 0) {
+        if (b == 0) {
+            *p = nullptr;
+            return hipErrorOutOfMemory;
+        }
+        if (g_alloc_budget.compare_exchange_weak(b, b - 1)) break;
+    }
+    return hipMalloc(p, bytes);
+}
+
+// The status code of a failed HIP call.
+int hip_status(hipError_t e) { return e == hipErrorOutOfMemory ? CTFP_ERR_NOMEM : CTFP_ERR_DEVICE; }
 
 // ------------------------------------------------------------- kernels ---
 
@@ -2646,16 +2668,16 @@ static int alloc_volume_buffers(ctfp_context *ctx, size_t bytes, int64_t obx, in
                                 int n_groups)
 {
     if (ctx->vol && (ctx->vol_bytes != bytes)) dfree(ctx->vol);
-    if (!ctx->vol) HIP_TRY(hipMalloc(&ctx->vol, bytes));
+    if (!ctx->vol) HIP_TRY(dev_malloc(&ctx->vol, bytes));
     ctx->vol_bytes = bytes;
     if (ctx->occ && (ctx->ob_x != obx || ctx->ob_y != oby || ctx->ob_z != obz || ctx->n_groups != n_groups)) {
         dfree(ctx->occ);
         dfree(ctx->obox);
         dfree(ctx->odist);
     }
-    if (!ctx->occ) HIP_TRY(hipMalloc(&ctx->occ, (size_t)(obx * oby * obz) * n_groups));
-    if (!ctx->obox) HIP_TRY(hipMalloc(&ctx->obox, (size_t)n_groups * 6 * sizeof(int32_t)));
-    if (!ctx->odist) HIP_TRY(hipMalloc(&ctx->odist, (size_t)(obx * oby * obz) * (n_groups + 1)));
+    if (!ctx->occ) HIP_TRY(dev_malloc(&ctx->occ, (size_t)(obx * oby * obz) * n_groups));
+    if (!ctx->obox) HIP_TRY(dev_malloc(&ctx->obox, (size_t)n_groups * 6 * sizeof(int32_t)));
+    if (!ctx->odist) HIP_TRY(dev_malloc(&ctx->odist, (size_t)(obx * oby * obz) * (n_groups + 1)));
     return CTFP_OK;
 }
 
@@ -2830,12 +2852,12 @@ int ctfp_create(const ctfp_scan_params *scan, int device, ctfp_context **out)
         double th = (double)ctfp::view_angle(scan, &sys, i);
         cs[(size_t)i] = make_double2(cos(th), sin(th));
     }
-    hipError_t e = hipMalloc(&ctx->view_cs, cs.size() * sizeof(double2));
+    hipError_t e = dev_malloc(&ctx->view_cs, cs.size() * sizeof(double2));
     if (e == hipSuccess)
         e = hipMemcpy(ctx->view_cs, cs.data(), cs.size() * sizeof(double2), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         ctfp_destroy(ctx);
-        return set_error(CTFP_ERR_DEVICE, "view table: %s", hipGetErrorString(e));
+        return set_error(hip_status(e), "view table: %s", hipGetErrorString(e));
     }
     *out = ctx;
     return CTFP_OK;
@@ -2882,7 +2904,7 @@ static int write_layout_impl(ctfp_context *ctx, const float *const *chan, int n_
 {
     int rc = alloc_volume(ctx, n_mat);
     if (rc) return rc;
-    if (!async && !ctx->d_flag) HIP_TRY(hipMalloc(&ctx->d_flag, sizeof(int)));
+    if (!async && !ctx->d_flag) HIP_TRY(dev_malloc(&ctx->d_flag, sizeof(int)));
     int *bad = async ? nullptr : ctx->d_flag;
     if (bad) HIP_TRY(hipMemsetAsync(bad, 0, sizeof(int), stream));
     const ctfp_scan_params &s = ctx->scan;
@@ -2970,13 +2992,13 @@ int ctfp_set_volumes(ctfp_context *ctx, int n_mat, const float *const *volumes,
     if (volumes_on_device) {
         for (int m = 0; m < n_mat; m++) chan[m] = volumes[m];
     } else {
-        HIP_TRY(hipMalloc(&stage, (size_t)nvox * n_mat * sizeof(float)));
+        HIP_TRY(dev_malloc(&stage, (size_t)nvox * n_mat * sizeof(float)));
         for (int m = 0; m < n_mat; m++) {
             hipError_t e = hipMemcpy(stage + m * nvox, volumes[m], (size_t)nvox * sizeof(float),
                                      hipMemcpyHostToDevice);
             if (e != hipSuccess) {
                 (void)hipFree(stage);
-                return set_error(CTFP_ERR_DEVICE, "volume upload: %s", hipGetErrorString(e));
+                return set_error(hip_status(e), "volume upload: %s", hipGetErrorString(e));
             }
             chan[m] = stage + m * nvox;
         }
@@ -3018,7 +3040,7 @@ int ctfp_set_focal_spot(ctfp_context *ctx, const float *offset_u, const float *o
             return set_error(CTFP_ERR_INVALID, "focal spot: offset of view %lld is not finite", (long long)i);
         f[(size_t)i] = make_double2((double)offset_u[i], (double)offset_v[i]);
     }
-    if (!ctx->focal) HIP_TRY(hipMalloc(&ctx->focal, (size_t)n * sizeof(double2)));
+    if (!ctx->focal) HIP_TRY(dev_malloc(&ctx->focal, (size_t)n * sizeof(double2)));
     HIP_TRY(hipMemcpy(ctx->focal, f.data(), (size_t)n * sizeof(double2), hipMemcpyHostToDevice));
     return CTFP_OK;
 }
@@ -3079,11 +3101,11 @@ int ctfp_set_labels(ctfp_context *ctx, const uint8_t *labels, int labels_on_devi
     const uint8_t *dl = labels;
     hipError_t e = hipSuccess;
     if (!labels_on_device) {
-        e = hipMalloc(&stage, (size_t)nvox);
+        e = dev_malloc(&stage, (size_t)nvox);
         if (e == hipSuccess) e = hipMemcpy(stage, labels, (size_t)nvox, hipMemcpyHostToDevice);
         dl = stage;
     }
-    if (e == hipSuccess) e = hipMalloc(&raw, (size_t)nvox * channels * sizeof(float));
+    if (e == hipSuccess) e = dev_malloc(&raw, (size_t)nvox * channels * sizeof(float));
     if (e == hipSuccess) {
         Densities rho{};
         for (int m = 0; m < channels; m++) rho.d[m] = density[m];
@@ -3099,14 +3121,13 @@ int ctfp_set_labels(ctfp_context *ctx, const uint8_t *labels, int labels_on_devi
     if (stage) (void)hipFree(stage);
     if (raw) (void)hipFree(raw);
     if (e != hipSuccess)
-        return set_error(e == hipErrorOutOfMemory ? CTFP_ERR_NOMEM : CTFP_ERR_DEVICE, "label channels: %s",
+        return set_error(hip_status(e), "label channels: %s",
                          hipGetErrorString(e));
     return rc;
 }
 
-int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
+static int set_spectral_impl(ctfp_context *ctx, const ctfp_spectral *sp)
 {
-    Range range_("ctfp_set_spectral");
     if (check_ctx(ctx)) return CTFP_ERR_INVALID;
     if (!sp || !sp->spectrum || !sp->atten)
         return set_error(CTFP_ERR_INVALID, "spectral: spectrum and atten are required");
@@ -3176,8 +3197,8 @@ int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
     dfree(ctx->A);
     dfree(ctx->Aq);
     dfree(ctx->W);
-    if (!ctx->I0) HIP_TRY(hipMalloc(&ctx->I0, (size_t)npix * sizeof(float)));
-    HIP_TRY(hipMalloc(&ctx->A, A.size() * sizeof(float)));
+    if (!ctx->I0) HIP_TRY(dev_malloc(&ctx->I0, (size_t)npix * sizeof(float)));
+    HIP_TRY(dev_malloc(&ctx->A, A.size() * sizeof(float)));
     HIP_TRY(hipMemcpy(ctx->A, A.data(), A.size() * sizeof(float), hipMemcpyHostToDevice));
     {
         // MFMA operand: row e, k = 3m + (0, 1, 2) = (hi, hi, lo) of 2^s A[e][m],
@@ -3202,12 +3223,12 @@ int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
                 }
             }
         }
-        HIP_TRY(hipMalloc(&ctx->Aq, Aq.size() * sizeof(_Float16)));
+        HIP_TRY(dev_malloc(&ctx->Aq, Aq.size() * sizeof(_Float16)));
         HIP_TRY(hipMemcpy(ctx->Aq, Aq.data(), Aq.size() * sizeof(_Float16), hipMemcpyHostToDevice));
     }
     // kWeightPadBins zero rows follow the last active bin (the matrix-core
     // spectrum reads bins in groups that may run past it)
-    HIP_TRY(hipMalloc(&ctx->W, (size_t)(std::max(na, 1) + kWeightPadBins) * rows * nu * sizeof(float)));
+    HIP_TRY(dev_malloc(&ctx->W, (size_t)(std::max(na, 1) + kWeightPadBins) * rows * nu * sizeof(float)));
     HIP_TRY(hipMemset(ctx->W + (size_t)std::max(na, 1) * rows * nu, 0,
                       (size_t)kWeightPadBins * rows * nu * sizeof(float)));
     // stage the per-bin tables of the active bins
@@ -3216,29 +3237,29 @@ int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
     auto cleanup = [&]() {
         dfree(dS); dfree(dT); dfree(dEta); dfree(dMu); dfree(dThick);
     };
-    hipError_t e = hipMalloc(&dS, S.size() * sizeof(float));
+    hipError_t e = dev_malloc(&dS, S.size() * sizeof(float));
     if (e == hipSuccess) e = hipMemcpy(dS, S.data(), S.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess && sp->grid && na > 0) {
         T.resize((size_t)na * nu);
         for (int i = 0; i < na; i++)
             std::memcpy(&T[(size_t)i * nu], sp->grid + (size_t)act[i] * nu, nu * sizeof(float));
-        e = hipMalloc(&dT, T.size() * sizeof(float));
+        e = dev_malloc(&dT, T.size() * sizeof(float));
         if (e == hipSuccess) e = hipMemcpy(dT, T.data(), T.size() * sizeof(float), hipMemcpyHostToDevice);
     }
     // rows * nu response entries per bin: every pixel, or row 0 when the
     // response does not vary along v
     const int64_t per = rows * nu;
     if (e == hipSuccess && sp->det && na > 0) {
-        e = hipMalloc(&dEta, (size_t)na * per * sizeof(float));
+        e = dev_malloc(&dEta, (size_t)na * per * sizeof(float));
         for (int i = 0; i < na && e == hipSuccess; i++)
             e = hipMemcpy(dEta + (size_t)i * per, sp->det + (size_t)act[i] * npix, (size_t)per * sizeof(float),
                           hipMemcpyHostToDevice);
     } else if (e == hipSuccess && sp->scint_mu && na > 0) {
         mu.resize(na);
         for (int i = 0; i < na; i++) mu[i] = sp->scint_mu[act[i]];
-        e = hipMalloc(&dMu, mu.size() * sizeof(float));
+        e = dev_malloc(&dMu, mu.size() * sizeof(float));
         if (e == hipSuccess) e = hipMemcpy(dMu, mu.data(), mu.size() * sizeof(float), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMalloc(&dThick, (size_t)per * sizeof(float));
+        if (e == hipSuccess) e = dev_malloc(&dThick, (size_t)per * sizeof(float));
         if (e == hipSuccess)
             e = hipMemcpy(dThick, sp->scint_thickness, (size_t)per * sizeof(float), hipMemcpyHostToDevice);
     }
@@ -3265,7 +3286,33 @@ int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
     // the tables are built on the null stream; wait for that stream only
     if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     cleanup();
-    if (e != hipSuccess) return set_error(CTFP_ERR_DEVICE, "spectral set-up: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return set_error(hip_status(e), "spectral set-up: %s", hipGetErrorString(e));
+    return CTFP_OK;
+}
+
+// A failure after the old tables were released leaves the context with no
+// spectral tables (forwards then report CTFP_ERR_STATE), not with tables that
+// disagree with its bin count; invalid arguments leave the old tables.
+int ctfp_set_spectral(ctfp_context *ctx, const ctfp_spectral *sp)
+{
+    Range range_("ctfp_set_spectral");
+    const uint64_t gen = ctx ? ctx->generation : 0;
+    const int rc = set_spectral_impl(ctx, sp);
+    if (rc != CTFP_OK && ctx && ctx->generation != gen) {
+        DeviceGuard g(ctx->device);
+        dfree(ctx->A);
+        dfree(ctx->Aq);
+        dfree(ctx->W);
+        dfree(ctx->I0);
+        ctx->n_act = 0;
+        ctx->spec_n_mat = 0;
+    }
+    return rc;
+}
+
+int ctfp_debug_fail_allocations(long long after)
+{
+    g_alloc_budget.store(after < 0 ? -1 : after);
     return CTFP_OK;
 }
 
@@ -3348,7 +3395,7 @@ static int ensure_l_ws(ctfp_context *ctx, int64_t views)
     if (ctx->l_ws_bytes >= need) return CTFP_OK;
     dfree(ctx->l_ws);
     ctx->l_ws_bytes = 0;
-    HIP_TRY(hipMalloc(&ctx->l_ws, need));
+    HIP_TRY(dev_malloc(&ctx->l_ws, need));
     ctx->l_ws_bytes = need;
     return CTFP_OK;
 }
@@ -3537,7 +3584,7 @@ static int backward_impl(ctfp_context *ctx, int64_t view_begin, int64_t view_cou
         if (ctx->adj_ws_bytes < need) {
             dfree(ctx->adj_ws);
             ctx->adj_ws_bytes = 0;
-            HIP_TRY(hipMalloc(&ctx->adj_ws, need));
+            HIP_TRY(dev_malloc(&ctx->adj_ws, need));
             ctx->adj_ws_bytes = need;
         }
         ws = ctx->adj_ws;
@@ -3656,8 +3703,9 @@ int ctfp_forward_host(ctfp_context *ctx, float *I, float *IAbsorb, float *I0)
     if (ctx->staged_views < chunk) {
         dfree(ctx->dI);
         dfree(ctx->dIA);
-        HIP_TRY(hipMalloc(&ctx->dI, (size_t)chunk * npix * sizeof(float)));
-        HIP_TRY(hipMalloc(&ctx->dIA, (size_t)chunk * npix * sizeof(float)));
+        ctx->staged_views = 0;
+        HIP_TRY(dev_malloc(&ctx->dI, (size_t)chunk * npix * sizeof(float)));
+        HIP_TRY(dev_malloc(&ctx->dIA, (size_t)chunk * npix * sizeof(float)));
         ctx->staged_views = chunk;
     }
     for (int64_t v0 = 0; v0 < ctx->scan.proj_num; v0 += chunk) {
@@ -3704,7 +3752,7 @@ int ctfp_count_samples(ctfp_context *ctx, int64_t view_begin, int64_t view_count
     if (view_count > 0 && (rc = fill_march_args(ctx, view_begin + view_count - 1, 1, P))) return rc;
     DeviceGuard g(ctx->device);
     unsigned long long *d = nullptr;
-    HIP_TRY(hipMalloc(&d, sizeof(unsigned long long)));
+    HIP_TRY(dev_malloc(&d, sizeof(unsigned long long)));
     unsigned long long h = 0;
     hipError_t e = hipMemcpy(d, &h, sizeof h, hipMemcpyHostToDevice);
     const int64_t per = views_per_launch(ctx);
@@ -3718,7 +3766,7 @@ int ctfp_count_samples(ctfp_context *ctx, int64_t view_begin, int64_t view_count
     }
     if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost);
     (void)hipFree(d);
-    if (e != hipSuccess) return set_error(CTFP_ERR_DEVICE, "count: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return set_error(hip_status(e), "count: %s", hipGetErrorString(e));
     *marched = (int64_t)h;
     return CTFP_OK;
 }
@@ -3732,7 +3780,7 @@ int ctfp_count_marched(ctfp_context *ctx, int64_t view_begin, int64_t view_count
                          (long long)(view_begin + view_count), (long long)ctx->scan.proj_num);
     DeviceGuard g(ctx->device);
     unsigned long long *d = nullptr;
-    HIP_TRY(hipMalloc(&d, sizeof(unsigned long long)));
+    HIP_TRY(dev_malloc(&d, sizeof(unsigned long long)));
     unsigned long long h = 0;
     hipError_t e = hipMemcpy(d, &h, sizeof h, hipMemcpyHostToDevice);
     const int64_t per = views_per_launch(ctx);
@@ -3754,7 +3802,7 @@ int ctfp_count_marched(ctfp_context *ctx, int64_t view_begin, int64_t view_count
     }
     if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost);
     (void)hipFree(d);
-    if (e != hipSuccess) return set_error(CTFP_ERR_DEVICE, "count: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return set_error(hip_status(e), "count: %s", hipGetErrorString(e));
     *marched = (int64_t)h;
     return CTFP_OK;
 }
@@ -3782,8 +3830,8 @@ int ctfp_autotune(ctfp_context *ctx, int max_lds_bytes, ctfp_tuning *best)
     for (int gi = 0; gi < kGroups; gi++) starts[gi] = std::min<int64_t>(gi * s.proj_num / 8, s.proj_num - per_group);
     DeviceGuard g(ctx->device);
     float *I = nullptr, *IA = nullptr;
-    HIP_TRY(hipMalloc(&I, (size_t)per_group * npix * sizeof(float)));
-    hipError_t e = hipMalloc(&IA, (size_t)per_group * npix * sizeof(float));
+    HIP_TRY(dev_malloc(&I, (size_t)per_group * npix * sizeof(float)));
+    hipError_t e = dev_malloc(&IA, (size_t)per_group * npix * sizeof(float));
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&ev0);
     if (e == hipSuccess) e = hipEventCreate(&ev1);
@@ -3797,7 +3845,7 @@ int ctfp_autotune(ctfp_context *ctx, int max_lds_bytes, ctfp_tuning *best)
     };
     if (e != hipSuccess) {
         cleanup();
-        return set_error(CTFP_ERR_DEVICE, "autotune set-up: %s", hipGetErrorString(e));
+        return set_error(hip_status(e), "autotune set-up: %s", hipGetErrorString(e));
     }
     // time one candidate (ms over all groups)
     auto time_it = [&](const ctfp_tuning &t, float &ms) -> int {
@@ -3985,8 +4033,8 @@ int ctfp_batch_create(const ctfp_batch_item *items, int n_items, ctfp_batch **ou
     };
     b->grouped = grouped;
     if (l_ws_bytes) {
-        e = hipMalloc(&b->l_ws, l_ws_bytes);
-        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? CTFP_ERR_NOMEM : CTFP_ERR_DEVICE, hipGetErrorString(e));
+        e = dev_malloc(&b->l_ws, l_ws_bytes);
+        if (e != hipSuccess) return fail(hip_status(e), hipGetErrorString(e));
         b->l_ws_bytes = l_ws_bytes;
     }
     for (int budget : budgets) {
@@ -4013,17 +4061,17 @@ int ctfp_batch_create(const ctfp_batch_item *items, int n_items, ctfp_batch **ou
         if (e != hipSuccess) return fail(CTFP_ERR_DEVICE, hipGetErrorString(e));
         if (per_cu < 1) return fail(CTFP_ERR_UNSUPPORTED, "kernel does not fit a CU");
         grp.grid = std::max(8u, (unsigned)(per_cu * prop.multiProcessorCount) / 8 * 8);
-        e = hipMalloc(&grp.d_items, gi.size() * sizeof(BatchItem));
+        e = dev_malloc(&grp.d_items, gi.size() * sizeof(BatchItem));
         if (e == hipSuccess) e = hipMemcpy(grp.d_items, gi.data(), gi.size() * sizeof(BatchItem), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMalloc(&grp.d_base, base.size() * sizeof(uint32_t));
+        if (e == hipSuccess) e = dev_malloc(&grp.d_base, base.size() * sizeof(uint32_t));
         if (e == hipSuccess)
             e = hipMemcpy(grp.d_base, base.data(), base.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
         b->groups.push_back(grp);
-        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? CTFP_ERR_NOMEM : CTFP_ERR_DEVICE,
+        if (e != hipSuccess) return fail(hip_status(e),
                                          hipGetErrorString(e));
     }
     const int ng = (int)b->groups.size();
-    if (ng > 0) e = hipMalloc(&b->d_taken, (size_t)ng * kXcds * sizeof(uint32_t));
+    if (ng > 0) e = dev_malloc(&b->d_taken, (size_t)ng * kXcds * sizeof(uint32_t));
     // the launch sequence (zero every queue counter, one persistent kernel
     // per group, then the items of more than four materials) is captured
     // once on a private stream and replayed
@@ -4057,7 +4105,7 @@ int ctfp_batch_create(const ctfp_batch_item *items, int n_items, ctfp_batch **ou
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
     if (e != hipSuccess)
-        return fail(e == hipErrorOutOfMemory ? CTFP_ERR_NOMEM : CTFP_ERR_DEVICE, hipGetErrorString(e));
+        return fail(hip_status(e), hipGetErrorString(e));
     *out = b;
     return CTFP_OK;
 }

@@ -277,6 +277,15 @@ int ctfp_count_samples(ctfp_context *ctx, int64_t view_begin, int64_t view_count
  * of more than four materials is marched on its own).  Synchronous. */
 int ctfp_count_marched(ctfp_context *ctx, int64_t view_begin, int64_t view_count, int64_t *marched);
 
+/* Test hook; no counterpart in the reference.  After `after` more device
+ * allocations by this library (by any context, any thread), every device
+ * allocation fails as out of memory (CTFP_ERR_NOMEM) until the hook is reset
+ * with a negative value.  For tests of the error paths: a failed call leaves
+ * its context usable (an upload or table update that failed part way leaves
+ * no volume or no spectral tables, and calls that need them report
+ * CTFP_ERR_STATE). */
+int ctfp_debug_fail_allocations(long long after);
+
 
 /* Times candidate launch options (ray-tile shape, samples per staged chunk,
  * LDS budget) on four groups of the scan's views at 0, 1/8, 1/4 and 3/8 of

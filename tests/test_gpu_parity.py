@@ -403,6 +403,52 @@ def test_errors_without_inputs(require_gpu):
     proj.close()
 
 
+def test_allocation_failures_leave_contexts_usable(require_gpu):
+    """Every device allocation of a create / upload / spectral / forward /
+    backward / host-forward sequence made to fail in turn (the library's
+    allocation-failure hook): each call either succeeds or reports an error
+    (CTFP_ERR_NOMEM, or CTFP_ERR_STATE for inputs a failed call dropped), and
+    once allocations succeed again the same context reproduces the outputs of
+    a context that never saw a failure, bit for bit."""
+    import torch
+    from ct_forward_projection_amd import _lib as L
+    s, vols = small_three_material(n=32, views=4, det=(48, 40), d_size=0.9)
+    kw = P.spectral_inputs(s, *s.det)
+
+    def run(proj):
+        proj.set_volumes(vols, storage_dtype=P.CTFP_F16)
+        proj.set_spectral(**kw)
+        I, IA, Lm = proj.forward(want_L=True)
+        g = proj.backward(gI=torch.ones_like(I), L_in=Lm)
+        Ih, IAh = proj.forward_host()[:2]
+        torch.cuda.synchronize()
+        return I, IA, Lm, g, torch.from_numpy(np.ascontiguousarray(Ih))
+
+    with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
+        ref = run(proj)
+    failed_calls = 0
+    for n in range(40):
+        print(f"allocation failure after {n} allocations", flush=True)
+        proj = None
+        L.lib().ctfp_debug_fail_allocations(n)
+        try:
+            proj = P.Projector(P.scan_params(**s.scan_kwargs()))
+            run(proj)
+        except P.CTFPError as err:
+            assert err.code in (L.CTFP_ERR_NOMEM, L.CTFP_ERR_STATE), err
+            failed_calls += 1
+        finally:
+            L.lib().ctfp_debug_fail_allocations(-1)
+        if proj is None:
+            continue
+        got = run(proj)
+        proj.close()
+        for a, b in zip(got, ref):
+            assert torch.equal(a, b), n
+    # the early budgets fail somewhere in the sequence, the late ones not
+    assert 10 <= failed_calls < 40, failed_calls
+
+
 # ------------------------------------------------------- full-size config 2
 
 @pytest.fixture(scope="module")
