@@ -249,3 +249,53 @@ def test_adjoint_with_focal_spot(require_gpu, mode):
     scale = O.backproject_f64(g, p, np.abs(gL), focal=focal)
     err = np.abs(grad - ref) / np.maximum(scale, 1e-3 * scale.max())
     assert err.max() < 1e-4, err.max()
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_adjoint_random_geometries(require_gpu, case):
+    """Seeded random scans (volume shape and pitch per axis, SOD and SDD,
+    rotation direction, detector size and pitch, 1-3 materials of balls and
+    boxes): the gather adjoint, whose bricks stage per-pixel records and
+    walk 2 x 2 voxel quads, against the FP64 CPU adjoint."""
+    import torch
+    rng = np.random.default_rng(2000 + case)
+    nx, ny, nz = (int(v) for v in rng.integers(10, 40, 3))
+    pitch = rng.uniform(0.3, 1.2, 3)
+    n_mat = int(rng.integers(1, 4))
+    zz, yy, xx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    vols = [np.zeros((nz, ny, nx), np.float32) for _ in range(n_mat)]
+    for k in range(int(rng.integers(2, 5))):
+        m = int(rng.integers(0, n_mat))
+        c = rng.uniform(0, 1, 3) * (nx, ny, nz)
+        r = rng.uniform(1.5, 0.4 * min(nx, ny, nz) + 1.5)
+        vols[m][(xx - c[0]) ** 2 + (yy - c[1]) ** 2 + (zz - c[2]) ** 2 <= r * r] = rng.uniform(0.2, 2.0)
+    s = S.config2()
+    s.materials = (S.WATER, S.BONE, S.IODINE)[:n_mat]
+    s.n_vox = (nx, ny, nz)
+    s.length_mm = tuple(float(n * p_) for n, p_ in zip((nx, ny, nz), pitch))
+    s.sod = float(rng.uniform(300.0, 1000.0))
+    s.sdd = float(s.sod * rng.uniform(1.2, 2.5))
+    s.rot = int(rng.choice([-1, 1]))
+    s.det = (int(rng.integers(16, 64)), int(rng.integers(12, 48)))
+    s.d_size = float(rng.uniform(0.4, 1.5)) * s.sdd / s.sod * float(pitch.min())
+    s.views = 4
+    s.scint_thickness_err = 0.05
+    print(f"adjoint random geometry {case}: {nx}x{ny}x{nz} pitch {np.round(pitch, 2)} M {n_mat} "
+          f"SOD {s.sod:.0f} SDD {s.sdd:.0f} det {s.det} {s.d_size:.2f} mm rot {s.rot}", flush=True)
+    kw = P.spectral_inputs(s, *s.det)
+    gI = rng.normal(size=(s.views, s.det[1], s.det[0])).astype(np.float32)
+    with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
+        proj.set_volumes(vols)
+        proj.set_spectral(**kw)
+        grad = proj.backward(torch.from_numpy(gI).to(torch.device("cuda")))
+        torch.cuda.synchronize()
+        grad = grad.cpu().numpy()
+    g = O.geom_from_setup(s)
+    p = O.paras(g)
+    L64 = O.march_f64(g, p, vols)
+    gL = O.spectrum_grad_f64(L64, kw["atten"], kw["spectrum"], kw["i0_val"], gI, None, kw["grid"],
+                             det_table(kw))
+    ref = O.backproject_f64(g, p, gL)
+    scale = O.backproject_f64(g, p, np.abs(gL))
+    err = np.abs(grad - ref) / np.maximum(scale, 1e-3 * scale.max())
+    assert err.max() < 1e-4, err.max()

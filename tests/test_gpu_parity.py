@@ -204,6 +204,61 @@ def test_anisotropic_volume_and_material_counts(require_gpu, n_mat, dtype):
             assert torch.equal(a, b_)
 
 
+@pytest.mark.parametrize("case", range(10))
+def test_random_geometries(require_gpu, case):
+    """Seeded random scans: volume shape and pitch per axis, magnification
+    (SOD, SDD), rotation direction, detector size and pitch, 1-3 materials
+    holding a few balls and boxes anywhere in the volume (touching its faces
+    too, so rays clip the occupied bricks and step their window ends through
+    wide empty space), a random launch tuning.  The LDS march with empty
+    space skipped equals the plain global march (no skipping) bit for bit,
+    and FP32 storage meets the FP64 evaluation to TOL."""
+    import torch
+    rng = np.random.default_rng(1000 + case)
+    nx, ny, nz = (int(v) for v in rng.integers(12, 48, 3))
+    pitch = rng.uniform(0.3, 1.2, 3)
+    n_mat = int(rng.integers(1, 4))
+    zz, yy, xx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    vols = [np.zeros((nz, ny, nx), np.float32) for _ in range(n_mat)]
+    for k in range(int(rng.integers(2, 6))):
+        m = int(rng.integers(0, n_mat))
+        c = rng.uniform(0, 1, 3) * (nx, ny, nz)
+        r = rng.uniform(1.0, 0.35 * min(nx, ny, nz))
+        if k % 2:
+            inside = (np.abs(xx - c[0]) <= r) & (np.abs(yy - c[1]) <= 0.6 * r) & (np.abs(zz - c[2]) <= r)
+        else:
+            inside = (xx - c[0]) ** 2 + (yy - c[1]) ** 2 + (zz - c[2]) ** 2 <= r * r
+        vols[m][inside] = rng.uniform(0.2, 2.0)
+    s = S.config2()
+    s.materials = (S.WATER, S.BONE, S.IODINE)[:n_mat]
+    s.n_vox = (nx, ny, nz)
+    s.length_mm = tuple(float(n * p_) for n, p_ in zip((nx, ny, nz), pitch))
+    s.sod = float(rng.uniform(300.0, 1000.0))
+    s.sdd = float(s.sod * rng.uniform(1.2, 2.5))
+    s.rot = int(rng.choice([-1, 1]))
+    s.det = (int(rng.integers(16, 72)), int(rng.integers(12, 56)))
+    s.d_size = float(rng.uniform(0.3, 1.5)) * s.sdd / s.sod * float(pitch.min())
+    s.views = 5
+    kw = P.spectral_inputs(s, *s.det)
+    tuning = dict(chunk=int(rng.choice([16, 32, 48])), wave_u=int(rng.choice([4, 8, 16, 32])),
+                  block_u=int(rng.choice([1, 2, 4])), lds_bytes=int(rng.choice([16384, 40960, 65536])))
+    print(f"random geometry {case}: {nx}x{ny}x{nz} pitch {np.round(pitch, 2)} M {n_mat} "
+          f"SOD {s.sod:.0f} SDD {s.sdd:.0f} det {s.det} {s.d_size:.2f} mm rot {s.rot} {tuning}", flush=True)
+    with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
+        proj.set_volumes(vols)
+        proj.set_spectral(**kw)
+        proj.set_tuning(**tuning)
+        lds = proj.forward(want_L=True)
+        proj.set_tuning(march="global", skip_empty=False)
+        glob = proj.forward(want_L=True)
+        I0 = proj.flat_field()
+        torch.cuda.synchronize()
+        for a_, b_ in zip(lds, glob):
+            assert torch.equal(a_, b_)
+        got = (to_np(lds[0]), to_np(lds[1]), to_np(lds[2]), to_np(I0))
+    check_all(got, oracle_expect(s, vols, kw, 0, s.views), name=f"random_geometry_{case}")
+
+
 @pytest.mark.parametrize("shape", [(7, 5, 1), (1, 6, 4), (3, 1, 9)])
 @pytest.mark.parametrize("dtype", [P.CTFP_F32, P.CTFP_F16])
 def test_degenerate_volume_shapes(require_gpu, shape, dtype):
