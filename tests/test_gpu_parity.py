@@ -212,7 +212,8 @@ def test_random_geometries(require_gpu, case):
     too, so rays clip the occupied bricks and step their window ends through
     wide empty space), a random launch tuning.  The LDS march with empty
     space skipped equals the plain global march (no skipping) bit for bit,
-    and FP32 storage meets the FP64 evaluation to TOL."""
+    and the FP64 evaluation is met to TOL (FP32 storage) or to the 16-bit
+    bound (FP16 / BF16 storage, every third case each)."""
     import torch
     rng = np.random.default_rng(1000 + case)
     nx, ny, nz = (int(v) for v in rng.integers(12, 48, 3))
@@ -244,8 +245,9 @@ def test_random_geometries(require_gpu, case):
                   block_u=int(rng.choice([1, 2, 4])), lds_bytes=int(rng.choice([16384, 40960, 65536])))
     print(f"random geometry {case}: {nx}x{ny}x{nz} pitch {np.round(pitch, 2)} M {n_mat} "
           f"SOD {s.sod:.0f} SDD {s.sdd:.0f} det {s.det} {s.d_size:.2f} mm rot {s.rot} {tuning}", flush=True)
+    dtype = (P.CTFP_F32, P.CTFP_F16, P.CTFP_BF16)[case % 3]
     with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
-        proj.set_volumes(vols)
+        proj.set_volumes(vols, storage_dtype=dtype)
         proj.set_spectral(**kw)
         proj.set_tuning(**tuning)
         lds = proj.forward(want_L=True)
@@ -256,7 +258,12 @@ def test_random_geometries(require_gpu, case):
         for a_, b_ in zip(lds, glob):
             assert torch.equal(a_, b_)
         got = (to_np(lds[0]), to_np(lds[1]), to_np(lds[2]), to_np(I0))
-    check_all(got, oracle_expect(s, vols, kw, 0, s.views), name=f"random_geometry_{case}")
+    refs = oracle_expect(s, vols, kw, 0, s.views)
+    if dtype == P.CTFP_F32:
+        check_all(got, refs, name=f"random_geometry_{case}")
+    else:
+        kind = "f16" if dtype == P.CTFP_F16 else "bf16"
+        check_mixed(got, refs[1], f"random_geometry_{case}_{kind}", kind)
 
 
 @pytest.mark.parametrize("shape", [(7, 5, 1), (1, 6, 4), (3, 1, 9)])
