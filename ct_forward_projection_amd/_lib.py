@@ -43,6 +43,7 @@ CTFP_MARCH_GLOBAL = 0
 CTFP_MARCH_LDS = 1
 CTFP_ADJOINT_GATHER = 0
 CTFP_ADJOINT_SCATTER = 1
+CTFP_ADJOINT_CELL = 2
 # bytes of the static dl/dL workspace of ctfp_backward_noalloc (per GPU)
 CTFP_NOALLOC_SCRATCH_BYTES = 64 << 20
 

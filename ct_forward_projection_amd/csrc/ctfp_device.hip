@@ -112,6 +112,9 @@ struct ctfp_context {
     // dl/dL workspace of the gather adjoint
     float *adj_ws = nullptr;
     size_t adj_ws_bytes = 0;
+    // corner sums of the cell gather, [brick][M][corners]
+    float *cell_part = nullptr;
+    size_t cell_part_bytes = 0;
     // bumped by every change of volumes or spectral tables (batches check it)
     uint64_t generation = 0;
 };
@@ -2196,6 +2199,7 @@ struct GatherArgs {
     int32_t bricks_x, bricks_y;
     uint32_t brick_base; // brick of this launch's first workgroup
     int32_t cap;       // staged pixels that fit the LDS budget
+    float *part;       // cell gather: [brick][M][corners] sums, or null
 };
 
 // Each thread owns a quad of 2 x 2 voxels (x, y); a wavefront holds one z
@@ -2438,6 +2442,265 @@ __global__ __launch_bounds__(kGatherBlock) void gather_kernel(GatherArgs A)
         }
 }
 
+// Cell-driven gather.  Each thread owns one cell of an 8 x 8 x 4-cell brick:
+// the cube between voxels (cx, cy, cz) and (cx + 1, cy + 1, cz + 1); a
+// wavefront holds one z layer of the brick.  Every lattice sample lies in
+// exactly one cell, whose thread weights it into the cell's eight corners
+// with the trilinear weights (1 - t, t) per axis, so no weight is evaluated
+// twice or found to be zero.  The samples of a cell come from the ray's
+// plane crossings: on each axis they are the k with
+// min(B(X), B(X + 1)) <= k < max(B(X), B(X + 1)), B(X) = fma(X, 1/d, Q) the
+// (fractional) sample index at which the ray crosses plane X (absolute index
+// space).  Every cell and brick that shares a plane computes its B from the
+// same three operands, so the cells split the ray's samples without a gap or
+// an overlap.  A sample within rounding of a plane may go to either cell;
+// both weight it alike (t = 1 in one cell is t = 0 in the other) up to that
+// rounding (the weights come from the sample's offset in the brick's frame).
+// Bricks start at cell -1 (cx = bx * 8 - 1 + lx), so every
+// cell [-1, n - 1] of each axis is someone's.  A brick's corner sums
+// (9 x 9 x 5 voxels, from cell_corner_sums) go to `part`, and
+// cell_combine_kernel adds the (up to eight) bricks' sums of each voxel to
+// grad in a fixed order: no atomics, a deterministic result.
+constexpr int kCellX = 8, kCellY = 8, kCellZ = 4;
+constexpr int kCellCorners = (kCellX + 1) * (kCellY + 1) * (kCellZ + 1);
+
+// Staged dwords per shadow pixel of the cell gather: the plane-crossing base
+// Q = -q / d and 1 / d (3 axes each; q the ray's position at k = 0, index
+// space, so every brick stages the same values and a plane X is crossed at
+// sample k = fma(X, 1 / d, Q) in all of them), floor(k*) as an int and
+// k* - floor(k*) (k* the sample closest to the brick centre), the offset e of
+// that closest approach from the centre and the step d (3 axes each) and
+// dl/dL_m * dx.  Odd length, as for the voxel gather's records.
+__host__ __device__ constexpr int cell_pixel_words(int m) { return (14 + m) | 1; }
+constexpr int kCQ = 0, kCId = 3, kCKf = 6, kCKr = 7, kCE = 8, kCD = 11, kCG = 14;
+
+// The thread's cell, lower corner (cx, cy, cz) = (X, Y, Z) and (ox, oy, oz)
+// from the brick centre: for each staged pixel of rows [q0, q1] x columns
+// [w0, w1], the lattice samples between the ray's crossings of the cell's
+// planes X, X + 1 (and Y, Z), weighted into the eight corners, times the
+// pixel's dl/dL into acc[corner].  Per sample: three positions, three
+// complements, four (y, z) products and eight accumulations.
+template <int M>
+__device__ __forceinline__ void cell_pixels(const float *rec0, int rs, int q0, int q1, int w0, int w1, float X,
+                                            float Y, float Z, float ox, float oy, float oz, float Kf,
+                                            float (&acc)[8][M])
+{
+    constexpr int PW = cell_pixel_words(M);
+    for (int pv = q0; pv <= q1; pv++) {
+        const float *rec = rec0 + (pv - q0) * rs * PW;
+        for (int pu = w0; pu <= w1; pu++, rec += PW) {
+            const float iX = rec[kCId + 0], iY = rec[kCId + 1], iZ = rec[kCId + 2];
+            const float bX0 = fmaf(X, iX, rec[kCQ + 0]), bX1 = fmaf(X + 1.0f, iX, rec[kCQ + 0]);
+            const float bY0 = fmaf(Y, iY, rec[kCQ + 1]), bY1 = fmaf(Y + 1.0f, iY, rec[kCQ + 1]);
+            const float bZ0 = fmaf(Z, iZ, rec[kCQ + 2]), bZ1 = fmaf(Z + 1.0f, iZ, rec[kCQ + 2]);
+            // k in [max of the entries, min of the exits), inside [0, K)
+            const float lo = fmaxf(fmaxf(fminf(bX0, bX1), fminf(bY0, bY1)), fmaxf(fminf(bZ0, bZ1), 0.0f));
+            const float hi = fminf(fminf(fmaxf(bX0, bX1), fmaxf(bY0, bY1)), fminf(fmaxf(bZ0, bZ1), Kf));
+            const int k0 = (int)ceilf(lo), k1 = (int)ceilf(hi);
+            if (k0 >= k1) continue;
+            const float fX = rec[kCE + 0] - ox, fY = rec[kCE + 1] - oy, fZ = rec[kCE + 2] - oz;
+            const float dX = rec[kCD + 0], dY = rec[kCD + 1], dZ = rec[kCD + 2];
+            float w[8];
+#pragma unroll
+            for (int c = 0; c < 8; c++) w[c] = 0.0f;
+            // s = k - k*, from the brick's closest approach
+            float sj = (float)(k0 - __float_as_int(rec[kCKf])) - rec[kCKr];
+            for (int k = k0; k < k1; k++, sj += 1.0f) {
+                const float tx = fmaf(sj, dX, fX), ty = fmaf(sj, dY, fY), tz = fmaf(sj, dZ, fZ);
+                const float ax = 1.0f - tx, ay = 1.0f - ty, az = 1.0f - tz;
+                const float y0z0 = ay * az, y1z0 = ty * az, y0z1 = ay * tz, y1z1 = ty * tz;
+                w[0] = fmaf(y0z0, ax, w[0]);
+                w[1] = fmaf(y0z0, tx, w[1]);
+                w[2] = fmaf(y1z0, ax, w[2]);
+                w[3] = fmaf(y1z0, tx, w[3]);
+                w[4] = fmaf(y0z1, ax, w[4]);
+                w[5] = fmaf(y0z1, tx, w[5]);
+                w[6] = fmaf(y1z1, ax, w[6]);
+                w[7] = fmaf(y1z1, tx, w[7]);
+            }
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                const float g = rec[kCG + m];
+#pragma unroll
+                for (int c = 0; c < 8; c++) acc[c][m] = fmaf(w[c], g, acc[c][m]);
+            }
+        }
+    }
+}
+
+template <int M>
+__global__ __launch_bounds__(kGatherBlock) void cell_gather_kernel(GatherArgs A)
+{
+    extern __shared__ char lds_raw[];
+    const MarchArgs &P = A.P;
+    const int cap = A.cap;
+    constexpr int PW = cell_pixel_words(M);
+    BrickView *s_view = (BrickView *)(lds_raw + 256); // kGatherViews entries
+    float *s_px = (float *)(lds_raw + 256 + kGatherViews * sizeof(BrickView));
+    const uint32_t brick = A.brick_base + blockIdx.x;
+    const int bx = (int)(brick % (uint32_t)A.bricks_x);
+    const int by = (int)((brick / (uint32_t)A.bricks_x) % (uint32_t)A.bricks_y);
+    const int bz = (int)(brick / ((uint32_t)A.bricks_x * (uint32_t)A.bricks_y));
+    const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
+    // the cell's lower corner; cells -1 .. n - 1 of each axis take samples
+    const int cx = bx * kCellX - 1 + lx, cy = by * kCellY - 1 + ly, cz = bz * kCellZ - 1 + lz;
+    const bool inside = cx < (int)P.nx && cy < (int)P.ny && cz < (int)P.nz;
+    // the brick's centre (index space) and the cell's lower corner from it;
+    // the planes in absolute index space (exact floats, the same in every
+    // cell and brick that shares one)
+    const double ca = bx * kCellX - 1 + 0.5 * kCellX, cb = by * kCellY - 1 + 0.5 * kCellY;
+    const double cc = bz * kCellZ - 1 + 0.5 * kCellZ;
+    const float ox = (float)(lx - kCellX / 2), oy = (float)(ly - kCellY / 2), oz = (float)(lz - kCellZ / 2);
+    const float X = (float)cx, Y = (float)cy, Z = (float)cz, Kf = (float)P.K;
+    const int64_t npix = (int64_t)P.nu * P.nv;
+    float acc[8][M];
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+#pragma unroll
+        for (int m = 0; m < M; m++) acc[c][m] = 0.0f;
+    for (int vt = 0; vt < P.n_views; vt += kGatherViews) {
+        const int nvt = min(kGatherViews, P.n_views - vt);
+        __syncthreads(); // the previous views' table is consumed
+        if ((int)threadIdx.x < nvt) {
+            const int view = (int)(P.view_begin + vt + threadIdx.x);
+            double u0, v0, ua, va, ub, vb, uc, vc;
+            project_point(P, view, ca, cb, cc, u0, v0);
+            project_point(P, view, ca + 1.0, cb, cc, ua, va);
+            project_point(P, view, ca, cb + 1.0, cc, ub, vb);
+            project_point(P, view, ca, cb, cc + 1.0, uc, vc);
+            BrickView bv;
+            bv.u0 = (float)u0;
+            bv.v0 = (float)v0;
+            bv.jua = (float)(ua - u0);
+            bv.jub = (float)(ub - u0);
+            bv.juc = (float)(uc - u0);
+            bv.jva = (float)(va - v0);
+            bv.jvb = (float)(vb - v0);
+            bv.jvc = (float)(vc - v0);
+            // a cell spans +-0.5 voxel around its centre: +-h pixels (plus a
+            // quarter pixel for the FP32 centre and Jacobian)
+            bv.hu = 0.5f * (fabsf(bv.jua) + fabsf(bv.jub) + fabsf(bv.juc)) + 0.25f;
+            bv.hv = 0.5f * (fabsf(bv.jva) + fabsf(bv.jvb) + fabsf(bv.jvc)) + 0.25f;
+            const double eu = fabs(ua - u0) * (0.5 * kCellX) + fabs(ub - u0) * (0.5 * kCellY) +
+                              fabs(uc - u0) * (0.5 * kCellZ) + 0.5;
+            const double ev = fabs(va - v0) * (0.5 * kCellX) + fabs(vb - v0) * (0.5 * kCellY) +
+                              fabs(vc - v0) * (0.5 * kCellZ) + 0.5;
+            bv.ru0 = max(0, (int)ceil(u0 - eu));
+            bv.ru1 = min(P.nu - 1, (int)floor(u0 + eu));
+            bv.rv0 = max(0, (int)ceil(v0 - ev));
+            bv.rv1 = min(P.nv - 1, (int)floor(v0 + ev));
+            s_view[threadIdx.x] = bv;
+        }
+        __syncthreads();
+        for (int vw = vt; vw < vt + nvt; vw++) {
+            const BrickView bv = s_view[vw - vt];
+            const int ru0 = bv.ru0, ru1 = bv.ru1, rv0 = bv.rv0, rv1 = bv.rv1;
+            if (ru0 > ru1 || rv0 > rv1) continue; // block-uniform
+            // the cell's pixel window (inside the brick's)
+            const float cu = bv.u0 + bv.jua * (ox + 0.5f) + bv.jub * (oy + 0.5f) + bv.juc * (oz + 0.5f);
+            const float cv = bv.v0 + bv.jva * (ox + 0.5f) + bv.jvb * (oy + 0.5f) + bv.jvc * (oz + 0.5f);
+            const int pu0 = max(ru0, (int)ceilf(cu - bv.hu)), pu1 = min(ru1, (int)floorf(cu + bv.hu));
+            const int pv0 = max(rv0, (int)ceilf(cv - bv.hv)), pv1 = min(rv1, (int)floorf(cv + bv.hv));
+            for (int c0 = ru0; c0 <= ru1; c0 += cap) {
+                const int c1 = min(ru1, c0 + cap - 1);
+                const int rw = c1 - c0 + 1;
+                const int band = max(1, cap / rw);
+                for (int b0 = rv0; b0 <= rv1; b0 += band) {
+                    const int b1 = min(rv1, b0 + band - 1);
+                    const int count = (b1 - b0 + 1) * rw;
+                    __syncthreads(); // the previous band's entries are consumed
+                    for (int i = threadIdx.x; i < count; i += kGatherBlock) {
+                        const int pv = b0 + i / rw, pu = c0 + i % rw;
+                        const RayLine Ln = ray_line(P, pu, pv, (int)(P.view_begin + vw));
+                        const double dxa = Ln.da * P.dx, dxb = Ln.db * P.dx, dxc = Ln.dc * P.dx;
+                        // position at k = 0: absolute, and from the brick centre
+                        const double pa = Ln.sa + Ln.da * P.t0, pb = Ln.sb + Ln.db * P.t0, pc = Ln.sc + Ln.dc * P.t0;
+                        const double qa = pa - ca, qb = pb - cb, qc = pc - cc;
+                        const double ks =
+                            -(qa * dxa + qb * dxb + qc * dxc) / (dxa * dxa + dxb * dxb + dxc * dxc);
+                        const double kf = floor(ks);
+                        const double p[3] = {pa, pb, pc};
+                        const float e[3] = {(float)(qa + ks * dxa), (float)(qb + ks * dxb), (float)(qc + ks * dxc)};
+                        const float d[3] = {(float)dxa, (float)dxb, (float)dxc};
+                        float *rec = s_px + i * PW;
+#pragma unroll
+                        for (int a = 0; a < 3; a++) {
+                            // a zero step makes that axis's crossings +-huge:
+                            // all of the ray's samples in the cells holding
+                            // it, none in the others
+                            const float id = __builtin_amdgcn_rcpf(fabsf(d[a]) > 1e-30f ? d[a] : 1e-30f);
+                            rec[kCQ + a] = (float)(-p[a] * (double)id);
+                            rec[kCId + a] = id;
+                            rec[kCE + a] = e[a];
+                            rec[kCD + a] = d[a];
+                        }
+                        rec[kCKf] = __int_as_float((int)kf);
+                        rec[kCKr] = (float)(ks - kf);
+                        const int64_t pix = (int64_t)pv * P.nu + pu;
+#pragma unroll
+                        for (int m = 0; m < M; m++)
+                            rec[kCG + m] = A.gL[((int64_t)vw * A.gl_mats + A.mat_off + m) * npix + pix];
+                    }
+                    __syncthreads();
+                    if (!inside) continue;
+                    const int q0 = max(pv0, b0), q1 = min(pv1, b1);
+                    const int w0 = max(pu0, c0), w1 = min(pu1, c1);
+                    if (q0 > q1 || w0 > w1) continue;
+                    cell_pixels<M>(s_px + ((q0 - b0) * rw + (w0 - c0)) * PW, rw, q0, q1, w0, w1, X, Y, Z, ox, oy,
+                                   oz, Kf, acc);
+                }
+            }
+        }
+    }
+    // the brick's corner sums, one corner of every cell at a time (the cells
+    // of one pass add to distinct voxels), then out to `part`
+    __syncthreads();
+    float *V = (float *)lds_raw; // [M][kCellZ + 1][kCellY + 1][kCellX + 1]
+    for (int i = threadIdx.x; i < M * kCellCorners; i += kGatherBlock) V[i] = 0.0f;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int vi = ((lz + (c >> 2)) * (kCellY + 1) + ly + ((c >> 1) & 1)) * (kCellX + 1) + lx + (c & 1);
+#pragma unroll
+        for (int m = 0; m < M; m++) V[m * kCellCorners + vi] += acc[c][m];
+        __syncthreads();
+    }
+    float *dst = A.part + (int64_t)brick * (M * kCellCorners);
+    for (int i = threadIdx.x; i < M * kCellCorners; i += kGatherBlock) dst[i] = V[i];
+}
+
+// grad[mat_off + m] += the cell gather's corner sums of each voxel: voxel x
+// is corner (x + 1) % 8 of brick (x + 1) / 8 and, when that corner is 0,
+// also corner 8 of the brick before (4 for z).  Fixed order: deterministic.
+template <int M>
+__global__ __launch_bounds__(256) void cell_combine_kernel(const float *__restrict__ part, float *grad, int nx, int ny,
+                                                           int nz, int bricks_x, int bricks_y, int mat_off)
+{
+    const int64_t nvox = (int64_t)nx * ny * nz;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nvox;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(q % nx), y = (int)((q / nx) % ny), z = (int)(q / ((int64_t)nx * ny));
+        const int bx = (x + 1) / kCellX, ix = (x + 1) % kCellX;
+        const int by = (y + 1) / kCellY, iy = (y + 1) % kCellY;
+        const int bz = (z + 1) / kCellZ, iz = (z + 1) % kCellZ;
+        float s[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) s[m] = 0.0f;
+        for (int sz = 0; sz < (iz == 0 && bz > 0 ? 2 : 1); sz++)
+            for (int sy = 0; sy < (iy == 0 && by > 0 ? 2 : 1); sy++)
+                for (int sx = 0; sx < (ix == 0 && bx > 0 ? 2 : 1); sx++) {
+                    const int64_t b = ((int64_t)(bz - sz) * bricks_y + (by - sy)) * bricks_x + (bx - sx);
+                    const int vi = ((iz + sz * kCellZ) * (kCellY + 1) + iy + sy * kCellY) * (kCellX + 1) + ix +
+                                   sx * kCellX;
+                    const float *src = part + b * (M * kCellCorners) + vi;
+#pragma unroll
+                    for (int m = 0; m < M; m++) s[m] += src[m * kCellCorners];
+                }
+#pragma unroll
+        for (int m = 0; m < M; m++) grad[(int64_t)(mat_off + m) * nvox + q] += s[m];
+    }
+}
+
 template <int M, int ST>
 __global__ __launch_bounds__(kBlock) void adjoint_kernel(MarchArgs P, AdjointArgs G)
 {
@@ -2562,6 +2825,29 @@ GatherFn pick_gather(int n_mat)
     case 2: return gather_kernel<2>;
     case 3: return gather_kernel<3>;
     case 4: return gather_kernel<4>;
+    default: return nullptr;
+    }
+}
+
+GatherFn pick_cell_gather(int n_mat)
+{
+    switch (n_mat) {
+    case 1: return cell_gather_kernel<1>;
+    case 2: return cell_gather_kernel<2>;
+    case 3: return cell_gather_kernel<3>;
+    case 4: return cell_gather_kernel<4>;
+    default: return nullptr;
+    }
+}
+
+using CombineFn = void (*)(const float *, float *, int, int, int, int, int, int);
+CombineFn pick_cell_combine(int n_mat)
+{
+    switch (n_mat) {
+    case 1: return cell_combine_kernel<1>;
+    case 2: return cell_combine_kernel<2>;
+    case 3: return cell_combine_kernel<3>;
+    case 4: return cell_combine_kernel<4>;
     default: return nullptr;
     }
 }
@@ -2876,6 +3162,7 @@ int ctfp_destroy(ctfp_context *ctx)
     dfree(ctx->d_flag);
     dfree(ctx->l_ws);
     dfree(ctx->adj_ws);
+    dfree(ctx->cell_part);
     dfree(ctx->focal);
     dfree(ctx->Aq);
     dfree(ctx->A);
@@ -3063,6 +3350,7 @@ int ctfp_device_bytes(const ctfp_context *ctx, int64_t *bytes)
     if (ctx->view_cs) b += ctx->scan.proj_num * (int64_t)sizeof(double2);
     if (ctx->dI) b += 2 * ctx->staged_views * npix * (int64_t)sizeof(float);
     if (ctx->adj_ws) b += (int64_t)ctx->adj_ws_bytes;
+    if (ctx->cell_part) b += (int64_t)ctx->cell_part_bytes;
     if (ctx->l_ws) b += (int64_t)ctx->l_ws_bytes;
     if (ctx->occ) b += ctx->ob_x * ctx->ob_y * ctx->ob_z * ctx->n_groups;
     *bytes = b;
@@ -3492,7 +3780,7 @@ int ctfp_set_tuning(ctfp_context *ctx, const ctfp_tuning *t)
         return set_error(CTFP_ERR_INVALID, "tuning: wave_u %d is not 4, 8, 16 or 32", t->wave_u);
     if (t->block_u != 1 && t->block_u != 2 && t->block_u != 4)
         return set_error(CTFP_ERR_INVALID, "tuning: block_u %d is not 1, 2 or 4", t->block_u);
-    if (t->adjoint != CTFP_ADJOINT_GATHER && t->adjoint != CTFP_ADJOINT_SCATTER)
+    if (t->adjoint != CTFP_ADJOINT_GATHER && t->adjoint != CTFP_ADJOINT_SCATTER && t->adjoint != CTFP_ADJOINT_CELL)
         return set_error(CTFP_ERR_INVALID, "tuning: adjoint mode %d", t->adjoint);
     if (t->skip_empty != 0 && t->skip_empty != 1)
         return set_error(CTFP_ERR_INVALID, "tuning: skip_empty %d is not 0 or 1", t->skip_empty);
@@ -3529,6 +3817,9 @@ static int backward_impl(ctfp_context *ctx, int64_t view_begin, int64_t view_cou
     if (rc) return rc;
     if ((rc = spectral_args(ctx, P))) return rc;
     if (!grad) return set_error(CTFP_ERR_INVALID, "null gradient output");
+    // a caller's workspace (the call allocates nothing): the voxel gather,
+    // which needs no buffer of corner sums
+    const bool cell = ctx->tune.adjoint == CTFP_ADJOINT_CELL && !ws;
     const bool grouped = ctx->n_groups > 1;
     // CTFP_MARCH_GLOBAL: no chunk is staged; every sample gathers and
     // scatters through global memory (the LDS then only holds the box scratch)
@@ -3597,11 +3888,28 @@ static int backward_impl(ctfp_context *ctx, int64_t view_begin, int64_t view_cou
     A.grad = grad;
     A.gL = ws;
     A.gl_mats = ctx->n_mat;
-    A.bricks_x = (int32_t)((s.p_num_x + kBrickX - 1) / kBrickX);
-    A.bricks_y = (int32_t)((s.p_num_y + kBrickY - 1) / kBrickY);
-    const int64_t bricks_z = (s.p_num_z + kBrickZ - 1) / kBrickZ;
+    A.part = nullptr;
+    // voxel gather: bricks of 16 x 16 x 4 voxels; cell gather: 8 x 8 x 4
+    // cells from cell -1, so n + 1 cells per axis
+    const int64_t ex = cell ? 1 : 0;
+    const int bxd = cell ? kCellX : kBrickX, byd = cell ? kCellY : kBrickY, bzd = cell ? kCellZ : kBrickZ;
+    A.bricks_x = (int32_t)((s.p_num_x + ex + bxd - 1) / bxd);
+    A.bricks_y = (int32_t)((s.p_num_y + ex + byd - 1) / byd);
+    const int64_t bricks_z = (s.p_num_z + ex + bzd - 1) / bzd;
     const int64_t bricks = (int64_t)A.bricks_x * A.bricks_y * bricks_z;
     if (bricks >= (1ll << 31)) return set_error(CTFP_ERR_UNSUPPORTED, "volume too large for the gather");
+    if (cell) {
+        int gmax = 1;
+        for (int gi = 0; gi < ctx->n_groups; gi++) gmax = std::max(gmax, ctx->group_mats[gi]);
+        const size_t need = (size_t)bricks * (size_t)gmax * kCellCorners * sizeof(float);
+        if (ctx->cell_part_bytes < need) {
+            dfree(ctx->cell_part);
+            ctx->cell_part_bytes = 0;
+            HIP_TRY(dev_malloc(&ctx->cell_part, need));
+            ctx->cell_part_bytes = need;
+        }
+        A.part = ctx->cell_part;
+    }
     for (int64_t v0 = 0; v0 < view_count; v0 += group) {
         const int64_t n = std::min(group, view_count - v0);
         AdjointArgs G{gI ? gI + v0 * npix : nullptr, gIAbsorb ? gIAbsorb + v0 * npix : nullptr,
@@ -3634,17 +3942,27 @@ static int backward_impl(ctfp_context *ctx, int64_t view_begin, int64_t view_cou
         // the voxel pass, per channel group, in launches of at most
         // kMaxGatherBricks bricks
         for (int gi = 0; gi < ctx->n_groups; gi++) {
-            GatherFn gfn = pick_gather(ctx->group_mats[gi]);
-            if (!gfn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", ctx->group_mats[gi]);
+            const int gm = ctx->group_mats[gi];
+            GatherFn gfn = cell ? pick_cell_gather(gm) : pick_gather(gm);
+            if (!gfn) return set_error(CTFP_ERR_UNSUPPORTED, "n_mat=%d", gm);
             if ((rc = fill_march_args(ctx, view_begin + v0, n, A.P, gi))) return rc;
             A.mat_off = ctx->group_moff[gi];
             A.cap = (int32_t)((kGatherLds - 256 - kGatherViews * (int)sizeof(BrickView)) /
-                              (gather_pixel_words(ctx->group_mats[gi]) * 4));
+                              ((cell ? cell_pixel_words(gm) : gather_pixel_words(gm)) * 4));
             for (int64_t b0 = 0; b0 < bricks; b0 += (int64_t)kMaxGatherBricks) {
                 A.brick_base = (uint32_t)b0;
                 const int64_t nb = std::min<int64_t>((int64_t)kMaxGatherBricks, bricks - b0);
                 hipLaunchKernelGGL(gfn, dim3((unsigned)nb), dim3(kGatherBlock), (size_t)kGatherLds,
                                    (hipStream_t)stream, A);
+                HIP_TRY(hipGetLastError());
+            }
+            if (cell) {
+                // the bricks' corner sums into grad, one pass over the voxels
+                const int64_t nvox = s.p_num_x * s.p_num_y * s.p_num_z;
+                const int64_t blocks = std::min<int64_t>((nvox + 255) / 256, 1ll << 20);
+                hipLaunchKernelGGL(pick_cell_combine(gm), dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                                   (const float *)A.part, grad, (int)s.p_num_x, (int)s.p_num_y, (int)s.p_num_z,
+                                   A.bricks_x, A.bricks_y, A.mat_off);
                 HIP_TRY(hipGetLastError());
             }
         }

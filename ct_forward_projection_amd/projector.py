@@ -192,13 +192,14 @@ class Projector:
         """Launch options of the march ("global" or "lds", samples per staged
         chunk, LDS bytes per workgroup, ray-tile shape, skipping of chunks
         whose cells are all zero; projections are identical for all) and of
-        the adjoint ("gather" or "scatter")."""
+        the adjoint ("gather", "cell" or "scatter")."""
         t = L.Tuning()
         L.check(L.lib().ctfp_get_tuning(self._live(), C.byref(t)))
         if march is not None:
             t.march = {"global": L.CTFP_MARCH_GLOBAL, "lds": L.CTFP_MARCH_LDS}[march]
         if adjoint is not None:
-            t.adjoint = {"gather": L.CTFP_ADJOINT_GATHER, "scatter": L.CTFP_ADJOINT_SCATTER}[adjoint]
+            t.adjoint = {"gather": L.CTFP_ADJOINT_GATHER, "scatter": L.CTFP_ADJOINT_SCATTER,
+                         "cell": L.CTFP_ADJOINT_CELL}[adjoint]
         if skip_empty is not None:
             t.skip_empty = int(bool(skip_empty))
         for name, v in (("chunk", chunk), ("lds_bytes", lds_bytes), ("wave_u", wave_u),
@@ -234,7 +235,7 @@ class Projector:
         L.check(L.lib().ctfp_get_tuning(self._live(), C.byref(t)))
         return {"march": "lds" if t.march == L.CTFP_MARCH_LDS else "global", "chunk": t.chunk,
                 "lds_bytes": t.lds_bytes, "wave_u": t.wave_u, "block_u": t.block_u,
-                "adjoint": "scatter" if t.adjoint == L.CTFP_ADJOINT_SCATTER else "gather",
+                "adjoint": {L.CTFP_ADJOINT_SCATTER: "scatter", L.CTFP_ADJOINT_CELL: "cell"}.get(t.adjoint, "gather"),
                 "skip_empty": bool(t.skip_empty)}
 
     def set_focal_spot(self, offsets=None):

@@ -126,13 +126,16 @@ typedef struct ctfp_tuning {
     int32_t lds_bytes; /* LDS per workgroup (LDS mode), 1 KiB..64 KiB; default 40 KiB */
     int32_t wave_u;    /* ray tile: pixels along u per wavefront, 4/8/16/32 (64 / wave_u along v); default 8 */
     int32_t block_u;   /* ray tile: wavefronts along u per workgroup, 1/2/4 (4 / block_u along v); default 2 */
-    int32_t adjoint;   /* ctfp_backward: CTFP_ADJOINT_GATHER (default) or CTFP_ADJOINT_SCATTER */
+    int32_t adjoint;   /* ctfp_backward: CTFP_ADJOINT_GATHER (default), CTFP_ADJOINT_CELL or CTFP_ADJOINT_SCATTER */
     int32_t skip_empty; /* 1 (default): a staged chunk whose cell box is zero in every channel
                            (checked on an occupancy grid of 8^3-cell bricks built with the
                            volumes) is skipped; it would add exact zeros.  0: every chunk marched */
 } ctfp_tuning;
 #define CTFP_ADJOINT_GATHER 0  /* ray pass to dl/dL, then a voxel-driven gather: no atomics, deterministic */
 #define CTFP_ADJOINT_SCATTER 1 /* ray-driven scatter through LDS boxes and float atomics */
+#define CTFP_ADJOINT_CELL 2    /* ray pass to dl/dL, then a cell-driven gather (each sample's eight corner weights
+                                  once) and a combine pass: no atomics, deterministic; holds a per-brick buffer of
+                                  corner sums (about 1.4 x M floats per voxel) */
 
 typedef struct ctfp_context ctfp_context;
 

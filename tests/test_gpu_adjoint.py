@@ -32,7 +32,7 @@ def det_table(kw):
                      for m in kw["scint_mu"]])
 
 
-@pytest.mark.parametrize("mode", ["gather", "scatter"])
+@pytest.mark.parametrize("mode", ["gather", "cell", "scatter"])
 @pytest.mark.parametrize("with_L", [False, True])
 def test_adjoint_matches_fp64_cpu_adjoint(require_gpu, with_L, mode):
     import torch
@@ -66,7 +66,7 @@ def test_adjoint_matches_fp64_cpu_adjoint(require_gpu, with_L, mode):
 
 
 @pytest.mark.parametrize("n_mat", [1, 2, 4])
-@pytest.mark.parametrize("mode", ["gather", "scatter"])
+@pytest.mark.parametrize("mode", ["gather", "cell", "scatter"])
 def test_adjoint_anisotropic_material_counts(require_gpu, n_mat, mode):
     """A non-cubic volume (40 x 32 x 24) with 0.6 / 0.45 / 0.8 mm voxels and
     1, 2 or 4 materials: the adjoint against the FP64 CPU adjoint."""
@@ -159,8 +159,9 @@ def test_adjoint_finite_differences(require_gpu):
 
 
 def test_gather_adjoint_is_deterministic_and_matches_scatter(require_gpu):
-    """The gather adjoint sums every voxel in a fixed order (bit-identical on
-    repeat); the scatter adjoint reaches the same gradient through atomics.
+    """The voxel and cell gathers sum every voxel in a fixed order
+    (bit-identical on repeat); the scatter adjoint reaches the same gradient
+    through atomics.
     The volume (45 x 37 x 30) does not divide into 8 x 8 x 4 bricks, and the
     view range does not start at view 0."""
     import torch
@@ -176,10 +177,16 @@ def test_gather_adjoint_is_deterministic_and_matches_scatter(require_gpu):
     with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
         proj.set_volumes(vols)
         proj.set_spectral(**kw)
+        proj.set_tuning(adjoint="gather")
         a = proj.backward(gI, gIA, view_begin=2, view_count=6)
         b2 = proj.backward(gI, gIA, view_begin=2, view_count=6)
         torch.cuda.synchronize()
         assert torch.equal(a, b2)
+        proj.set_tuning(adjoint="cell")
+        e = proj.backward(gI, gIA, view_begin=2, view_count=6)
+        e2 = proj.backward(gI, gIA, view_begin=2, view_count=6)
+        torch.cuda.synchronize()
+        assert torch.equal(e, e2)
         proj.set_tuning(adjoint="scatter")
         c = proj.backward(gI, gIA, view_begin=2, view_count=6)
         torch.cuda.synchronize()
@@ -190,7 +197,7 @@ def test_gather_adjoint_is_deterministic_and_matches_scatter(require_gpu):
                              gIA.cpu().numpy(), kw["grid"], det_table(kw))
     ref = O.backproject_f64(g, p, gL, view0=2)
     scale = O.backproject_f64(g, p, np.abs(gL), view0=2)
-    for got in (a, c):
+    for got in (a, e, c):
         err = np.abs(got.cpu().numpy() - ref) / np.maximum(scale, 1e-3 * scale.max())
         assert err.max() < 1e-4, err.max()
 
@@ -224,7 +231,7 @@ def test_adjoint_lds_equals_global_atomics(require_gpu):
             assert err < 1e-4, (march, chunk, lds, err)
 
 
-@pytest.mark.parametrize("mode", ["gather", "scatter"])
+@pytest.mark.parametrize("mode", ["gather", "cell", "scatter"])
 def test_adjoint_with_focal_spot(require_gpu, mode):
     import torch
     s, vols = setup_case(n=36, views=5, det=(64, 48))
@@ -251,12 +258,14 @@ def test_adjoint_with_focal_spot(require_gpu, mode):
     assert err.max() < 1e-4, err.max()
 
 
+@pytest.mark.parametrize("mode", ["gather", "cell"])
 @pytest.mark.parametrize("case", range(6))
-def test_adjoint_random_geometries(require_gpu, case):
+def test_adjoint_random_geometries(require_gpu, case, mode):
     """Seeded random scans (volume shape and pitch per axis, SOD and SDD,
     rotation direction, detector size and pitch, 1-3 materials of balls and
-    boxes): the gather adjoint, whose bricks stage per-pixel records and
-    walk 2 x 2 voxel quads, against the FP64 CPU adjoint."""
+    boxes): the voxel gather (bricks staging per-pixel records, 2 x 2 voxel
+    quads) and the cell gather (each sample weighted once, into its cell's
+    corners, by the plane crossings) against the FP64 CPU adjoint."""
     import torch
     rng = np.random.default_rng(2000 + case)
     nx, ny, nz = (int(v) for v in rng.integers(10, 40, 3))
@@ -287,6 +296,7 @@ def test_adjoint_random_geometries(require_gpu, case):
     with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
         proj.set_volumes(vols)
         proj.set_spectral(**kw)
+        proj.set_tuning(adjoint=mode)
         grad = proj.backward(torch.from_numpy(gI).to(torch.device("cuda")))
         torch.cuda.synchronize()
         grad = grad.cpu().numpy()

@@ -68,10 +68,11 @@ def test_six_materials_forward_16bit(require_gpu, dtype, name):
         assert e[k] < MIXED_TOL[name], (k, e)
 
 
-def test_six_materials_adjoint(require_gpu):
-    """The gather adjoint of a 6-material scan (two channel groups) against
-    the FP64 CPU adjoint, with L recomputed and with L given; the scatter
-    adjoint (up to four materials) refuses it."""
+@pytest.mark.parametrize("mode", ["gather", "cell"])
+def test_six_materials_adjoint(require_gpu, mode):
+    """The voxel and cell gather adjoints of a 6-material scan (two channel
+    groups) against the FP64 CPU adjoint, with L recomputed and with L given;
+    the scatter adjoint (up to four materials) refuses it."""
     import torch
     s, vols = six_material_case(n=32, views=5, det=(56, 44))
     kw = P.spectral_inputs(s, *s.det)
@@ -82,6 +83,7 @@ def test_six_materials_adjoint(require_gpu):
     with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
         proj.set_volumes(vols)
         proj.set_spectral(**kw)
+        proj.set_tuning(adjoint=mode)
         _, _, Lg = proj.forward(want_L=True)
         g1 = proj.backward(torch.from_numpy(gI).to(dev), torch.from_numpy(gIA).to(dev))
         g2 = proj.backward(torch.from_numpy(gI).to(dev), torch.from_numpy(gIA).to(dev), L_in=Lg)
