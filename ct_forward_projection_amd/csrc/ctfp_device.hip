@@ -108,7 +108,7 @@ struct ctfp_context {
     float *dI = nullptr, *dIA = nullptr;
     int64_t staged_views = 0;
     // launch tuning
-    ctfp_tuning tune{CTFP_MARCH_LDS, 16, 40 * 1024, 8, 2, CTFP_ADJOINT_GATHER, 1};
+    ctfp_tuning tune{CTFP_MARCH_LDS, 16, 40 * 1024, 8, 2, CTFP_ADJOINT_CELL, 1};
     // dl/dL workspace of the gather adjoint
     float *adj_ws = nullptr;
     size_t adj_ws_bytes = 0;

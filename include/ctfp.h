@@ -126,7 +126,7 @@ typedef struct ctfp_tuning {
     int32_t lds_bytes; /* LDS per workgroup (LDS mode), 1 KiB..64 KiB; default 40 KiB */
     int32_t wave_u;    /* ray tile: pixels along u per wavefront, 4/8/16/32 (64 / wave_u along v); default 8 */
     int32_t block_u;   /* ray tile: wavefronts along u per workgroup, 1/2/4 (4 / block_u along v); default 2 */
-    int32_t adjoint;   /* ctfp_backward: CTFP_ADJOINT_GATHER (default), CTFP_ADJOINT_CELL or CTFP_ADJOINT_SCATTER */
+    int32_t adjoint;   /* ctfp_backward: CTFP_ADJOINT_CELL (default), CTFP_ADJOINT_GATHER or CTFP_ADJOINT_SCATTER */
     int32_t skip_empty; /* 1 (default): a staged chunk whose cell box is zero in every channel
                            (checked on an occupancy grid of 8^3-cell bricks built with the
                            volumes) is skipped; it would add exact zeros.  0: every chunk marched */
@@ -236,10 +236,12 @@ int ctfp_line_integrals(ctfp_context *ctx, int64_t view_begin, int64_t view_coun
  * dl/d(density) to `grad` (device FP32 [n_mat][p_num_z][p_num_y][p_num_x];
  * the caller zeroes it).  L (device [view_count][n_mat][d_num_v][d_num_u],
  * from ctfp_forward) may be NULL, then the line integrals are recomputed per
- * ray.  With CTFP_ADJOINT_GATHER (the default) the result is deterministic;
- * the call holds a workspace of dl/dL for up to 1 GiB of views, kept by the
- * context.  With CTFP_ADJOINT_SCATTER accumulation uses FP32 atomics, so the
- * last bits can differ between runs. */
+ * ray.  With CTFP_ADJOINT_CELL (the default) or CTFP_ADJOINT_GATHER the
+ * result is deterministic; the call holds a workspace of dl/dL for up to
+ * 1 GiB of views, and for the cell gather a buffer of per-brick corner sums
+ * (about 1.4 x n_mat floats per voxel, at most 4 materials' worth), both
+ * kept by the context.  With CTFP_ADJOINT_SCATTER accumulation uses FP32
+ * atomics, so the last bits can differ between runs. */
 int ctfp_backward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, const float *gI,
                   const float *gIAbsorb, const float *L, float *grad, void *stream);
 
@@ -248,7 +250,10 @@ int ctfp_backward(ctfp_context *ctx, int64_t view_begin, int64_t view_count, con
  * forbid hipMalloc): ctfp_backward with the gather adjoint's dl/dL workspace
  * in a static device array of CTFP_NOALLOC_SCRATCH_BYTES per GPU (held until
  * the process ends) instead of the context's allocation.  Calls no hipMalloc
- * or hipFree.  Views run in groups whose dl/dL (n_mat * d_num_v * d_num_u * 4
+ * or hipFree.  Under CTFP_ADJOINT_CELL it runs the voxel gather
+ * (CTFP_ADJOINT_GATHER), which needs no corner-sum buffer: the result then
+ * equals ctfp_backward's under CTFP_ADJOINT_GATHER bit for bit and its
+ * default result to FP32 rounding.  Views run in groups whose dl/dL (n_mat * d_num_v * d_num_u * 4
  * bytes per view) fits the array; a scan one view of which needs more is
  * rejected with CTFP_ERR_UNSUPPORTED.  Every call on one GPU shares the
  * array: calls that overlap on one GPU (several streams or host threads)

@@ -1,9 +1,11 @@
 """ctfp_backward_noalloc (the adjoint with its workspace in a static device
 array, for callers that cannot allocate) against ctfp_backward, through the C
 ABI: two calls in a row, the second on a smaller scan, must give the same
-gradients bit for bit (both run one group of views); a scan whose dl/dL
-needs several groups of the static workspace agrees to FP32 rounding; a scan
-one view of which exceeds the workspace is rejected."""
+gradients bit for bit under the voxel gather (both run one group of views),
+and agree to FP32 rounding with ctfp_backward's default cell gather (the
+no-allocation call runs the voxel gather); a scan whose dl/dL needs several
+groups of the static workspace agrees to FP32 rounding; a scan one view of
+which exceeds the workspace is rejected."""
 import numpy as np
 import pytest
 
@@ -51,9 +53,13 @@ def test_noalloc_backward_equals_backward_twice(require_gpu):
         with P.Projector(P.scan_params(**s.scan_kwargs())) as proj:
             proj.set_volumes(vols)
             proj.set_spectral(**P.spectral_inputs(s, *s.det))
+            cell, got_cell = grads(proj, s, seed=20 + k)
+            proj.set_tuning(adjoint="gather")
             ref, got = grads(proj, s, seed=20 + k)
         assert np.abs(ref).max() > 0
         np.testing.assert_array_equal(got, ref)
+        np.testing.assert_array_equal(got_cell, ref)
+        assert np.abs(cell - ref).max() <= 1e-5 * np.abs(ref).max()
 
 
 def test_noalloc_backward_view_groups(require_gpu):
