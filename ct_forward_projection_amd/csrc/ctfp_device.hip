@@ -2462,6 +2462,7 @@ __global__ __launch_bounds__(kGatherBlock) void gather_kernel(GatherArgs A)
 // cell_combine_kernel adds the (up to eight) bricks' sums of each voxel to
 // grad in a fixed order: no atomics, a deterministic result.
 constexpr int kCellX = 8, kCellY = 8, kCellZ = 4;
+constexpr int kCellBlock = kCellX * kCellY * kCellZ; // one cell per thread
 constexpr int kCellCorners = (kCellX + 1) * (kCellY + 1) * (kCellZ + 1);
 
 // Staged dwords per shadow pixel of the cell gather: the plane-crossing base
@@ -2500,12 +2501,25 @@ __device__ __forceinline__ void cell_pixels(const float *rec0, int rs, int q0, i
             if (k0 >= k1) continue;
             const float fX = rec[kCE + 0] - ox, fY = rec[kCE + 1] - oy, fZ = rec[kCE + 2] - oz;
             const float dX = rec[kCD + 0], dY = rec[kCD + 1], dZ = rec[kCD + 2];
-            float w[8];
-#pragma unroll
-            for (int c = 0; c < 8; c++) w[c] = 0.0f;
             // s = k - k*, from the brick's closest approach
             float sj = (float)(k0 - __float_as_int(rec[kCKf])) - rec[kCKr];
-            for (int k = k0; k < k1; k++, sj += 1.0f) {
+            float w[8];
+            {
+                // the first sample (k0 < k1) sets the weights
+                const float tx = fmaf(sj, dX, fX), ty = fmaf(sj, dY, fY), tz = fmaf(sj, dZ, fZ);
+                const float ax = 1.0f - tx, ay = 1.0f - ty, az = 1.0f - tz;
+                const float y0z0 = ay * az, y1z0 = ty * az, y0z1 = ay * tz, y1z1 = ty * tz;
+                w[0] = y0z0 * ax;
+                w[1] = y0z0 * tx;
+                w[2] = y1z0 * ax;
+                w[3] = y1z0 * tx;
+                w[4] = y0z1 * ax;
+                w[5] = y0z1 * tx;
+                w[6] = y1z1 * ax;
+                w[7] = y1z1 * tx;
+                sj += 1.0f;
+            }
+            for (int k = k0 + 1; k < k1; k++, sj += 1.0f) {
                 const float tx = fmaf(sj, dX, fX), ty = fmaf(sj, dY, fY), tz = fmaf(sj, dZ, fZ);
                 const float ax = 1.0f - tx, ay = 1.0f - ty, az = 1.0f - tz;
                 const float y0z0 = ay * az, y1z0 = ty * az, y0z1 = ay * tz, y1z1 = ty * tz;
@@ -2529,7 +2543,7 @@ __device__ __forceinline__ void cell_pixels(const float *rec0, int rs, int q0, i
 }
 
 template <int M>
-__global__ __launch_bounds__(kGatherBlock) void cell_gather_kernel(GatherArgs A)
+__global__ __launch_bounds__(kCellBlock) void cell_gather_kernel(GatherArgs A)
 {
     extern __shared__ char lds_raw[];
     const MarchArgs &P = A.P;
@@ -2541,7 +2555,7 @@ __global__ __launch_bounds__(kGatherBlock) void cell_gather_kernel(GatherArgs A)
     const int bx = (int)(brick % (uint32_t)A.bricks_x);
     const int by = (int)((brick / (uint32_t)A.bricks_x) % (uint32_t)A.bricks_y);
     const int bz = (int)(brick / ((uint32_t)A.bricks_x * (uint32_t)A.bricks_y));
-    const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
+    const int lx = threadIdx.x % kCellX, ly = (threadIdx.x / kCellX) % kCellY, lz = threadIdx.x / (kCellX * kCellY);
     // the cell's lower corner; cells -1 .. n - 1 of each axis take samples
     const int cx = bx * kCellX - 1 + lx, cy = by * kCellY - 1 + ly, cz = bz * kCellZ - 1 + lz;
     const bool inside = cx < (int)P.nx && cy < (int)P.ny && cz < (int)P.nz;
@@ -2609,7 +2623,7 @@ __global__ __launch_bounds__(kGatherBlock) void cell_gather_kernel(GatherArgs A)
                     const int b1 = min(rv1, b0 + band - 1);
                     const int count = (b1 - b0 + 1) * rw;
                     __syncthreads(); // the previous band's entries are consumed
-                    for (int i = threadIdx.x; i < count; i += kGatherBlock) {
+                    for (int i = threadIdx.x; i < count; i += kCellBlock) {
                         const int pv = b0 + i / rw, pu = c0 + i % rw;
                         const RayLine Ln = ray_line(P, pu, pv, (int)(P.view_begin + vw));
                         const double dxa = Ln.da * P.dx, dxb = Ln.db * P.dx, dxc = Ln.dc * P.dx;
@@ -2656,7 +2670,7 @@ __global__ __launch_bounds__(kGatherBlock) void cell_gather_kernel(GatherArgs A)
     // of one pass add to distinct voxels), then out to `part`
     __syncthreads();
     float *V = (float *)lds_raw; // [M][kCellZ + 1][kCellY + 1][kCellX + 1]
-    for (int i = threadIdx.x; i < M * kCellCorners; i += kGatherBlock) V[i] = 0.0f;
+    for (int i = threadIdx.x; i < M * kCellCorners; i += kCellBlock) V[i] = 0.0f;
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < 8; c++) {
@@ -2666,7 +2680,7 @@ __global__ __launch_bounds__(kGatherBlock) void cell_gather_kernel(GatherArgs A)
         __syncthreads();
     }
     float *dst = A.part + (int64_t)brick * (M * kCellCorners);
-    for (int i = threadIdx.x; i < M * kCellCorners; i += kGatherBlock) dst[i] = V[i];
+    for (int i = threadIdx.x; i < M * kCellCorners; i += kCellBlock) dst[i] = V[i];
 }
 
 // grad[mat_off + m] += the cell gather's corner sums of each voxel: voxel x
@@ -3015,8 +3029,8 @@ int alloc_volume(ctfp_context *ctx, int n_mat)
 // Workgroups of one launch: the grid's work-items (blocks x 256) must stay
 // below 2^32, and the march launches a multiple of 8 blocks.
 constexpr uint64_t kMaxLaunchBlocks = ((1ull << 32) - 1) / kBlock / 8 * 8;
-// the gather's workgroups hold kGatherBlock work-items
-constexpr uint64_t kMaxGatherBricks = ((1ull << 32) - 1) / kGatherBlock;
+// the gathers' workgroups hold kGatherBlock or kCellBlock work-items
+constexpr uint64_t kMaxGatherBricks = ((1ull << 32) - 1) / (kGatherBlock > kCellBlock ? kGatherBlock : kCellBlock);
 
 // Views per march launch so that no launch exceeds kMaxLaunchBlocks (every
 // view has the same tiles).
@@ -3952,7 +3966,7 @@ static int backward_impl(ctfp_context *ctx, int64_t view_begin, int64_t view_cou
             for (int64_t b0 = 0; b0 < bricks; b0 += (int64_t)kMaxGatherBricks) {
                 A.brick_base = (uint32_t)b0;
                 const int64_t nb = std::min<int64_t>((int64_t)kMaxGatherBricks, bricks - b0);
-                hipLaunchKernelGGL(gfn, dim3((unsigned)nb), dim3(kGatherBlock), (size_t)kGatherLds,
+                hipLaunchKernelGGL(gfn, dim3((unsigned)nb), dim3(cell ? kCellBlock : kGatherBlock), (size_t)kGatherLds,
                                    (hipStream_t)stream, A);
                 HIP_TRY(hipGetLastError());
             }
