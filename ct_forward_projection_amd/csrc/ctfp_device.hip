@@ -2219,6 +2219,126 @@ struct BrickView {
 // Views whose brick shadows a workgroup computes at once (one lane each).
 constexpr int kGatherViews = 64;
 
+// project_point in FP32, for bounds only: detector coordinates to ~1e-4
+// pixel, so differences of them resolve curvature to about that.
+__device__ __forceinline__ void project_point_f32(const MarchArgs &P, float cs_x, float cs_y, float fx, float fy,
+                                                  float a, float b, float c, float &u, float &v)
+{
+    const float qx = a * (float)(1.0 / P.inv_px) - (float)P.half_x + (float)P.sod * cs_x;
+    const float qy = b * (float)(1.0 / P.inv_py) - (float)P.half_y - (float)P.sod * cs_y;
+    const float qz = c * (float)(1.0 / P.inv_pz) - (float)P.half_z;
+    float X = qx * cs_x - qy * cs_y, Y = -qx * cs_y - qy * cs_x, Z = -qz;
+    if (P.focal) {
+        const float sdd = (float)P.sdd, lam = sdd / X;
+        Y = fx + lam * (Y - fx);
+        Z = fy + lam * (Z - fy);
+        X = sdd;
+    }
+    const float al = Z / sqrtf(X * X + Y * Y), be = Y / X;
+    const float den = 1.0f - al * al * be * be;
+    const float detV = al * (float)P.sdd * sqrtf((1.0f + be * be) / den);
+    const float detU = be * (float)P.sdd * sqrtf((1.0f + al * al) / den);
+    const float ds = (float)P.dsize;
+    u = ((float)P.half_u - 0.5f * ds - detU) / ds;
+    v = ((float)P.half_v - 0.5f * ds - detV) / ds;
+}
+
+// The shadow of a brick in view `view` for elements (voxel supports or
+// cells) of half-width `s` voxels whose centres lie within (ha, hb, hc) of
+// the brick centre (ca, cb, cc): the centre's projection, the Jacobian per
+// voxel step (central differences, FP64), an element's window half-extent
+// and the brick's pixel rectangle.  Both widen by a bound on how far the
+// projection departs from that linear model over the brick (reach
+// a_i = h_i + s): |e(r)| <= 1/2 sum_ij |H_ij| a_i a_j, the Hessian entries from central differences of projections at the six
+// face centres (H_ii a_i^2) and twelve edge midpoints (4 H_ij a_i a_j from
+// the four sign pairs), times 1.25 for the cubic terms, plus 1e-3 pixel for
+// the FP32 projections.  At magnification 4 with 1.5 mm voxels and 0.6 mm
+// pixels the bound reaches about a pixel; for common scans it is a few
+// hundredths.
+__device__ __forceinline__ BrickView brick_view(const MarchArgs &P, int view, double ca, double cb, double cc,
+                                                double ha, double hb, double hc, double s)
+{
+    double u0, v0, ju[3], jv[3];
+    project_point(P, view, ca, cb, cc, u0, v0);
+#pragma nounroll
+    for (int i = 0; i < 3; i++) {
+        double up, vp, um, vm;
+        project_point(P, view, ca + (i == 0 ? 1.0 : 0.0), cb + (i == 1 ? 1.0 : 0.0), cc + (i == 2 ? 1.0 : 0.0), up, vp);
+        project_point(P, view, ca - (i == 0 ? 1.0 : 0.0), cb - (i == 1 ? 1.0 : 0.0), cc - (i == 2 ? 1.0 : 0.0), um, vm);
+        const double du = 0.5 * (up - um), dv = 0.5 * (vp - vm);
+        ju[0] = i == 0 ? du : ju[0];
+        ju[1] = i == 1 ? du : ju[1];
+        ju[2] = i == 2 ? du : ju[2];
+        jv[0] = i == 0 ? dv : jv[0];
+        jv[1] = i == 1 ? dv : jv[1];
+        jv[2] = i == 2 ? dv : jv[2];
+    }
+    const double a[3] = {ha + s, hb + s, hc + s};
+    const float cx = (float)P.view_cs[view].x, cy = (float)P.view_cs[view].y;
+    const float fx = P.focal ? (float)P.focal[view].x : 0.0f, fy = P.focal ? (float)P.focal[view].y : 0.0f;
+    const float fa = (float)ca, fb = (float)cb, fc = (float)cc;
+    float pu0, pv0;
+    project_point_f32(P, cx, cy, fx, fy, fa, fb, fc, pu0, pv0);
+    const float a0 = (float)a[0], a1 = (float)a[1], a2 = (float)a[2];
+    // one projection at a time (loops kept rolled: the kernels' registers
+    // belong to their sample loops)
+    float bu = 0.0f, bv = 0.0f;
+#pragma nounroll
+    for (int i = 0; i < 3; i++) {
+        const float ai = i == 0 ? a0 : i == 1 ? a1 : a2;
+        const float ox = i == 0 ? ai : 0.0f, oy = i == 1 ? ai : 0.0f, oz = i == 2 ? ai : 0.0f;
+        float pu, pv, mu, mv;
+        project_point_f32(P, cx, cy, fx, fy, fa + ox, fb + oy, fc + oz, pu, pv);
+        project_point_f32(P, cx, cy, fx, fy, fa - ox, fb - oy, fc - oz, mu, mv);
+        // 1/2 |H_ii| a_i^2
+        bu += 0.5f * fabsf(pu + mu - 2.0f * pu0);
+        bv += 0.5f * fabsf(pv + mv - 2.0f * pv0);
+    }
+#pragma nounroll
+    for (int q = 0; q < 3; q++) {
+        // axis pairs (0, 1), (0, 2), (1, 2)
+        const int i = q == 2 ? 1 : 0, j = q == 0 ? 1 : 2;
+        const float ai = i == 0 ? a0 : a1, aj = j == 1 ? a1 : a2;
+        float hu = 0.0f, hv = 0.0f;
+#pragma nounroll
+        for (int k = 0; k < 4; k++) {
+            const float si = (k & 1) ? -ai : ai, sj = (k & 2) ? -aj : aj;
+            float du, dv;
+            project_point_f32(P, cx, cy, fx, fy, fa + (i == 0 ? si : 0.0f), fb + (i == 1 ? si : 0.0f) + (j == 1 ? sj : 0.0f),
+                              fc + (j == 2 ? sj : 0.0f), du, dv);
+            const float sg = ((k & 1) ^ ((k >> 1) & 1)) ? -1.0f : 1.0f;
+            hu += sg * du;
+            hv += sg * dv;
+        }
+        // |H_ij| a_i a_j, a quarter of the four-point sum
+        bu += 0.25f * fabsf(hu);
+        bv += 0.25f * fabsf(hv);
+    }
+    bu = 1.25f * bu + 1e-3f;
+    bv = 1.25f * bv + 1e-3f;
+    BrickView w;
+    w.u0 = (float)u0;
+    w.v0 = (float)v0;
+    w.jua = (float)ju[0];
+    w.jub = (float)ju[1];
+    w.juc = (float)ju[2];
+    w.jva = (float)jv[0];
+    w.jvb = (float)jv[1];
+    w.jvc = (float)jv[2];
+    // an element spans +-s voxels around its centre: +-h pixels under the
+    // linear model, plus the bound and 0.01 pixel for the FP32 centre and
+    // Jacobian the kernels use
+    w.hu = (float)(s * (fabs(ju[0]) + fabs(ju[1]) + fabs(ju[2])) + 0.01 + bu);
+    w.hv = (float)(s * (fabs(jv[0]) + fabs(jv[1]) + fabs(jv[2])) + 0.01 + bv);
+    const double eu = fabs(ju[0]) * a[0] + fabs(ju[1]) * a[1] + fabs(ju[2]) * a[2] + 0.01 + bu;
+    const double ev = fabs(jv[0]) * a[0] + fabs(jv[1]) * a[1] + fabs(jv[2]) * a[2] + 0.01 + bv;
+    w.ru0 = max(0, (int)ceil(u0 - eu));
+    w.ru1 = min(P.nu - 1, (int)floor(u0 + eu));
+    w.rv0 = max(0, (int)ceil(v0 - ev));
+    w.rv1 = min(P.nv - 1, (int)floor(v0 + ev));
+    return w;
+}
+
 // Staged dwords per shadow pixel, one record per pixel: the support bounds
 // at the brick centre in j = k - floor(k*) (on x and y already widened to a
 // quad: the union over offsets 0 and 1) and the inverse step (3 axes each),
@@ -2323,35 +2443,11 @@ __global__ __launch_bounds__(kGatherBlock) void gather_kernel(GatherArgs A)
         const int nvt = min(kGatherViews, P.n_views - vt);
         __syncthreads(); // the previous views' table is consumed
         if ((int)threadIdx.x < nvt) {
-            // the brick's shadow in view vt + lane: centre projection and
-            // the Jacobian per voxel step (one lane per view)
-            const int view = (int)(P.view_begin + vt + threadIdx.x);
-            double u0, v0, ua, va, ub, vb, uc, vc;
-            project_point(P, view, ca, cb, cc, u0, v0);
-            project_point(P, view, ca + 1.0, cb, cc, ua, va);
-            project_point(P, view, ca, cb + 1.0, cc, ub, vb);
-            project_point(P, view, ca, cb, cc + 1.0, uc, vc);
-            BrickView bv;
-            bv.u0 = (float)u0;
-            bv.v0 = (float)v0;
-            bv.jua = (float)(ua - u0);
-            bv.jub = (float)(ub - u0);
-            bv.juc = (float)(uc - u0);
-            bv.jva = (float)(va - v0);
-            bv.jvb = (float)(vb - v0);
-            bv.jvc = (float)(vc - v0);
-            // a voxel's support cube spans +-1 voxel: +-h pixels around its
-            // centre (plus a quarter pixel for the FP32 centre and Jacobian)
-            bv.hu = fabsf(bv.jua) + fabsf(bv.jub) + fabsf(bv.juc) + 0.25f;
-            bv.hv = fabsf(bv.jva) + fabsf(bv.jvb) + fabsf(bv.jvc) + 0.25f;
-            const double eu = fabs(ua - u0) * (0.5 * kBrickX + 0.5) + fabs(ub - u0) * (0.5 * kBrickY + 0.5) +
-                              fabs(uc - u0) * (0.5 * kBrickZ + 0.5) + 0.5;
-            const double ev = fabs(va - v0) * (0.5 * kBrickX + 0.5) + fabs(vb - v0) * (0.5 * kBrickY + 0.5) +
-                              fabs(vc - v0) * (0.5 * kBrickZ + 0.5) + 0.5;
-            bv.ru0 = max(0, (int)ceil(u0 - eu));
-            bv.ru1 = min(P.nu - 1, (int)floor(u0 + eu));
-            bv.rv0 = max(0, (int)ceil(v0 - ev));
-            bv.rv1 = min(P.nv - 1, (int)floor(v0 + ev));
+            // the brick's shadow in view vt + lane (one lane per view):
+            // voxel centres within (7.5, 7.5, 1.5) of the brick centre,
+            // supports +-1 voxel
+            const BrickView bv = brick_view(P, (int)(P.view_begin + vt + threadIdx.x), ca, cb, cc,
+                                            0.5 * (kBrickX - 1), 0.5 * (kBrickY - 1), 0.5 * (kBrickZ - 1), 1.0);
             s_view[threadIdx.x] = bv;
         }
         __syncthreads();
@@ -2543,7 +2639,7 @@ __device__ __forceinline__ void cell_pixels(const float *rec0, int rs, int q0, i
 }
 
 template <int M>
-__global__ __launch_bounds__(kCellBlock) void cell_gather_kernel(GatherArgs A)
+__global__ __launch_bounds__(kCellBlock) __attribute__((amdgpu_waves_per_eu(4))) void cell_gather_kernel(GatherArgs A)
 {
     extern __shared__ char lds_raw[];
     const MarchArgs &P = A.P;
@@ -2576,33 +2672,10 @@ __global__ __launch_bounds__(kCellBlock) void cell_gather_kernel(GatherArgs A)
         const int nvt = min(kGatherViews, P.n_views - vt);
         __syncthreads(); // the previous views' table is consumed
         if ((int)threadIdx.x < nvt) {
-            const int view = (int)(P.view_begin + vt + threadIdx.x);
-            double u0, v0, ua, va, ub, vb, uc, vc;
-            project_point(P, view, ca, cb, cc, u0, v0);
-            project_point(P, view, ca + 1.0, cb, cc, ua, va);
-            project_point(P, view, ca, cb + 1.0, cc, ub, vb);
-            project_point(P, view, ca, cb, cc + 1.0, uc, vc);
-            BrickView bv;
-            bv.u0 = (float)u0;
-            bv.v0 = (float)v0;
-            bv.jua = (float)(ua - u0);
-            bv.jub = (float)(ub - u0);
-            bv.juc = (float)(uc - u0);
-            bv.jva = (float)(va - v0);
-            bv.jvb = (float)(vb - v0);
-            bv.jvc = (float)(vc - v0);
-            // a cell spans +-0.5 voxel around its centre: +-h pixels (plus a
-            // quarter pixel for the FP32 centre and Jacobian)
-            bv.hu = 0.5f * (fabsf(bv.jua) + fabsf(bv.jub) + fabsf(bv.juc)) + 0.25f;
-            bv.hv = 0.5f * (fabsf(bv.jva) + fabsf(bv.jvb) + fabsf(bv.jvc)) + 0.25f;
-            const double eu = fabs(ua - u0) * (0.5 * kCellX) + fabs(ub - u0) * (0.5 * kCellY) +
-                              fabs(uc - u0) * (0.5 * kCellZ) + 0.5;
-            const double ev = fabs(va - v0) * (0.5 * kCellX) + fabs(vb - v0) * (0.5 * kCellY) +
-                              fabs(vc - v0) * (0.5 * kCellZ) + 0.5;
-            bv.ru0 = max(0, (int)ceil(u0 - eu));
-            bv.ru1 = min(P.nu - 1, (int)floor(u0 + eu));
-            bv.rv0 = max(0, (int)ceil(v0 - ev));
-            bv.rv1 = min(P.nv - 1, (int)floor(v0 + ev));
+            // cell centres within (3.5, 3.5, 1.5) of the brick centre, cells
+            // +-0.5 voxel
+            const BrickView bv = brick_view(P, (int)(P.view_begin + vt + threadIdx.x), ca, cb, cc,
+                                            0.5 * (kCellX - 1), 0.5 * (kCellY - 1), 0.5 * (kCellZ - 1), 0.5);
             s_view[threadIdx.x] = bv;
         }
         __syncthreads();
