@@ -2554,7 +2554,7 @@ __global__ __launch_bounds__(kGatherBlock) void gather_kernel(GatherArgs A)
 // rounding (the weights come from the sample's offset in the brick's frame).
 // Bricks start at cell -1 (cx = bx * 8 - 1 + lx), so every
 // cell [-1, n - 1] of each axis is someone's.  A brick's corner sums
-// (9 x 9 x 5 voxels, from cell_corner_sums) go to `part`, and
+// (9 x 9 x 5 voxels, summed in LDS at the end) go to `part`, and
 // cell_combine_kernel adds the (up to eight) bricks' sums of each voxel to
 // grad in a fixed order: no atomics, a deterministic result.
 constexpr int kCellX = 8, kCellY = 8, kCellZ = 4;
